@@ -1,0 +1,179 @@
+/*
+ * spk.h — C ABI of the MI355X spatial-interpolation kernels (libspk.so).
+ *
+ * Plain pointers and sizes, row-major C-contiguous arrays, int status codes.
+ * Functions named after a reference cpdef take HOST pointers and keep the
+ * reference's argument order and in-place semantics; the library copies to
+ * the GPU, computes there and copies back on the context's stream.  The
+ * spk_*_dev functions take DEVICE pointers and run asynchronously on the
+ * context's stream.
+ *
+ * References are to faizan90/spinterps:  "pyx" = cyth/interpmthds.pyx.
+ */
+#ifndef SPK_H
+#define SPK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define SPK_OK          0
+#define SPK_EBADMODEL   1   /* unknown or malformed variogram string      */
+#define SPK_ESHAPE      2   /* inconsistent sizes                         */
+#define SPK_EHIP        3   /* HIP runtime error                          */
+#define SPK_ENOTSPD     4   /* kriging system not positive definite       */
+#define SPK_EARG        5   /* invalid argument (assert in the reference) */
+#define SPK_ENOMEM      6
+#define SPK_EINDEX      7   /* index out of range                         */
+
+/* Text of the last error on the calling thread. */
+const char *spk_last_error(void);
+int spk_version(void);
+
+/* ---- context: device, stream, workspaces ---------------------------- */
+typedef struct spk_ctx spk_ctx;
+int spk_ctx_create(int device, spk_ctx **out);
+int spk_ctx_destroy(spk_ctx *ctx);
+/* Use an existing hipStream_t (NULL = the null stream). */
+int spk_ctx_set_stream(spk_ctx *ctx, void *hip_stream);
+int spk_ctx_sync(spk_ctx *ctx);
+/* Release cached device workspaces. */
+int spk_ctx_trim(spk_ctx *ctx);
+
+/* ---- variogram programs (pyx:86-95, pyx:174-184) -------------------- */
+#define SPK_VG_MAX 8
+enum { SPK_VG_RNG = 0, SPK_VG_NUG, SPK_VG_SPH, SPK_VG_EXP, SPK_VG_LIN,
+       SPK_VG_GAU, SPK_VG_POW, SPK_VG_HOL };
+typedef struct spk_vg {
+  int32_t n;                    /* number of components                  */
+  int32_t type[SPK_VG_MAX];     /* SPK_VG_*                              */
+  double sill[SPK_VG_MAX];
+  double range[SPK_VG_MAX];     /* clamped to >= 1e-5 (pyx:183)          */
+  double sill_sum;              /* sum of sills (misc.py:1093)           */
+  double range_max_raw;         /* max unclamped range (misc.py:1094)    */
+} spk_vg;
+
+/* Parse one variogram string as fill_vg_var_arr does (pyx:174-184). */
+int spk_vg_parse(const char *vg_str, spk_vg *out);
+/* misc.py:1074-1105 check_full_nuggetness. */
+int spk_vg_full_nuggetness(const char *vg_str, double min_vg_val, int *out);
+
+/* ---- host-pointer mirrors of the cyth API (cyth/__init__.py:9-21) --- */
+/* pyx:123-143 */
+int spk_fill_dists_2d_mat(spk_ctx *ctx, const double *x1s, const double *y1s,
+                          int64_t n1, const double *x2s, const double *y2s,
+                          int64_t n2, double *dists);
+/* pyx:146-226 */
+int spk_fill_vg_var_arr(spk_ctx *ctx, const double *dists, int64_t rows,
+                        int64_t cols, double *in_vars, int covar_flag,
+                        int diag_mat_flag, const char *vg_models_str,
+                        double min_vg_val);
+/* pyx:229-248; subset_arr is [sub_rows x sub_cols], only the leading
+ * [n_rows x n_cols] block is written. */
+int spk_copy_2d_arr_at_idxs(spk_ctx *ctx, const double *arr, int64_t arr_rows,
+                            int64_t arr_cols, const int64_t *row_idxs,
+                            int64_t n_rows, const int64_t *col_idxs,
+                            int64_t n_cols, double *subset_arr,
+                            int64_t sub_rows, int64_t sub_cols);
+/* pyx:768-781 */
+int spk_fill_dists_one_pt(spk_ctx *ctx, double x, double y, const double *xs,
+                          const double *ys, int64_t n, double *dists);
+/* pyx:784-795; *wts_sum receives the return value of the cpdef. */
+int spk_fill_wts_and_sum(spk_ctx *ctx, const double *dists, double *wts,
+                         int64_t n, double idw_exp, double *wts_sum);
+/* pyx:798-808 */
+int spk_get_mults_sum(spk_ctx *ctx, const double *wts, const double *data,
+                      int64_t n, double *out);
+/* pyx:98-120: vg_arr += model(h, r, s), no range clamp. */
+int spk_fill_theo_vg_vals(spk_ctx *ctx, const char *vg_name,
+                          const double *h_arr, int64_t n, double r, double s,
+                          double *vg_arr);
+/* pyx:893-923: out[n(n-1)/2], condensed, i > j order. */
+int spk_get_nd_dists(spk_ctx *ctx, const double *pts, int64_t n_pts,
+                     int64_t n_dims, double *out);
+/* pyx:811-890 for one destination point. */
+int spk_sel_equidist_refs(spk_ctx *ctx, double dst_x, double dst_y,
+                          const double *ref_xs, const double *ref_ys,
+                          int64_t n_refs, uint64_t n_pies,
+                          double min_dist_thresh, int64_t not_neb_flag,
+                          double *dists, double *tem_ref_sel_dists,
+                          int64_t *ref_sel_pie_idxs, uint64_t *ref_pie_idxs,
+                          uint64_t *ref_pie_cts);
+
+/* ---- neighbour selection for all cells (interp/grps.py:141-247) ----- */
+/* nrst: out[n_dst x min(n_nebs, n_refs)] sorted indices of the nearest. */
+int spk_neb_idxs_nrst(spk_ctx *ctx, const double *dst_xs, const double *dst_ys,
+                      int64_t n_dst, const double *ref_xs,
+                      const double *ref_ys, int64_t n_refs, int64_t n_nebs,
+                      int64_t *out);
+/* pie: out[n_dst x n_nebs] (interp/grps.py:168-247, min_dist_thresh -1). */
+int spk_neb_idxs_pie(spk_ctx *ctx, const double *dst_xs, const double *dst_ys,
+                     int64_t n_dst, const double *ref_xs, const double *ref_ys,
+                     int64_t n_refs, int64_t n_nebs, int64_t n_pies,
+                     int64_t *out);
+
+/* ---- batched step interpolation (interp/steps.py:245-435) ----------- */
+enum { SPK_M_OK = 0, SPK_M_SK = 1, SPK_M_EDK = 2, SPK_M_IDW = 3,
+       SPK_M_NNB = 4 };
+/* SPK_P_FP64 : every step on its own, FP64 end to end, the reference's
+ *              per-cell lambda = K^-1 rhs as one FP64 MFMA GEMM per step.
+ * SPK_P_MIXED: all steps of the call in batched launches; estimates in the
+ *              dual form (FP64), variance by an FP32 MFMA contraction with
+ *              FP64 Cholesky/Schur terms.
+ * SPK_P_FP64B: the batched dual-form path with an FP64 contraction. */
+enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
+
+/* Per-step status written to info[t] (bit flags). */
+#define SPK_ST_NOSTNS    1   /* no station: outputs untouched (NaN)        */
+#define SPK_ST_MEANFILL  2   /* interp flag off or nugget-only: mean fill  */
+#define SPK_ST_FAILED    4   /* factorisation failed: nearest-station fill */
+#define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
+
+typedef struct spk_problem {
+  int64_t n_stns;               /* N                                      */
+  int64_t n_cells;              /* C                                      */
+  int32_t n_drifts;             /* m (EDK)                                */
+  int32_t pad_;
+  const double *stn_x, *stn_y;  /* device [N], metres                     */
+  const double *cell_x, *cell_y;/* device [C], metres                     */
+  const double *stn_drift;      /* device [N x m] or NULL                 */
+  const double *cell_drift;     /* device [m x C] or NULL                 */
+  double origin_x, origin_y;    /* subtracted before any FP32 arithmetic  */
+} spk_problem;
+
+typedef struct spk_opts {
+  int32_t method;               /* SPK_M_*                                */
+  int32_t precision;            /* SPK_P_*                                */
+  int32_t want_var;             /* OK variance                            */
+  int32_t pad_;
+  double idw_exp;
+  double min_vg_val;            /* interp/data.py default 0.0             */
+  double min_var_thr;           /* interp_steps_flags (steps.py:760-765)  */
+} spk_opts;
+
+/* Interpolate n_steps steps.  vals: device [n_steps x N], NaN = station
+ * missing at that step.  vgs: device [n_steps] (kriging only).  est, var:
+ * device [n_steps x C] float64 (var may be NULL).  info: device [n_steps].
+ * Outputs of steps without stations are left untouched. */
+int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
+                         const spk_opts *opts, int64_t n_steps,
+                         const double *vals, const spk_vg *vgs, double *est,
+                         double *var, int32_t *info);
+
+/* interp/steps.py:466-476 + the float32 cast of steps.py:815-831:
+ * out32[i] = (float)clip(in64[i]) on device; NaN passes through.
+ * has_min/has_max select the cuts (None in the reference = 0). */
+int spk_clip_cast_dev(spk_ctx *ctx, const double *in64, float *out32,
+                      int64_t n, int has_min, double min_cut, int has_max,
+                      double max_cut);
+
+/* Device timing of the last spk_interp_steps_dev call, per stage (ms). */
+int spk_last_timings(spk_ctx *ctx, double *ms, int max_n, int *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPK_H */

@@ -1,0 +1,704 @@
+"""CPU oracle (test infrastructure only) for the spinterps interpolation path.
+
+A float64 NumPy restatement of the reference's numeric kernels
+(``cyth/interpmthds.pyx``) and of the per-step driver semantics
+(``interp/steps.py``, ``interp/grps.py``, ``interp/vgclus.py``,
+``misc.py:1027-1105``).  Every function cites the reference lines it follows.
+
+This module is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it; the product path
+(``spinterps_amd``) never does.
+
+Pinning status (how each function is checked, see tests/test_oracle.py):
+  vg models / parse / fill_vg_var_arr / theo vg : closed forms + the
+      ``test/test_theo_vg.py:26-28`` string as a known-answer case.
+  distances / gathers / IDW helpers              : brute-force scalar loops.
+  kriging systems (pinv)                         : invariants (sum of weights,
+      drift reproduction, nugget-shift invariance, exact interpolation,
+      n=2 closed form, scipy solve).
+  sel_equidist_refs / nrst neighbours            : brute force + rules read
+      from pyx:846-889 and interp/grps.py:147-247.  Degenerate geometry
+      (stations on axes, equal distances) is parity unpinned.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# Variogram models, cyth/interpmthds.pyx:38-95
+# ---------------------------------------------------------------------------
+
+
+def rng_vg(h, r, s):
+    """pyx:38-39: returns the lag itself."""
+    return np.asarray(h, dtype=np.float64).copy()
+
+
+def nug_vg(h, r, s):
+    """pyx:42-43: the sill for every lag, h == 0 included."""
+    return np.full(np.shape(h), s, dtype=np.float64)
+
+
+def sph_vg(h, r, s):
+    """pyx:46-55: s for h >= r, else s*((1.5h)/r - h^3/(2 r^3))."""
+    h = np.asarray(h, dtype=np.float64)
+    a = (1.5 * h) / r
+    b = (h * h * h) / (2 * (r * r * r))
+    out = s * (a - b)
+    return np.where(h >= r, s, out)
+
+
+def exp_vg(h, r, s):
+    """pyx:58-59: s*(1 - exp(-3h/r))."""
+    h = np.asarray(h, dtype=np.float64)
+    return s * (1 - np.exp((-3 * h) / r))
+
+
+def lin_vg(h, r, s):
+    """pyx:62-66: s for h > r, else s*(h/r)."""
+    h = np.asarray(h, dtype=np.float64)
+    return np.where(h > r, s, s * (h / r))
+
+
+def gau_vg(h, r, s):
+    """pyx:69-70: s*(1 - exp(-3 h^2/r^2))."""
+    h = np.asarray(h, dtype=np.float64)
+    return s * (1 - np.exp(-3 * ((h * h) / (r * r))))
+
+
+def pow_vg(h, r, s):
+    """pyx:73-74: s*h^r (C pow, so 0^0 == 1)."""
+    h = np.asarray(h, dtype=np.float64)
+    return s * np.power(h, r)
+
+
+def hol_vg(h, r, s):
+    """pyx:77-83: 0 at h == 0, else s*(1 - sin(a)/a) with a = pi*h/r."""
+    h = np.asarray(h, dtype=np.float64)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        a = (math.pi * h) / r
+        out = s * (1 - (np.sin(a) / a))
+    return np.where(h == 0, 0.0, out)
+
+
+VG_FTNS = {
+    'Rng': rng_vg, 'Nug': nug_vg, 'Sph': sph_vg, 'Exp': exp_vg,
+    'Lin': lin_vg, 'Gau': gau_vg, 'Pow': pow_vg, 'Hol': hol_vg,
+}
+
+
+def parse_vg(vg_str: str):
+    """Parse a variogram string the way fill_vg_var_arr does (pyx:174-184).
+
+    Components are split on '+', stripped, split on exactly one space into
+    (sill, rest), then rest.split('(') into (name, range) and the range is
+    cut at ')'.  The range is clamped to max(1e-5, range) (pyx:183).
+    Returns a list of (name, sill, range).  Unknown names raise KeyError
+    (the reference dereferences a null function pointer there).
+    """
+    comps = []
+    for comp in vg_str.split('+'):
+        comp = comp.strip()
+        sill_s, vg_s = comp.split(' ')
+        name, range_s = vg_s.split('(')
+        range_s = range_s.split(')')[0]
+        if name not in VG_FTNS:
+            raise KeyError(name)
+        comps.append((name, float(sill_s), max(1e-5, float(range_s))))
+    return comps
+
+
+def vg_sum(d, vg_str, covar_flag=0, min_vg_val=None):
+    """Sum of components as fill_vg_var_arr forms it (pyx:162-216)."""
+    d = np.asarray(d, dtype=np.float64)
+    out = np.zeros(d.shape, dtype=np.float64)
+    cov_mult = 1.0 if covar_flag else 0.0
+    cov_sign = -1.0 if covar_flag else 1.0
+    for name, sill, rng in parse_vg(vg_str):
+        out += (cov_mult * sill) + (cov_sign * VG_FTNS[name](d, rng, sill))
+    if min_vg_val is not None:
+        out[out <= min_vg_val] = 0.0
+    return out
+
+
+def fill_vg_var_arr(dists, in_vars, covar_flag, diag_mat_flag, vg_str,
+                    min_vg_val):
+    """pyx:146-226.  With diag_mat_flag only g >= h is computed, then the
+    lower triangle is mirrored from the upper one."""
+    dists = np.asarray(dists, dtype=np.float64)
+    assert in_vars.shape == dists.shape
+    vals = vg_sum(dists, vg_str, covar_flag, min_vg_val)
+    if diag_mat_flag:
+        upper = np.triu(np.ones(dists.shape, dtype=bool))
+        vals = np.where(upper, vals, vals.T)
+    in_vars[...] = vals
+
+
+def fill_dists_2d_mat(x1s, y1s, x2s, y2s, dists):
+    """pyx:123-143: dists[i, j] = sqrt((x1i-x2j)^2 + (y1i-y2j)^2)."""
+    dx = x1s[:, None] - x2s[None, :]
+    dy = y1s[:, None] - y2s[None, :]
+    dists[...] = ((dx * dx) + (dy * dy)) ** 0.5
+
+
+def dists_2d(x1s, y1s, x2s, y2s):
+    out = np.empty((x1s.size, x2s.size), dtype=np.float64)
+    fill_dists_2d_mat(x1s, y1s, x2s, y2s, out)
+    return out
+
+
+def copy_2d_arr_at_idxs(arr, row_idxs, col_idxs, subset_arr):
+    """pyx:229-248: subset[i, j] = arr[row_idxs[i], col_idxs[j]] on the
+    leading (len(row_idxs), len(col_idxs)) block; the rest is untouched."""
+    subset_arr[:row_idxs.size, :col_idxs.size] = arr[np.ix_(row_idxs,
+                                                            col_idxs)]
+
+
+def fill_dists_one_pt(x, y, xs, ys, dists):
+    """pyx:768-781."""
+    dists[...] = (((x - xs) ** 2) + ((y - ys) ** 2)) ** 0.5
+
+
+def fill_wts_and_sum(dists, wts, idw_exp):
+    """pyx:784-795: wts = 1/d^p; returns the running sum (left to right)."""
+    with np.errstate(divide='ignore'):
+        wts[...] = 1.0 / (dists ** idw_exp)
+    s = 0.0
+    for w in wts:
+        s += w
+    return s
+
+
+def get_mults_sum(wts, data):
+    """pyx:798-808: left-to-right sum of wts*data."""
+    s = 0.0
+    for w, v in zip(wts, data):
+        s += w * v
+    return s
+
+
+def fill_theo_vg_vals(vg_str, h_arr, r, s, vg_arr):
+    """pyx:98-120: vg_arr += model(h, r, s); no range clamp; asserts."""
+    name = vg_str.strip()
+    ftn = VG_FTNS[name]
+    assert h_arr.shape[0]
+    assert h_arr.shape[0] == vg_arr.shape[0]
+    assert s >= 0
+    assert r >= 0
+    vg_arr += ftn(h_arr, r, s)
+
+
+def get_theo_vg_vals(in_model, h_arr):
+    """misc.py:1027-1047."""
+    vg_vals = np.zeros_like(h_arr, dtype=np.float64)
+    for sub in str(in_model).split('+'):
+        sub = sub.strip()
+        sill, sub = sub.split(' ')
+        sub, rng = sub.split('(')
+        rng = rng.split(')')[0]
+        fill_theo_vg_vals(sub, h_arr, float(rng), float(sill), vg_vals)
+    return vg_vals
+
+
+def check_full_nuggetness(in_model, min_vg_val):
+    """misc.py:1074-1105.  Note: the single-'Nug' test compares the whole
+    component text with 'Nug' and never fires; a nugget-only string is
+    caught by its zero range instead."""
+    in_model = str(in_model)
+    if in_model == 'nan':
+        return False
+    models = in_model.split('+')
+    sill = 0.0
+    rng = 0.0
+    for sub in models:
+        sub = sub.strip()
+        sill += float(sub.split('(')[0].strip()[:-3].strip())
+        rng = max(rng, float(sub.split('(')[1].split(')')[0]))
+    nug = bool(np.any(np.array([sill, rng]) <= min_vg_val))
+    if (len(models) == 1) and models[0] == 'Nug':
+        nug = True
+    return nug
+
+
+def get_nd_dists(pts):
+    """pyx:893-923: condensed distances for i > j, row-major over i."""
+    pts = np.asarray(pts, dtype=np.float64)
+    n = pts.shape[0]
+    out = []
+    for i in range(n):
+        for j in range(i):
+            out.append(float(np.sqrt(((pts[i] - pts[j]) ** 2).sum())))
+    return np.array(out, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# Neighbour selection, cyth/interpmthds.pyx:811-890 and interp/grps.py
+# ---------------------------------------------------------------------------
+
+
+def sel_equidist_refs(dst_x, dst_y, ref_xs, ref_ys, n_pies, min_dist_thresh,
+                      not_neb_flag, dists, tem_ref_sel_dists,
+                      ref_sel_pie_idxs, ref_pie_idxs, ref_pie_cts):
+    """pyx:811-890.  Each station gets its distance rank within its angular
+    sector.  The angle is atan(dy/dx) with the quadrant fixes of pyx:853-866
+    (dx == 0 gives angle 0).  Raises IndexError where the reference would
+    write past ref_pie_cts (angle rounding to exactly 2*pi)."""
+    n_refs = ref_xs.size
+    two_pi = 2 * math.pi
+    ref_sel_pie_idxs[:] = not_neb_flag
+    fill_dists_one_pt(dst_x, dst_y, ref_xs, ref_ys, dists)
+    min_dist = np.inf
+    min_idx = -1
+    for i in range(n_refs):
+        if (dists[i] <= min_dist_thresh) and (dists[i] < min_dist):
+            min_idx = i
+            min_dist = dists[i]
+    if min_dist < np.inf:
+        ref_sel_pie_idxs[min_idx] = 0
+        return
+    ref_pie_cts[:n_pies] = 0
+    for j in range(n_refs):
+        xd = ref_xs[j] - dst_x
+        yd = ref_ys[j] - dst_y
+        if not xd:
+            ang = 0.0
+        else:
+            ang = math.atan(yd / xd)
+            if (xd < 0) and (yd > 0):
+                ang = math.pi + ang
+            elif (xd < 0) and (yd < 0):
+                ang = math.pi + ang
+            elif (xd > 0) and (yd < 0):
+                ang = two_pi + ang
+        pie = int(ang * n_pies / two_pi)
+        if pie >= n_pies:
+            raise IndexError('pie index out of range')
+        ref_pie_idxs[j] = pie
+        ref_pie_cts[pie] += 1
+    for j in range(n_pies):
+        if not ref_pie_cts[j]:
+            continue
+        tem_ref_sel_dists[:] = np.where(ref_pie_idxs[:n_refs] == j,
+                                        dists[:n_refs], np.inf)
+        order = np.argsort(tem_ref_sel_dists, kind='stable')
+        for i in range(n_refs):
+            if tem_ref_sel_dists[order[i]] == np.inf:
+                break
+            ref_sel_pie_idxs[order[i]] = i
+
+
+def neb_idxs_all(n_dst, n_refs):
+    """interp/grps.py:141-145."""
+    return np.tile(np.arange(n_refs), (n_dst, 1))
+
+
+def neb_idxs_nrst(dst_xs, dst_ys, ref_xs, ref_ys, n_nebs):
+    """interp/grps.py:147-166: sorted indices of the n_nebs nearest.
+    Ties (measure zero for continuous coordinates) use a stable order."""
+    k = min(n_nebs, ref_xs.size)
+    out = np.empty((dst_xs.size, k), dtype=np.int64)
+    for i in range(dst_xs.size):
+        d = (((dst_xs[i] - ref_xs) ** 2) + ((dst_ys[i] - ref_ys) ** 2)) ** 0.5
+        out[i] = np.sort(np.argsort(d, kind='stable')[:n_nebs])
+    return out
+
+
+def neb_idxs_pie(dst_xs, dst_ys, ref_xs, ref_ys, n_nebs, n_pies):
+    """interp/grps.py:168-247 (min_dist_thresh = -1)."""
+    n_refs = ref_xs.size
+    dists = np.zeros(n_refs)
+    pie_idxs = np.zeros(n_refs, dtype=np.uint64)
+    tem = np.zeros(n_refs)
+    sel = np.zeros(n_refs, dtype=np.int64)
+    cts = np.zeros(n_pies, dtype=np.uint64)
+    out = np.full((dst_xs.size, n_nebs), -1, dtype=np.int64)
+    for i in range(dst_xs.size):
+        sel_equidist_refs(dst_xs[i], dst_ys[i], ref_xs, ref_ys, n_pies, -1.0,
+                          -1, dists, tem, sel, pie_idxs, cts)
+        assert np.all(sel != -1)
+        full = []
+        for rank in np.unique(sel):
+            same = np.where(sel == rank)[0]
+            full.extend(same[np.argsort(dists[same], kind='stable')].tolist())
+        full = np.array(full)
+        assert full.size == n_refs
+        out[i, :] = np.sort(full[:n_nebs])
+    return out
+
+
+def neb_groups(all_neb_idxs):
+    """interp/grps.py:103-139: cells with identical index rows, in order of
+    first appearance."""
+    groups = {}
+    order = []
+    for i in range(all_neb_idxs.shape[0]):
+        key = all_neb_idxs[i].tobytes()
+        if key not in groups:
+            groups[key] = []
+            order.append(key)
+        groups[key].append(i)
+    return [np.array(groups[k], dtype=np.int64) for k in order]
+
+
+def time_groups(data):
+    """interp/grps.py:57-101 on a [T x N] array with NaN for missing: steps
+    with the same available-station set, in order of first appearance.
+    Returns a list of (station_index_array, step_bool_mask)."""
+    avail = ~np.isnan(data)
+    keys = {}
+    order = []
+    for t in range(data.shape[0]):
+        key = np.where(avail[t])[0].tobytes()
+        if key not in keys:
+            keys[key] = np.zeros(data.shape[0], dtype=bool)
+            order.append((key, np.where(avail[t])[0]))
+        keys[key][t] = True
+    return [(stns, keys[key]) for key, stns in order]
+
+
+# ---------------------------------------------------------------------------
+# Kriging systems and the step driver, interp/steps.py:170-435
+# ---------------------------------------------------------------------------
+
+
+def krig_system(method, vg_str, ref_ref_dists, dst_ref_dists, ref_drfts,
+                dst_drfts, min_vg_val):
+    """interp/steps.py:170-243 for one step group: returns (K, R) where K is
+    the (n+add) square system and R the [C x (n+add)] right-hand sides.
+
+    OK : K = [[G, 1], [1^T, 0]], R = [g | 1]          (steps.py:212-216)
+    SK : K = Cov, R = cov                            (steps.py:218-219)
+    EDK: K = [[G, 1, S], [1^T, 0, 0], [S^T, 0, 0]], R = [g | 1 | s]
+                                                     (steps.py:221-234)
+    G and g are semivariances (covar_flag 0), Cov and cov covariances
+    (covar_flag 1), both from fill_vg_var_arr (pyx:146-226).
+    """
+    n = ref_ref_dists.shape[0]
+    ncell = dst_ref_dists.shape[0]
+    if method == 'OK':
+        add, cflag = 1, 0
+    elif method == 'SK':
+        add, cflag = 0, 1
+    elif method == 'EDK':
+        add, cflag = 1 + ref_drfts.shape[1], 0
+    else:
+        raise NotImplementedError(method)
+    K = np.full((n + add, n + add), np.nan)
+    R = np.full((ncell, n + add), np.nan)
+    tmp = np.empty((n, n))
+    fill_vg_var_arr(ref_ref_dists, tmp, cflag, 1, vg_str, min_vg_val)
+    K[:n, :n] = tmp
+    tmp2 = np.empty((ncell, n))
+    fill_vg_var_arr(dst_ref_dists, tmp2, cflag, 0, vg_str, min_vg_val)
+    R[:, :n] = tmp2
+    if method == 'OK':
+        K[n, :n] = 1.0
+        K[:, n] = 1.0
+        K[n, n] = 0.0
+        R[:, n] = 1.0
+    elif method == 'EDK':
+        K[n, :n] = 1.0
+        K[:n, n] = 1.0
+        R[:, n] = 1.0
+        K[n:, n:] = 0.0
+        for k in range(ref_drfts.shape[1]):
+            K[n + 1 + k, :n] = ref_drfts[:, k]
+            K[:n, n + 1 + k] = ref_drfts[:, k]
+            R[:, n + 1 + k] = dst_drfts[k, :]
+    return K, R
+
+
+def krige_contract(Kinv, R, z, dst_ref_dists, want_var):
+    """interp/steps.py:403-435 for one step: per cell lmds = Kinv @ rhs; if
+    not isclose(sum(lmds[:n]), 1) the cell takes the nearest station's
+    value (first index on ties) and variance 0; else estimate = lmds[:n].z
+    and variance = sum(lmds * rhs) + lmds[n].  Vectorised over cells with
+    one GEMM; the per-cell arithmetic is the same."""
+    n = z.size
+    L = R @ Kinv.T                                   # [C x (n+add)]
+    lsum = L[:, :n].sum(axis=1)
+    ok = np.isclose(lsum, 1.0)
+    nnb = np.argmin(dst_ref_dists, axis=1)
+    est = np.where(ok, (L[:, :n] * z[None, :]).sum(axis=1), z[nnb])
+    var = None
+    if want_var:
+        v = (L * R).sum(axis=1) + L[:, n]
+        var = np.where(ok, v, 0.0)
+    return est, var, ok
+
+
+def get_interp(ref_data, interp_type, ref_drfts, dst_drfts, idw_exp, models,
+               interp_steps_flags, nuggetness_flags, dst_ref_dists,
+               ref_ref_dists, min_vg_val, want_var, prblm=None,
+               sub_steps=None):
+    """interp/steps.py:245-401 for one (time group, cell group).
+
+    ref_data [T x n] (no NaN), dst_ref_dists [C x n], ref_ref_dists [n x n].
+    Returns (dst_data [T x C] float64, est_vars [T x C] float64 or None).
+    The reference stores est_vars as float32 (steps.py:269-271); callers
+    cast.  IDW normalises each cell's distances by their maximum in place
+    (steps.py:297-301); that changes no argmin here, so the copy is kept.
+    """
+    n_dsts = dst_ref_dists.shape[0]
+    n_refs = ref_data.shape[1]
+    n_time = ref_data.shape[0]
+    dst_data = np.full((n_time, n_dsts), np.nan)
+    est_vars = None
+    if want_var and interp_type == 'OK':
+        est_vars = np.full((n_time, n_dsts), np.nan)
+    ref_means = ref_data.mean(axis=1)
+    if n_refs == 1:
+        for j in range(n_time):
+            dst_data[j, :] = ref_data[j, :]
+    elif interp_type == 'NNB':
+        nnb = np.argmin(dst_ref_dists, axis=1)
+        dst_data[:, :] = ref_data[:, nnb]
+    elif interp_type == 'IDW':
+        d = dst_ref_dists.copy()
+        dmax = d.max(axis=1)
+        pos = dmax > 0
+        d[pos] /= dmax[pos, None]
+        with np.errstate(divide='ignore', invalid='ignore'):
+            w = 1.0 / (d ** idw_exp)
+            wsum = w.sum(axis=1)
+            assert np.all(wsum >= 1e-14)
+            num = ref_data @ w.T
+            vals = num / wsum[None, :]
+        for j in range(n_time):
+            dst_data[j, :] = vals[j] if interp_steps_flags[j] else ref_means[j]
+    elif interp_type in ('OK', 'SK', 'EDK'):
+        order = np.argsort(np.asarray(models, dtype=object).astype(str),
+                           kind='stable')
+        old_model = ''
+        failed = False
+        Kinv = None
+        for j in order:
+            model = models[j]
+            if (not interp_steps_flags[j]) or nuggetness_flags[j]:
+                dst_data[j, :] = ref_means[j]
+                if est_vars is not None:
+                    est_vars[j, :] = 0.0
+                continue
+            if model == 'nan':
+                raise ValueError('NaN VG!')
+            if model != old_model:
+                K, R = krig_system(interp_type, model, ref_ref_dists,
+                                   dst_ref_dists, ref_drfts, dst_drfts,
+                                   min_vg_val)
+                old_model = model
+                try:
+                    Kinv = np.linalg.pinv(K)
+                    failed = False
+                except Exception:
+                    failed = True
+                    if prblm is not None and sub_steps is not None:
+                        if sub_steps[j] not in prblm:
+                            prblm.append(sub_steps[j])
+            if failed:
+                nnb = np.argmin(dst_ref_dists, axis=1)
+                dst_data[j, :] = ref_data[j, nnb]
+                if est_vars is not None:
+                    est_vars[j, :] = 0.0
+            else:
+                est, var, _ = krige_contract(Kinv, R, ref_data[j],
+                                             dst_ref_dists,
+                                             est_vars is not None)
+                dst_data[j, :] = est
+                if est_vars is not None:
+                    est_vars[j, :] = var
+    else:
+        raise NotImplementedError(interp_type)
+    return dst_data, est_vars
+
+
+def mod_min_max(fld, min_var_cut, max_var_cut):
+    """interp/steps.py:466-476 (clip estimates, NaN untouched)."""
+    with np.errstate(invalid='ignore'):
+        if min_var_cut is not None:
+            fld[fld < min_var_cut] = min_var_cut
+        if max_var_cut is not None:
+            fld[fld > max_var_cut] = max_var_cut
+
+
+def interp_all(stn_x, stn_y, cell_x, cell_y, data, vgs, methods, *,
+               stn_drfts=None, cell_drfts=None, min_var_thr=-np.inf,
+               min_var_cut=None, max_var_cut=None, min_vg_val=0.0,
+               neb_sel_mthd='all', n_nebs=None, n_pies=None,
+               est_vars_flag=True):
+    """The numeric core of interp/steps.py:478-835 without files.
+
+    data [T x N] with NaN for missing stations, vgs a length-T list of
+    variogram strings (or None without kriging), methods a list of
+    (type, label, idw_exp).  Returns (fields, fields64, prblm_steps):
+    fields[label] is the [T x C] float32 output (steps.py:659-663,
+    815-831), fields64 the same before the float32 cast (variances in
+    float64 too).  Steps without stations stay NaN (steps.py:677-688).
+    """
+    T, N = data.shape
+    C = cell_x.size
+    types = [m[0] for m in methods]
+    labels = [m[1] for m in methods]
+    edk = 'EDK' in types
+    want_var = est_vars_flag and ('OK' in types)
+    fields = {lab: np.full((T, C), np.nan, dtype=np.float32) for lab in labels}
+    f64 = {lab: np.full((T, C), np.nan) for lab in labels}
+    if want_var:
+        fields['EST_VARS_OK'] = np.full((T, C), np.nan, dtype=np.float32)
+        f64['EST_VARS_OK'] = np.full((T, C), np.nan)
+    prblm = []
+    steps = np.arange(T)
+    dst_ref_all = dists_2d(cell_x, cell_y, stn_x, stn_y)
+    ref_ref_all = dists_2d(stn_x, stn_y, stn_x, stn_y)
+    for stns, tmask in time_groups(data):
+        sub_steps = steps[tmask]
+        if not stns.size:
+            for s in sub_steps:
+                if s not in prblm:
+                    prblm.append(s)
+            continue
+        vals = data[np.ix_(tmask, stns)]
+        tidx = np.where(tmask)[0]
+        if vgs is not None:
+            models = [vgs[t] for t in tidx]
+            nugf = np.array([check_full_nuggetness(m, min_vg_val)
+                             for m in models])
+        else:
+            models, nugf = None, None
+        sx, sy = stn_x[stns], stn_y[stns]
+        if neb_sel_mthd == 'all':
+            neb = neb_idxs_all(C, stns.size)
+        elif neb_sel_mthd == 'nrst':
+            neb = neb_idxs_nrst(cell_x, cell_y, sx, sy, n_nebs)
+        elif neb_sel_mthd == 'pie':
+            neb = neb_idxs_pie(cell_x, cell_y, sx, sy, n_nebs, n_pies)
+        else:
+            raise NotImplementedError(neb_sel_mthd)
+        for cells in neb_groups(neb):
+            sub = neb[cells[0]]
+            sub_vals = vals[:, sub]
+            gstns = stns[sub]
+            dst_ref = dst_ref_all[np.ix_(cells, gstns)]
+            ref_ref = ref_ref_all[np.ix_(gstns, gstns)]
+            flags = np.array([np.any(sub_vals[j] >= min_var_thr)
+                              for j in range(sub_vals.shape[0])])
+            if edk:
+                s_rd = stn_drfts[gstns, :]
+                s_dd = cell_drfts[:, cells]
+            else:
+                s_rd = s_dd = None
+            for (typ, lab, idw_exp) in methods:
+                if lab == 'EST_VARS_OK':
+                    continue
+                ivals, evars = get_interp(
+                    sub_vals, typ, s_rd, s_dd,
+                    idw_exp if typ == 'IDW' else 5, models, flags, nugf,
+                    dst_ref, ref_ref, min_vg_val, want_var, prblm, sub_steps)
+                mod_min_max(ivals, min_var_cut, max_var_cut)
+                for j, t in enumerate(tidx):
+                    f64[lab][t, cells] = ivals[j]
+                    fields[lab][t, cells] = ivals[j].astype(np.float32)
+                    if evars is not None:
+                        f64['EST_VARS_OK'][t, cells] = evars[j]
+                        fields['EST_VARS_OK'][t, cells] = evars[j]
+    return fields, f64, prblm
+
+
+# ---------------------------------------------------------------------------
+# Class API, cyth/interpmthds.pyx:251-719
+# ---------------------------------------------------------------------------
+
+
+def _class_parse(model):
+    comps = []
+    for sub in model.split('+'):
+        sub = sub.strip()
+        sill, vg = sub.split(' ')
+        vg, rng = vg.split('(')
+        rng = rng.split(')')[0]
+        comps.append((vg, float(sill), max(1e-5, float(rng))))
+    return comps
+
+
+def _class_in_vars(xi, yi, comps, sign, start):
+    n = xi.size
+    d = dists_2d(xi, yi, xi, yi)
+    M = np.full((n, n), start, dtype=np.float64)
+    off = ~np.eye(n, dtype=bool)
+    for name, sill, rng in comps:
+        M[off] += sign * VG_FTNS[name](d[off], rng, sill)
+    return M
+
+
+def _class_out_vars(xi, yi, xk, yk, comps, sign, start):
+    d = dists_2d(xk, yk, xi, yi)
+    out = np.full(d.shape, start, dtype=np.float64)
+    nz = d != 0.0
+    for name, sill, rng in comps:
+        out[nz] += sign * VG_FTNS[name](d[nz], rng, sill)
+    return out
+
+
+def ordinary_kriging(xi, yi, zi, xk, yk, model='1.0 Sph(2)'):
+    """pyx:251-362: zero-diagonal Gamma, rhs skips d == 0, pinv, variance
+    max(0, lambda.gamma + mu).  Returns dict of the class attributes."""
+    comps = _class_parse(model)
+    n = xi.size
+    K = np.zeros((n + 1, n + 1))
+    K[:n, :n] = _class_in_vars(xi, yi, comps, 1.0, 0.0)
+    K[n, :n] = 1.0
+    K[:, n] = 1.0
+    K[n, n] = 0.0
+    Kinv = np.linalg.pinv(K)
+    g = _class_out_vars(xi, yi, xk, yk, comps, 1.0, 0.0)
+    R = np.concatenate([g, np.ones((xk.size, 1))], axis=1)
+    L = R @ Kinv.T
+    mus = L[:, n]
+    lam = L[:, :n]
+    zk = (lam * zi[None, :]).sum(axis=1)
+    ev = np.maximum(0.0, (lam * g).sum(axis=1) + mus)
+    return dict(zk=zk, est_vars=ev, mus=mus, lambdas=lam, rhss=R,
+                in_vars=K, in_vars_inv=Kinv)
+
+
+def simple_kriging(xi, yi, zi, xk, yk, model='1.0 Sph(2)'):
+    """pyx:365-471.  in_covars starts at 0 (self.covar before the sills are
+    summed), so K = -Gamma off the diagonal and 0 on it; rhs starts at the
+    summed sill.  est_covars = max(0, covar - lambda.rhs)."""
+    comps = _class_parse(model)
+    covar = sum(c[1] for c in comps)
+    K = _class_in_vars(xi, yi, comps, -1.0, 0.0)
+    Kinv = np.linalg.pinv(K)
+    rhs = _class_out_vars(xi, yi, xk, yk, comps, -1.0, covar)
+    lam = rhs @ Kinv.T
+    zk = (lam * zi[None, :]).sum(axis=1)
+    ec = np.maximum(0.0, covar - (lam * rhs).sum(axis=1))
+    return dict(zk=zk, est_covars=ec, lambdas=lam, rhss=rhs, out_covars=rhs,
+                in_covars=K, in_covars_inv=Kinv)
+
+
+def external_drift_kriging_md(xi, yi, zi, si, xk, yk, sk, model='1.0 Sph(2)'):
+    """pyx:588-719: zero-diagonal Gamma, a [1 | S] constraint block with a
+    zero corner, rhs skips d == 0; mus_arr[(m+1) x C]."""
+    assert si.ndim == 2 and xi.ndim == 1
+    assert xi.shape[0] == yi.shape[0] == zi.shape[0] == si.shape[1]
+    assert xk.shape[0] == yk.shape[0] == sk.shape[1]
+    assert si.shape[0] == sk.shape[0]
+    comps = _class_parse(model)
+    n, m = xi.size, si.shape[0]
+    K = np.zeros((n + m + 1, n + m + 1))
+    K[:n, :n] = _class_in_vars(xi, yi, comps, 1.0, 0.0)
+    K[n, :n] = 1.0
+    K[:n, n] = 1.0
+    for k in range(m):
+        K[n + 1 + k, :n] = si[k]
+        K[:n, n + 1 + k] = si[k]
+    Kinv = np.linalg.pinv(K)
+    g = _class_out_vars(xi, yi, xk, yk, comps, 1.0, 0.0)
+    R = np.concatenate([g, np.ones((xk.size, 1)), sk.T], axis=1)
+    L = R @ Kinv.T
+    lam = L[:, :n]
+    zk = (lam * zi[None, :]).sum(axis=1)
+    return dict(zk=zk, mus_arr=L[:, n:].T.copy(), lambdas=lam, rhss=R,
+                in_vars=K, in_vars_inv=Kinv)

@@ -1,0 +1,874 @@
+// Cell-side kernels of the step interpolation and the batch driver behind
+// spk_interp_steps_dev.
+//
+//  k_rhs_plain   station-cell distance -> semivariogram/covariance, written
+//                as the reference's right-hand sides R [(n+p) x cells]
+//                (interp/steps.py:203-234), plus the nearest active station
+//  k_gemm<..>    MFMA contraction over all cells of a step:
+//                PLAIN: Lambda = K^-1 R (interp/steps.py:416) with the
+//                       estimate / sum-of-weights / variance epilogue of
+//                       interp/steps.py:418-434 fused in
+//                TRI  : W = L^-1 K, q = |W|^2 per cell (variance term)
+//  k_kgen<..>    fused distance -> covariance for the batched path, with
+//                the dual-form estimate k.alpha accumulated in FP64
+//  k_idw_nnb     interp/steps.py:285-313 (IDW with per-cell max
+//                normalisation, nearest neighbour)
+//  k_special     one-station, mean-fill and failed-system steps
+//                (interp/steps.py:280-283, 325-331, 372-384)
+#include <algorithm>
+
+#include "spk_krige.h"
+
+namespace spk {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using V = d4_t;
+  __device__ static inline V op(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using V = f4_t;
+  __device__ static inline V op(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// np.isclose(x, 1.0) with rtol 1e-5, atol 1e-8 (interp/steps.py:418)
+__device__ inline bool isclose1(double x) {
+  const double tol = 1e-8 + 1e-5 * 1.0;
+  return fabs(x - 1.0) <= tol;
+}
+
+__global__ void k_recenter(const double *__restrict__ in,
+                           double *__restrict__ out, int64_t n, double o) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i] - o;
+}
+
+// ---- plain path right-hand sides ---------------------------------------------
+__global__ __launch_bounds__(256) void k_rhs_plain(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ idx, const StepSc *__restrict__ sc,
+    const spk_vg *__restrict__ vgp, const double *__restrict__ cx,
+    const double *__restrict__ cy, int64_t Cs, int64_t C,
+    const double *__restrict__ cell_drift, int method, int m, int covar_flag,
+    double min_vg_val, double *__restrict__ R, int64_t ldr,
+    int32_t *__restrict__ nnb) {
+  if (sc->status) return;
+  const int n = sc->n;
+  __shared__ double stx[256], sty[256];
+  __shared__ spk_vg vg;
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x * (int64_t)256 + tid;
+  const bool valid = c < Cs;
+  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  if (tid == 0) vg = *vgp;
+  double best = INFINITY;
+  int bi = 0;
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + tid < n) {
+      stx[tid] = sx[idx[base + tid]];
+      sty[tid] = sy[idx[base + tid]];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    if (valid) {
+      for (int j = 0; j < cnt; ++j) {
+        const double d = dist2(x, y, stx[j], sty[j]);
+        if (d < best) {
+          best = d;
+          bi = base + j;
+        }
+        R[(int64_t)(base + j) * ldr + c] =
+            vg_entry(vg, d, covar_flag, min_vg_val);
+      }
+    }
+  }
+  if (!valid) return;
+  if (method != SPK_M_SK) R[(int64_t)n * ldr + c] = 1.0;
+  if (method == SPK_M_EDK)
+    for (int k = 0; k < m; ++k)
+      R[(int64_t)(n + 1 + k) * ldr + c] = cell_drift[(int64_t)k * C + c];
+  nnb[c] = bi;
+}
+
+// ---- MFMA contraction --------------------------------------------------------
+// 256 threads = 4 waves; block tile 64 rows x 64 cells; wave (wr, wc) owns a
+// 32 x 32 sub-tile as 2 x 2 MFMA 16x16 tiles; k in chunks of 16.
+// A is column-major [P x P] (lda), B row-major [P x Cs] (ldb).
+// MFMA 16x16x4 operand layout: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; the result holds D[4 (l / 16) + r][l % 16] in slot r.
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(256) void k_gemm(
+    const T *__restrict__ A, int64_t lda, int64_t strideA,
+    const T *__restrict__ B, int64_t ldb, int64_t strideB,
+    const StepSc *__restrict__ sc, int p, int method, int want_var,
+    const double *__restrict__ zact, int64_t ldz,
+    const int32_t *__restrict__ nnb, int64_t ldn,
+    const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
+    double *__restrict__ est, double *__restrict__ var, int64_t ldo) {
+  using MF = Mfma<T>;
+  using V = typename MF::V;
+  const int sl = blockIdx.y;
+  const StepSc st = sc[sl];
+  if (st.status) return;
+  const int n = st.n;
+  const int P = PLAIN ? n + p : n;
+  A += (int64_t)sl * strideA;
+  B += (int64_t)sl * strideB;
+  const double *z = zact + (int64_t)sl * ldz;
+  constexpr int PAD = sizeof(T) == 8 ? 2 : 4;
+  __shared__ T As[16][64 + PAD];
+  __shared__ T Bs[16][64 + PAD];
+  __shared__ double red[4][2][16][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t c0 = blockIdx.x * (int64_t)64;
+  const int li = lane & 15, lg = lane >> 4;
+
+  double acc_e[2] = {0.0, 0.0}, acc_l[2] = {0.0, 0.0};
+  double acc_v[2] = {0.0, 0.0}, acc_m[2] = {0.0, 0.0};
+
+  for (int r0 = 0; r0 < P; r0 += 64) {
+    V acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
+    const int kend = PLAIN ? P : min(P, r0 + 64);
+    for (int k0 = 0; k0 < kend; k0 += 16) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = tid + q * 256;
+        const int r = e & 63, k = e >> 6;
+        const int gr = r0 + r, gk = k0 + k;
+        As[k][r] = (gr < P && gk < kend) ? A[gr + (int64_t)gk * lda] : T(0);
+        const int64_t gc = c0 + r;
+        Bs[k][r] = (gk < kend && gc < Cs) ? B[(int64_t)gk * ldb + gc] : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int ka = kk * 4 + lg;
+        const T a0 = As[ka][wr * 32 + li];
+        const T a1 = As[ka][wr * 32 + 16 + li];
+        const T b0 = Bs[ka][wc * 32 + li];
+        const T b1 = Bs[ka][wc * 32 + 16 + li];
+        acc[0][0] = MF::op(a0, b0, acc[0][0]);
+        acc[0][1] = MF::op(a0, b1, acc[0][1]);
+        acc[1][0] = MF::op(a1, b0, acc[1][0]);
+        acc[1][1] = MF::op(a1, b1, acc[1][1]);
+      }
+      __syncthreads();
+    }
+    // epilogue for this row block
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      const int64_t col = c0 + wc * 32 + tj * 16 + li;
+      if (col >= Cs) continue;
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + wr * 32 + ti * 16 + 4 * lg + r;
+          const double v = (double)acc[ti][tj][r];
+          if (PLAIN) {
+            if (row < n) {
+              acc_e[tj] += v * z[row];
+              acc_l[tj] += v;
+            }
+            if (want_var && row <= n)
+              acc_v[tj] += v * (double)B[(int64_t)row * ldb + col];
+            if (want_var && row == n) acc_m[tj] += v;
+          } else {
+            if (row < n) acc_v[tj] += v * v;
+          }
+        }
+      }
+    }
+  }
+  // reduce the four lane groups that share a column, then the two waves
+#pragma unroll
+  for (int tj = 0; tj < 2; ++tj) {
+    acc_e[tj] += __shfl_xor(acc_e[tj], 16, 64);
+    acc_e[tj] += __shfl_xor(acc_e[tj], 32, 64);
+    acc_l[tj] += __shfl_xor(acc_l[tj], 16, 64);
+    acc_l[tj] += __shfl_xor(acc_l[tj], 32, 64);
+    acc_v[tj] += __shfl_xor(acc_v[tj], 16, 64);
+    acc_v[tj] += __shfl_xor(acc_v[tj], 32, 64);
+    acc_m[tj] += __shfl_xor(acc_m[tj], 16, 64);
+    acc_m[tj] += __shfl_xor(acc_m[tj], 32, 64);
+    if (lg == 0) {
+      red[wave][tj][li][0] = acc_e[tj];
+      red[wave][tj][li][1] = acc_l[tj];
+      red[wave][tj][li][2] = acc_v[tj];
+      red[wave][tj][li][3] = acc_m[tj];
+    }
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int wcc = tid >> 5, tj = (tid >> 4) & 1, l = tid & 15;
+    const int64_t col = c0 + wcc * 32 + tj * 16 + l;
+    if (col < Cs) {
+      const int w0 = wcc, w1 = 2 + wcc;  // wr = 0 / 1 with the same wc
+      const double e = red[w0][tj][l][0] + red[w1][tj][l][0];
+      const double ls = red[w0][tj][l][1] + red[w1][tj][l][1];
+      const double vs = red[w0][tj][l][2] + red[w1][tj][l][2];
+      const double mu = red[w0][tj][l][3] + red[w1][tj][l][3];
+      double *eo = est + (int64_t)sl * ldo;
+      double *vo = var ? var + (int64_t)sl * ldo : nullptr;
+      if (PLAIN) {
+        if (isclose1(ls)) {
+          eo[col] = e;
+          if (want_var) vo[col] = vs + mu;
+        } else {
+          eo[col] = z[nnb[(int64_t)sl * ldn + col]];
+          if (want_var) vo[col] = 0.0;
+        }
+      } else {
+        // OK variance lambda.g + 2 mu (interp/steps.py:432-434) in
+        // covariance form: c0 - k^T C^-1 k + (a - 1)(a - 2) / b
+        const double a = abuf[(int64_t)sl * ldab + col];
+        vo[col] = st.c0 - vs + (a - 1.0) * (a - 2.0) / st.b;
+      }
+    }
+  }
+}
+
+// ---- batched path: fused distance -> covariance -------------------------------
+// For each active station i and cell c: k_ic = c0 - G(d_ic) (OK/EDK) or the
+// covariance sum (SK); the estimate k.alpha (+ constraint terms) and, for OK
+// with variance, a = k.w are accumulated in FP64 and k is stored as KT for the
+// contraction.  SK keeps the reference's sum-of-weights test (k.w == 1).
+template <typename KT>
+__global__ __launch_bounds__(256) void k_kgen(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ zact, int64_t ldz,
+    const StepSc *__restrict__ sc, const spk_vg *__restrict__ vgs,
+    const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
+    int64_t C, const double *__restrict__ cell_drift,
+    const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
+    double min_vg_val, int want_k, KT *__restrict__ K, int64_t ldk,
+    double *__restrict__ abuf, double *__restrict__ est, int64_t ldo) {
+  const int sl = blockIdx.y;
+  const StepSc st = sc[sl];
+  if (st.status) return;
+  const int n = st.n;
+  const int32_t *my = idx + (int64_t)sl * N;
+  const double *alpha = V + (int64_t)sl * ldv;
+  const double *w = alpha + ld;
+  __shared__ double stx[256], sty[256], sal[256], sw[256];
+  __shared__ spk_vg vg;
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x * (int64_t)256 + tid;
+  const bool valid = c < Cs;
+  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  if (tid == 0) vg = vgs[sl];
+  const int cflag = method == SPK_M_SK ? 1 : 0;
+  const double c0 = st.c0;
+  KT *Ks = K ? K + (int64_t)sl * ld * ldk : nullptr;
+  double e = 0.0, a = 0.0, best = INFINITY;
+  int bi = 0;
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + tid < n) {
+      const int g = my[base + tid];
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+      sal[tid] = alpha[base + tid];
+      sw[tid] = w[base + tid];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    if (valid) {
+      for (int j = 0; j < cnt; ++j) {
+        const double d = dist2(x, y, stx[j], sty[j]);
+        if (d < best) {
+          best = d;
+          bi = base + j;
+        }
+        const double g = vg_entry(vg, d, cflag, min_vg_val);
+        const double k = cflag ? g : c0 - g;
+        e += k * sal[j];
+        a += k * sw[j];
+        if (want_k) Ks[(int64_t)(base + j) * ldk + c] = (KT)k;
+      }
+    }
+  }
+  if (!valid) return;
+  double *eo = est + (int64_t)sl * ldo;
+  const double *z = zact + (int64_t)sl * ldz;
+  if (method == SPK_M_OK) {
+    eo[c] = e + st.a0;
+    if (abuf) abuf[(int64_t)sl * ldk + c] = a;
+  } else if (method == SPK_M_EDK) {
+    double v = e + st.h[0];
+    for (int k = 0; k < m; ++k) v += st.h[1 + k] * cell_drift[(int64_t)k * C + c];
+    eo[c] = v;
+  } else {  // SK
+    eo[c] = isclose1(a) ? e : z[bi];
+  }
+}
+
+// ---- IDW and nearest neighbour -------------------------------------------------
+__global__ __launch_bounds__(256) void k_idw_nnb(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ zact, int64_t ldz,
+    const StepSc *__restrict__ sc, const double *__restrict__ cx,
+    const double *__restrict__ cy, int64_t C, int method, double p,
+    double *__restrict__ est, int *__restrict__ bad) {
+#pragma clang fp contract(off)
+  const int sl = blockIdx.y;
+  const StepSc st = sc[sl];
+  if (st.status) return;
+  const int n = st.n;
+  const int32_t *my = idx + (int64_t)sl * N;
+  const double *z = zact + (int64_t)sl * ldz;
+  __shared__ double stx[256], sty[256], sz[256];
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x * (int64_t)256 + tid;
+  const bool valid = c < C;
+  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  double dmax = 0.0, best = INFINITY;
+  int bi = 0;
+  // pass 1: max distance (IDW) / nearest station (NNB)
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + tid < n) {
+      const int g = my[base + tid];
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+      sz[tid] = z[base + tid];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    if (valid)
+      for (int j = 0; j < cnt; ++j) {
+        const double d = dist2(x, y, stx[j], sty[j]);
+        dmax = fmax(dmax, d);
+        if (d < best) {
+          best = d;
+          bi = base + j;
+        }
+      }
+  }
+  double *eo = est + (int64_t)sl * C;
+  if (method == SPK_M_NNB) {
+    if (valid) eo[c] = z[bi];
+    return;
+  }
+  // pass 2: w = 1 / (d / dmax)^p, sums in station order (pyx:784-808)
+  double ws = 0.0, wz = 0.0;
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + tid < n) {
+      const int g = my[base + tid];
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+      sz[tid] = z[base + tid];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    if (valid)
+      for (int j = 0; j < cnt; ++j) {
+        double d = dist2(x, y, stx[j], sty[j]);
+        if (dmax > 0) d = d / dmax;
+        const double wj = 1.0 / (p == 2.0 ? d * d : pow(d, p));
+        ws = ws + wj;
+        wz = wz + wj * sz[j];
+      }
+  }
+  if (!valid) return;
+  if (!(ws >= 1e-14)) atomicOr(bad, 1);  // interp/steps.py:305
+  eo[c] = wz / ws;
+}
+
+// ---- special steps ---------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_special(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ zact, int64_t ldz,
+    const StepSc *__restrict__ sc, const double *__restrict__ cx,
+    const double *__restrict__ cy, int64_t C, double *__restrict__ est,
+    double *__restrict__ var) {
+  const int sl = blockIdx.y;
+  const StepSc st = sc[sl];
+  if (st.status == 0 || st.status == SPK_ST_NOSTNS) return;
+  const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+  const double *z = zact + (int64_t)sl * ldz;
+  double *eo = est + (int64_t)sl * C;
+  double *vo = var ? var + (int64_t)sl * C : nullptr;
+  if (st.status == SPK_ST_ONESTN) {
+    if (c < C) {
+      eo[c] = z[0];
+      if (vo) vo[c] = NAN;  // est_vars stays NaN (interp/steps.py:282-283)
+    }
+    return;
+  }
+  if (st.status == SPK_ST_MEANFILL) {
+    if (c < C) {
+      eo[c] = st.mean;
+      if (vo) vo[c] = 0.0;
+    }
+    return;
+  }
+  // failed system: nearest active station (interp/steps.py:372-384)
+  const int32_t *my = idx + (int64_t)sl * N;
+  __shared__ double stx[256], sty[256];
+  const int n = st.n;
+  const bool valid = c < C;
+  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  double best = INFINITY;
+  int bi = 0;
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + threadIdx.x < n) {
+      stx[threadIdx.x] = sx[my[base + threadIdx.x]];
+      sty[threadIdx.x] = sy[my[base + threadIdx.x]];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    if (valid)
+      for (int j = 0; j < cnt; ++j) {
+        const double d = dist2(x, y, stx[j], sty[j]);
+        if (d < best) {
+          best = d;
+          bi = base + j;
+        }
+      }
+  }
+  if (!valid) return;
+  eo[c] = z[bi];
+  if (vo) vo[c] = 0.0;
+}
+
+__global__ void k_status_out(const StepSc *__restrict__ sc, int S,
+                             int32_t *__restrict__ info) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S) info[s] = sc[s].status;
+}
+
+__global__ void k_clip_cast(const double *__restrict__ in,
+                            float *__restrict__ out, int64_t n, int has_min,
+                            double mn, int has_max, double mx) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = in[i];
+    if (has_min && v < mn) v = mn;
+    if (has_max && v > mx) v = mx;
+    out[i] = (float)v;
+  }
+}
+
+// ---- kernel profiling (hipEvents around the heavy launches) ----------------
+struct Prof {
+  bool on = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<std::string, std::pair<double, int64_t>>> stats;
+};
+
+static Prof &prof_of(spk_ctx *ctx) {
+  static std::vector<std::pair<spk_ctx *, Prof *>> all;
+  for (auto &kv : all)
+    if (kv.first == ctx) return *kv.second;
+  all.push_back({ctx, new Prof()});
+  return *all.back().second;
+}
+
+static hipEvent_t prof_event(Prof &pf) {
+  if (!pf.pool.empty()) {
+    hipEvent_t e = pf.pool.back();
+    pf.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  spk_ctx *ctx;
+  Prof &pf;
+  const char *name;
+  hipEvent_t a{}, b{};
+  ProfScope(spk_ctx *c, const char *nm) : ctx(c), pf(prof_of(c)), name(nm) {
+    if (pf.on) {
+      a = prof_event(pf);
+      hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~ProfScope() {
+    if (pf.on) {
+      b = prof_event(pf);
+      hipEventRecord(b, ctx->stream);
+      pf.pending.push_back({name, a, b});
+    }
+  }
+};
+
+static int prof_collect(spk_ctx *ctx) {
+  Prof &pf = prof_of(ctx);
+  if (pf.pending.empty()) return SPK_OK;
+  SPK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto &r : pf.pending) {
+    float ms = 0.f;
+    SPK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    bool found = false;
+    for (auto &kv : pf.stats)
+      if (kv.first == r.name) {
+        kv.second.first += ms;
+        kv.second.second += 1;
+        found = true;
+      }
+    if (!found) pf.stats.push_back({r.name, {(double)ms, (int64_t)1}});
+    pf.pool.push_back(r.a);
+    pf.pool.push_back(r.b);
+  }
+  pf.pending.clear();
+  return SPK_OK;
+}
+
+static int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" {
+
+int spk_prof_enable(spk_ctx *ctx, int on) {
+  if (!ctx) return set_error(SPK_EARG, "null ctx");
+  prof_of(ctx).on = on != 0;
+  return SPK_OK;
+}
+
+int spk_prof_reset(spk_ctx *ctx) {
+  if (!ctx) return set_error(SPK_EARG, "null ctx");
+  SPK_TRY(prof_collect(ctx));
+  prof_of(ctx).stats.clear();
+  return SPK_OK;
+}
+
+int spk_prof_get(spk_ctx *ctx, const char *name, double *total_ms,
+                 int64_t *count) {
+  if (!ctx || !name || !total_ms || !count)
+    return set_error(SPK_EARG, "null argument");
+  SPK_TRY(prof_collect(ctx));
+  *total_ms = 0.0;
+  *count = 0;
+  for (auto &kv : prof_of(ctx).stats)
+    if (kv.first == name) {
+      *total_ms = kv.second.first;
+      *count = kv.second.second;
+    }
+  return SPK_OK;
+}
+
+int spk_clip_cast_dev(spk_ctx *ctx, const double *in64, float *out32,
+                      int64_t n, int has_min, double min_cut, int has_max,
+                      double max_cut) {
+  if (!ctx) return set_error(SPK_EARG, "null ctx");
+  if (n <= 0) return SPK_OK;
+  SPK_HIP(hipSetDevice(ctx->device));
+  int64_t g = std::min<int64_t>(ceil_div(n, 256), 65536);
+  hipLaunchKernelGGL(k_clip_cast, dim3((unsigned)g), dim3(256), 0, ctx->stream,
+                     in64, out32, n, has_min, min_cut, has_max, max_cut);
+  return check_launch("k_clip_cast");
+}
+
+int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
+                         const spk_opts *opts, int64_t T, const double *vals,
+                         const spk_vg *vgs, double *est, double *var,
+                         int32_t *info) {
+  if (!ctx || !prob || !opts) return set_error(SPK_EARG, "null argument");
+  if (T < 0) return set_error(SPK_ESHAPE, "negative step count");
+  if (T == 0) return SPK_OK;
+  const int64_t N = prob->n_stns, C = prob->n_cells;
+  if (N < 1 || C < 1) return set_error(SPK_ESHAPE, "need stations and cells");
+  if (N > (int64_t)1 << 20)
+    return set_error(SPK_ESHAPE, "more than 2^20 stations");
+  if (!vals || !est || !info || !prob->stn_x || !prob->stn_y ||
+      !prob->cell_x || !prob->cell_y)
+    return set_error(SPK_EARG, "null device pointer");
+  const int method = opts->method;
+  if (method < SPK_M_OK || method > SPK_M_NNB)
+    return set_error(SPK_EARG, "unknown method %d", method);
+  const bool krig = method <= SPK_M_EDK;
+  if (krig && !vgs) return set_error(SPK_EARG, "kriging needs variograms");
+  const int m = method == SPK_M_EDK ? prob->n_drifts : 0;
+  if (method == SPK_M_EDK &&
+      (m < 1 || m > kMaxP - 1 || !prob->stn_drift || !prob->cell_drift))
+    return set_error(SPK_EARG, "EDK needs 1..%d drifts", kMaxP - 1);
+  const int prec = opts->precision;
+  if (prec < SPK_P_FP64 || prec > SPK_P_FP64B)
+    return set_error(SPK_EARG, "unknown precision %d", prec);
+  const bool want_var = opts->want_var && method == SPK_M_OK && var;
+  SPK_HIP(hipSetDevice(ctx->device));
+  hipStream_t sm = ctx->stream;
+
+  // re-centred coordinates (exact for origins on the coordinates' grid)
+  double *sx, *sy, *cx, *cy;
+  SPK_TRY(ws_get(ctx, "stn_xy", 2 * N * sizeof(double), (void **)&sx));
+  sy = sx + N;
+  SPK_TRY(ws_get(ctx, "cell_xy", 2 * C * sizeof(double), (void **)&cx));
+  cy = cx + C;
+  {
+    const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(N, 256), 4096);
+    const unsigned gc = (unsigned)std::min<int64_t>(ceil_div(C, 256), 65536);
+    hipLaunchKernelGGL(k_recenter, dim3(gs), dim3(256), 0, sm, prob->stn_x, sx,
+                       N, prob->origin_x);
+    hipLaunchKernelGGL(k_recenter, dim3(gs), dim3(256), 0, sm, prob->stn_y, sy,
+                       N, prob->origin_y);
+    hipLaunchKernelGGL(k_recenter, dim3(gc), dim3(256), 0, sm, prob->cell_x, cx,
+                       C, prob->origin_x);
+    hipLaunchKernelGGL(k_recenter, dim3(gc), dim3(256), 0, sm, prob->cell_y, cy,
+                       C, prob->origin_y);
+    SPK_TRY(check_launch("k_recenter"));
+  }
+  const int64_t ldz = round_up(N, 64);
+  const int p = method == SPK_M_OK ? 1 : method == SPK_M_EDK ? 1 + m : 0;
+  const int64_t ld = round_up(N + (p > 0 ? p : 1), 64);
+
+  // batch size
+  int64_t S;
+  if (!krig) {
+    S = std::min<int64_t>(T, 512);
+  } else {
+    const int64_t per_step =
+        ld * ld * 8 * 2 + (prec == SPK_P_FP64 ? ld * ld * 8
+                                              : (prec == SPK_P_MIXED && want_var
+                                                     ? ld * ld * 4
+                                                     : 0));
+    const int64_t budget = (int64_t)6 << 30;
+    S = std::max<int64_t>(1, std::min<int64_t>(budget / per_step, 64));
+    S = std::min<int64_t>(S, T);
+  }
+  int32_t *idx;
+  double *zact;
+  StepSc *sc;
+  int *bad;
+  SPK_TRY(ws_get(ctx, "idx", S * N * sizeof(int32_t), (void **)&idx));
+  SPK_TRY(ws_get(ctx, "zact", S * ldz * sizeof(double), (void **)&zact));
+  SPK_TRY(ws_get(ctx, "stepsc", S * sizeof(StepSc), (void **)&sc));
+  SPK_TRY(ws_get(ctx, "bad", 64, (void **)&bad));
+  SPK_HIP(hipMemsetAsync(bad, 0, sizeof(int), sm));
+  const double thr = opts->min_var_thr;
+  const double mvg = opts->min_vg_val;
+
+  // work buffers for kriging
+  double *A = nullptr, *Li = nullptr, *M = nullptr, *V = nullptr;
+  double *T0 = nullptr;
+  float *L32 = nullptr;
+  const int64_t strideM = ld * ld;
+  if (krig) {
+    SPK_TRY(ws_get(ctx, "A", S * strideM * 8, (void **)&A));
+    SPK_TRY(ws_get(ctx, "Li", S * strideM * 8, (void **)&Li));
+    SPK_TRY(ws_get(ctx, "T0", S * ld * 2 * (kMaxP + 1) * 8, (void **)&T0));
+    if (prec == SPK_P_FP64) {
+      SPK_TRY(ws_get(ctx, "M", S * strideM * 8, (void **)&M));
+    } else {
+      SPK_TRY(ws_get(ctx, "V", S * 2 * ld * 8, (void **)&V));
+      if (prec == SPK_P_MIXED && want_var)
+        SPK_TRY(ws_get(ctx, "L32", S * strideM * 4, (void **)&L32));
+    }
+  }
+  // cell stripes for the materialised right-hand sides / covariances
+  int64_t Cs = C;
+  const size_t rhs_budget = (size_t)4 << 30;
+  if (krig && prec == SPK_P_FP64) {
+    Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
+  } else if (krig && want_var) {
+    const int64_t eb = prec == SPK_P_MIXED ? 4 : 8;
+    Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / (S * ld * eb)));
+  }
+  Cs = std::max<int64_t>(64, Cs / 64 * 64);
+  if (Cs > C) Cs = C;
+  const int64_t ldk = round_up(Cs, 64);
+  double *R = nullptr, *abuf = nullptr;
+  void *Kbuf = nullptr;
+  int32_t *nnb = nullptr;
+  if (krig && prec == SPK_P_FP64) {
+    SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
+    SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
+  } else if (krig && want_var) {
+    const int64_t eb = prec == SPK_P_MIXED ? 4 : 8;
+    SPK_TRY(ws_get(ctx, "K", S * ld * ldk * eb, &Kbuf));
+    SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
+  }
+
+  for (int64_t t0 = 0; t0 < T; t0 += S) {
+    const int Sb = (int)std::min<int64_t>(S, T - t0);
+    const double *vb = vals + t0 * N;
+    const spk_vg *gb = krig ? vgs + t0 : nullptr;
+    double *eb = est + t0 * C;
+    double *vbo = want_var ? var + t0 * C : nullptr;
+    {
+      ProfScope ps(ctx, "compact");
+      hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm, vb, N, idx,
+                         zact, ldz, sc, gb, method, thr, mvg);
+    }
+    SPK_TRY(check_launch("k_compact"));
+    if (!krig) {
+      ProfScope ps(ctx, "idw_nnb");
+      hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
+                         dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
+                         cy, C, method, opts->idw_exp, eb, bad);
+      SPK_TRY(check_launch("k_idw_nnb"));
+    } else {
+      const int tiles = (int)ceil_div(N, 64);
+      {
+        ProfScope ps(ctx, "assemble");
+        hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles, Sb), dim3(256), 0,
+                           sm, sx, sy, idx, N, sc, gb,
+                           method == SPK_M_SK ? 1 : 0, mvg, A, ld, strideM);
+      }
+      SPK_TRY(check_launch("k_assemble"));
+      {
+        ProfScope ps(ctx, "potrf");
+        hipLaunchKernelGGL(k_potrf, dim3(Sb), dim3(256), 0, sm, A, ld, strideM,
+                           sc, gb, method == SPK_M_SK ? 0 : 1);
+      }
+      SPK_TRY(check_launch("k_potrf"));
+      {
+        ProfScope ps(ctx, "trtri");
+        hipLaunchKernelGGL(k_trtri, dim3(Sb), dim3(256), 0, sm, A, Li, ld,
+                           strideM, sc);
+      }
+      SPK_TRY(check_launch("k_trtri"));
+      if (prec == SPK_P_FP64) {
+        {
+          ProfScope ps(ctx, "minv");
+          hipLaunchKernelGGL(k_minv, dim3(Sb), dim3(256), 0, sm, Li, ld,
+                             strideM, sc, idx, N, prob->stn_drift, m, method,
+                             M, T0);
+        }
+        SPK_TRY(check_launch("k_minv"));
+        for (int s = 0; s < Sb; ++s) {
+          for (int64_t cb = 0; cb < C; cb += Cs) {
+            const int64_t cn = std::min<int64_t>(Cs, C - cb);
+            {
+              ProfScope ps(ctx, "rhs_plain");
+              hipLaunchKernelGGL(
+                  k_rhs_plain, dim3((unsigned)ceil_div(cn, 256)), dim3(256), 0,
+                  sm, sx, sy, idx + s * N, sc + s, gb + s, cx + cb, cy + cb, cn,
+                  C, prob->cell_drift ? prob->cell_drift + cb : nullptr,
+                  method, m, method == SPK_M_SK ? 1 : 0, mvg, R, ldk, nnb);
+            }
+            SPK_TRY(check_launch("k_rhs_plain"));
+            {
+              ProfScope ps(ctx, "gemm_plain_f64");
+              hipLaunchKernelGGL((k_gemm<double, true>),
+                                 dim3((unsigned)ceil_div(cn, 64), 1),
+                                 dim3(256), 0, sm, M + s * strideM, ld,
+                                 (int64_t)0, R, ldk, (int64_t)0, sc + s, p,
+                                 method, want_var ? 1 : 0, zact + s * ldz, ldz,
+                                 nnb, ldk, (const double *)nullptr,
+                                 (int64_t)0, cn, eb + s * C + cb,
+                                 vbo ? vbo + s * C + cb : nullptr, C);
+            }
+            SPK_TRY(check_launch("k_gemm<double,plain>"));
+          }
+        }
+      } else {
+        {
+          ProfScope ps(ctx, "dual");
+          hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sm, Li, ld,
+                             strideM, sc, zact, ldz, idx, N, prob->stn_drift,
+                             m, method, V, 2 * ld, T0, nullptr);
+        }
+        SPK_TRY(check_launch("k_dual"));
+        if (prec == SPK_P_MIXED && want_var) {
+          ProfScope ps(ctx, "cast_tri");
+          hipLaunchKernelGGL(k_cast_tri, dim3(1024, Sb), dim3(256), 0, sm, Li,
+                             L32, ld, strideM, sc);
+        }
+        SPK_TRY(check_launch("k_cast_tri"));
+        for (int64_t cb = 0; cb < C; cb += Cs) {
+          const int64_t cn = std::min<int64_t>(Cs, C - cb);
+          const double *cd =
+              prob->cell_drift ? prob->cell_drift + cb : nullptr;
+          if (prec == SPK_P_MIXED) {
+            ProfScope ps(ctx, "kgen_f32");
+            hipLaunchKernelGGL((k_kgen<float>),
+                               dim3((unsigned)ceil_div(cn, 256), Sb),
+                               dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc,
+                               gb, cx + cb, cy + cb, cn, C, cd, V, 2 * ld, ld,
+                               method, m, mvg, want_var ? 1 : 0,
+                               (float *)Kbuf, ldk, want_var ? abuf : nullptr,
+                               eb + cb, C);
+          } else {
+            ProfScope ps(ctx, "kgen_f64");
+            hipLaunchKernelGGL((k_kgen<double>),
+                               dim3((unsigned)ceil_div(cn, 256), Sb),
+                               dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc,
+                               gb, cx + cb, cy + cb, cn, C, cd, V, 2 * ld, ld,
+                               method, m, mvg, want_var ? 1 : 0,
+                               (double *)Kbuf, ldk, want_var ? abuf : nullptr,
+                               eb + cb, C);
+          }
+          SPK_TRY(check_launch("k_kgen"));
+          if (want_var) {
+            if (prec == SPK_P_MIXED) {
+              ProfScope ps(ctx, "gemm_tri_f32");
+              hipLaunchKernelGGL((k_gemm<float, false>),
+                                 dim3((unsigned)ceil_div(cn, 64), Sb),
+                                 dim3(256), 0, sm, L32, ld, strideM,
+                                 (const float *)Kbuf, ldk, ld * ldk, sc, p,
+                                 method, 1, zact, ldz, (const int32_t *)nullptr,
+                                 (int64_t)0, abuf, ldk, cn, eb + cb,
+                                 vbo + cb, C);
+            } else {
+              ProfScope ps(ctx, "gemm_tri_f64");
+              hipLaunchKernelGGL((k_gemm<double, false>),
+                                 dim3((unsigned)ceil_div(cn, 64), Sb),
+                                 dim3(256), 0, sm, Li, ld, strideM,
+                                 (const double *)Kbuf, ldk, ld * ldk, sc, p,
+                                 method, 1, zact, ldz, (const int32_t *)nullptr,
+                                 (int64_t)0, abuf, ldk, cn, eb + cb,
+                                 vbo + cb, C);
+            }
+            SPK_TRY(check_launch("k_gemm<tri>"));
+          }
+        }
+      }
+    }
+    {
+      ProfScope ps(ctx, "special");
+      hipLaunchKernelGGL(k_special, dim3((unsigned)ceil_div(C, 256), Sb),
+                         dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
+                         cy, C, eb, vbo);
+    }
+    SPK_TRY(check_launch("k_special"));
+    hipLaunchKernelGGL(k_status_out, dim3((unsigned)ceil_div(Sb, 256)),
+                       dim3(256), 0, sm, sc, Sb, info + t0);
+    SPK_TRY(check_launch("k_status_out"));
+  }
+  if (!krig) {
+    int hbad = 0;
+    SPK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, sm));
+    SPK_HIP(hipStreamSynchronize(sm));
+    if (hbad)
+      return set_error(SPK_EARG, "IDW weight sum below 1e-14 "
+                       "(interp/steps.py:305)");
+  }
+  return SPK_OK;
+}
+
+}  // extern "C"

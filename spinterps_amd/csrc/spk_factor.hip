@@ -1,0 +1,729 @@
+// Per-step kriging systems: station compaction, covariance assembly,
+// Cholesky, triangular inverse, the dual-form vectors of the batched path
+// and the explicit inverse operator of the plain path.
+//
+// Every kernel here runs one workgroup per time step (blockIdx.x = step),
+// so a launch factorises a whole batch of uneven systems at once.  Matrices
+// are column-major with a common leading dimension `ld`.
+//
+// Reference: interp/steps.py:170-243 (system layout), :350-371 (inverse),
+// misc.py:1074-1105 (nugget-only check), interp/steps.py:276, :760-765
+// (step means and interpolation flags).
+#include "spk_krige.h"
+
+namespace spk {
+
+// ---- step compaction -------------------------------------------------------
+// Active stations (non-NaN values) of each step, in station order, as
+// interp/grps.py:57-101 forms a step's station set; the step mean over them
+// (interp/steps.py:276) and the "any value >= min_var_thr" flag
+// (interp/steps.py:760-765); then the step status.
+__global__ __launch_bounds__(256) void k_compact(
+    const double *__restrict__ vals, int64_t N, int32_t *__restrict__ idx,
+    double *__restrict__ zact, int64_t ldz, StepSc *__restrict__ sc,
+    const spk_vg *__restrict__ vgs, int method, double thr,
+    double min_vg_val) {
+  const int s = blockIdx.x;
+  const double *row = vals + (int64_t)s * N;
+  int32_t *my_idx = idx + (int64_t)s * N;
+  double *my_z = zact + (int64_t)s * ldz;
+  __shared__ int wave_cnt[4];
+  __shared__ double red[256];
+  __shared__ int red_any[256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int base_off = 0;
+  double sum = 0.0;
+  int any = 0;
+  for (int64_t b = 0; b < N; b += 256) {
+    const int64_t i = b + tid;
+    const double v = (i < N) ? row[i] : NAN;
+    const bool act = !isnan(v);
+    const unsigned long long m = __ballot(act);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_cnt[wave] = __popcll(m);
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wave; ++w) woff += wave_cnt[w];
+    if (act) {
+      const int pos = base_off + woff + pre;
+      my_idx[pos] = (int32_t)i;
+      my_z[pos] = v;
+      sum += v;
+      any |= (v >= thr) ? 1 : 0;
+    }
+    const int tot = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    __syncthreads();
+    base_off += tot;
+  }
+  red[tid] = sum;
+  red_any[tid] = any;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      red[tid] += red[tid + w];
+      red_any[tid] |= red_any[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int n = base_off;
+    StepSc &o = sc[s];
+    o.n = n;
+    o.mean = n > 0 ? red[0] / n : NAN;
+    int st = 0;
+    if (n == 0) {
+      st = SPK_ST_NOSTNS;
+    } else if (n == 1) {
+      st = SPK_ST_ONESTN;
+    } else if (method == SPK_M_IDW) {
+      if (!red_any[0]) st = SPK_ST_MEANFILL;
+    } else if (method == SPK_M_NNB) {
+      st = 0;
+    } else {
+      const spk_vg &vg = vgs[s];
+      const bool nug =
+          (vg.sill_sum <= min_vg_val) || (vg.range_max_raw <= min_vg_val);
+      if (!red_any[0] || nug) st = SPK_ST_MEANFILL;
+    }
+    o.status = st;
+    o.c0 = 0.0;
+    o.b = 0.0;
+    o.a0 = 0.0;
+    for (int k = 0; k < kMaxP; ++k) o.h[k] = 0.0;
+  }
+}
+
+// ---- system assembly ---------------------------------------------------------
+// A[i, j] for i, j < n: the semivariogram sum G (OK, EDK) or the covariance
+// sum (SK) of fill_vg_var_arr (pyx:146-226) at the distance of active
+// stations i and j.  Coordinates are already re-centred.
+__global__ __launch_bounds__(256) void k_assemble(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ idx, int64_t N, const StepSc *__restrict__ sc,
+    const spk_vg *__restrict__ vgs, int covar_flag, double min_vg_val,
+    double *__restrict__ A, int64_t ld, int64_t strideA) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int tiles = (n + 63) / 64;
+  const int t = blockIdx.x;
+  if (t >= tiles * tiles) return;
+  const int bi = t % tiles, bj = t / tiles;
+  if (bi < bj) return;  // lower block triangle, mirrored on write
+  __shared__ double rx[64], ry[64], cx[64], cy[64];
+  __shared__ spk_vg vg;
+  const int tid = threadIdx.x;
+  const int32_t *my = idx + (int64_t)s * N;
+  if (tid < 64) {
+    const int i = bi * 64 + tid;
+    rx[tid] = i < n ? sx[my[i]] : 0.0;
+    ry[tid] = i < n ? sy[my[i]] : 0.0;
+  } else if (tid < 128) {
+    const int j = bj * 64 + tid - 64;
+    cx[tid - 64] = j < n ? sx[my[j]] : 0.0;
+    cy[tid - 64] = j < n ? sy[my[j]] : 0.0;
+  }
+  if (tid == 0) vg = vgs[s];
+  __syncthreads();
+  double *As = A + (int64_t)s * strideA;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    const int i = bi * 64 + r, j = bj * 64 + c;
+    if (i >= n || j >= n || (bi == bj && i < j)) continue;
+    const double d = dist2(rx[r], ry[r], cx[c], cy[c]);
+    const double v = vg_entry(vg, d, covar_flag, min_vg_val);
+    As[i + (int64_t)j * ld] = v;
+    As[j + (int64_t)i * ld] = v;
+  }
+}
+
+// ---- Cholesky ------------------------------------------------------------------
+// For OK/EDK the semivariogram matrix G is first shifted to covariance form
+// C = c0 - G (c0 = sum of sills for bounded models, otherwise twice the
+// largest entry plus the sill sum).  The constant shift leaves the kriging
+// weights unchanged because they sum to one, which is also why the
+// reference's nugget (added to every entry, pyx:192-201) cancels.  Then
+// C = L L^T, right-looking, 32-wide panels, 64x64 trailing tiles.  A failed
+// pivot marks the step SPK_ST_FAILED, which interp/steps.py:363-384 answers
+// with the nearest station's value.
+__global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
+                                               int64_t ld, int64_t strideA,
+                                               StepSc *__restrict__ sc,
+                                               const spk_vg *__restrict__ vgs,
+                                               int shift) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  double *As = A + (int64_t)s * strideA;
+  const int tid = threadIdx.x;
+  __shared__ double D[32][33];
+  __shared__ double Pa[64][33];
+  __shared__ double Pb[64][33];
+  __shared__ int fail;
+  __shared__ double red[256];
+  __shared__ double c0s;
+
+  if (shift) {
+    const spk_vg &vg = vgs[s];
+    bool bounded = true;
+    for (int k = 0; k < vg.n; ++k) {
+      const int ty = vg.type[k];
+      if (ty == SPK_VG_RNG || ty == SPK_VG_POW || ty == SPK_VG_HOL)
+        bounded = false;
+    }
+    if (bounded) {
+      if (tid == 0) c0s = vg.sill_sum;
+    } else {
+      double mx = 0.0;
+      for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
+        const int i = (int)(e % n), j = (int)(e / n);
+        if (i >= j) mx = fmax(mx, fabs(As[i + (int64_t)j * ld]));
+      }
+      red[tid] = mx;
+      __syncthreads();
+      for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+        __syncthreads();
+      }
+      if (tid == 0) c0s = 2.0 * red[0] + fabs(vg.sill_sum);
+    }
+    __syncthreads();
+    const double c0 = c0s;
+    for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
+      const int i = (int)(e % n), j = (int)(e / n);
+      if (i >= j) As[i + (int64_t)j * ld] = c0 - As[i + (int64_t)j * ld];
+    }
+    if (tid == 0) sc[s].c0 = c0;
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+
+  for (int k0 = 0; k0 < n; k0 += 32) {
+    const int jb = min(32, n - k0);
+    for (int e = tid; e < 32 * 32; e += 256) {
+      const int i = e & 31, j = e >> 5;
+      D[i][j] = (i < jb && j < jb && i >= j)
+                    ? As[(k0 + i) + (int64_t)(k0 + j) * ld]
+                    : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+      if (tid == 0) {
+        const double d = D[j][j];
+        if (!(d > 0.0) || !isfinite(d))
+          fail = k0 + j + 1;
+        else
+          D[j][j] = sqrt(d);
+      }
+      __syncthreads();
+      if (fail) break;
+      if (tid > j && tid < jb) D[tid][j] /= D[j][j];
+      __syncthreads();
+      for (int e = tid; e < 32 * 32; e += 256) {
+        const int i = e & 31, c = e >> 5;
+        if (c > j && i >= c && i < jb) D[i][c] -= D[i][j] * D[c][j];
+      }
+      __syncthreads();
+    }
+    if (fail) break;
+    for (int e = tid; e < 32 * 32; e += 256) {
+      const int i = e & 31, j = e >> 5;
+      if (i < jb && j < jb && i >= j)
+        As[(k0 + i) + (int64_t)(k0 + j) * ld] = D[i][j];
+    }
+    // panel: rows r >= k0 + jb solve x L_kk^T = a
+    for (int r = k0 + jb + tid; r < n; r += 256) {
+      double x[32];
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        x[c] = c < jb ? As[r + (int64_t)(k0 + c) * ld] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 32; ++c) {
+        if (c < jb) {
+          double v = x[c];
+#pragma unroll
+          for (int q = 0; q < c; ++q) v -= x[q] * D[c][q];
+          x[c] = v / D[c][c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        if (c < jb) As[r + (int64_t)(k0 + c) * ld] = x[c];
+    }
+    __syncthreads();
+    // trailing update of rows/cols >= m0
+    const int m0 = k0 + jb;
+    const int mt = n - m0;
+    if (mt <= 0) break;
+    const int nt = (mt + 63) / 64;
+    const int ty = tid >> 4, tx = tid & 15;
+    for (int bi = 0; bi < nt; ++bi) {
+      for (int bj = 0; bj <= bi; ++bj) {
+        const int i0 = m0 + bi * 64, j0 = m0 + bj * 64;
+        for (int e = tid; e < 64 * 32; e += 256) {
+          const int r = e & 63, c = e >> 6;
+          Pa[r][c] = (i0 + r < n && c < jb)
+                         ? As[(i0 + r) + (int64_t)(k0 + c) * ld]
+                         : 0.0;
+          Pb[r][c] = (j0 + r < n && c < jb)
+                         ? As[(j0 + r) + (int64_t)(k0 + c) * ld]
+                         : 0.0;
+        }
+        __syncthreads();
+        double acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+        for (int c = 0; c < jb; ++c) {
+          double pa[4], pb[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) pa[a] = Pa[ty * 4 + a][c];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) pb[b] = Pb[tx * 4 + b][c];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int i = i0 + ty * 4 + a;
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int j = j0 + tx * 4 + b;
+            if (i < n && j < n && i >= j) As[i + (int64_t)j * ld] -= acc[a][b];
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && fail) sc[s].status = SPK_ST_FAILED;
+}
+
+// ---- triangular inverse ----------------------------------------------------------
+// X = L^-1 (lower, zeros above the diagonal) by block rows of 64:
+// X[r, c] = inv(L_rr) (delta_rc I - sum_{c <= k < r} L[r, k] X[k, c]).
+// The n x n region of X is fully written (upper part zero).
+__global__ __launch_bounds__(256) void k_trtri(const double *__restrict__ L,
+                                               double *__restrict__ X,
+                                               int64_t ld, int64_t stride,
+                                               const StepSc *__restrict__ sc) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const double *Ls = L + (int64_t)s * stride;
+  double *Xs = X + (int64_t)s * stride;
+  const int tid = threadIdx.x;
+  const int ty = tid >> 4, tx = tid & 15;
+  __shared__ double Di[64][65];  // L_rr, inverted in place
+  __shared__ union {
+    struct {
+      double a[64][33];
+      double b[32][65];
+    } t;
+    double acc[64][65];
+  } sm;
+  const int nb = (n + 63) / 64;
+  // zero the strictly-upper block tiles
+  for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i / 64 < j / 64) Xs[i + (int64_t)j * ld] = 0.0;
+  }
+  for (int rb = 0; rb < nb; ++rb) {
+    const int r0 = rb * 64;
+    const int rn = min(64, n - r0);
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int i = e & 63, j = e >> 6;
+      Di[i][j] = (i < rn && j < rn && i >= j)
+                     ? Ls[(r0 + i) + (int64_t)(r0 + j) * ld]
+                     : 0.0;
+    }
+    __syncthreads();
+    // in-place inverse, last column first:
+    // X_jj = 1/L_jj, X[i, j] = -X_jj sum_{j<k<=i} X[i, k] L[k, j]
+    for (int j = rn - 1; j >= 0; --j) {
+      double v = 0.0;
+      const double djj = 1.0 / Di[j][j];
+      if (tid > j && tid < rn)
+        for (int k = j + 1; k <= tid; ++k) v += Di[tid][k] * Di[k][j];
+      __syncthreads();
+      if (tid > j && tid < rn) Di[tid][j] = -v * djj;
+      if (tid == j) Di[j][j] = djj;
+      __syncthreads();
+    }
+    for (int cb = 0; cb <= rb; ++cb) {
+      const int c0 = cb * 64;
+      double acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = (rb == cb && ty * 4 + a == tx * 4 + b) ? 1.0 : 0.0;
+      // acc -= L[r, k] X[k, c] for k in [c0, r0)
+      for (int k0 = c0; k0 < r0; k0 += 32) {
+        for (int e = tid; e < 64 * 32; e += 256) {
+          const int r = e & 63, k = e >> 6;
+          sm.t.a[r][k] =
+              (r < rn) ? Ls[(r0 + r) + (int64_t)(k0 + k) * ld] : 0.0;
+        }
+        for (int e = tid; e < 32 * 64; e += 256) {
+          const int k = e & 31, c = e >> 5;
+          sm.t.b[k][c] = (c0 + c < n)
+                             ? Xs[(k0 + k) + (int64_t)(c0 + c) * ld]
+                             : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < 32; ++k) {
+          double pa[4], pb[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) pa[a] = sm.t.a[ty * 4 + a][k];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) pb[b] = sm.t.b[k][tx * 4 + b];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] -= pa[a] * pb[b];
+        }
+        __syncthreads();
+      }
+      // X[r, c] = inv(L_rr) acc
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) sm.acc[ty * 4 + a][tx * 4 + b] = acc[a][b];
+      __syncthreads();
+      double out[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
+      for (int k = 0; k < 64; ++k) {
+        double pa[4], pb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) pa[a] = Di[ty * 4 + a][k];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) pb[b] = sm.acc[k][tx * 4 + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) out[a][b] += pa[a] * pb[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int i = r0 + ty * 4 + a;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int j = c0 + tx * 4 + b;
+          if (i < n && j < n)
+            Xs[i + (int64_t)j * ld] = (i >= j) ? out[a][b] : 0.0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---- dual-form vectors (batched path) -----------------------------------------
+// With Li = L^-1:  C^-1 x = Li^T (Li x).
+//   OK : w = C^-1 1, b = 1^T w, az = C^-1 z, sz = 1^T az,
+//        alpha = az - w sz / b, a0 = sz / b  ->  est = k.alpha + a0
+//        (the Schur complement of the Lagrange row, interp/steps.py:212-216)
+//   SK : alpha = C^-1 z, w = C^-1 1 (sum of weights for the isclose test of
+//        interp/steps.py:418)
+//   EDK: F = [1 | S], U = Li F, W = Li^T U, B = U^T U, h = B^-1 U^T (Li z),
+//        alpha = C^-1 z - W h  ->  est = k.alpha + h.[1; s_cell]
+// Vectors are stored per step in V[s][0:ld] (alpha) and V[s][ld:2 ld] (w).
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ void small_inverse(double *B, int p, double *Binv) {
+  // Gauss-Jordan with partial pivoting on a p x p (p <= kMaxP) row-major.
+  double a[kMaxP][2 * kMaxP];
+  for (int i = 0; i < p; ++i)
+    for (int j = 0; j < 2 * p; ++j)
+      a[i][j] = j < p ? B[i * p + j] : (j - p == i ? 1.0 : 0.0);
+  for (int c = 0; c < p; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < p; ++r)
+      if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+    if (piv != c)
+      for (int j = 0; j < 2 * p; ++j) {
+        double t = a[c][j];
+        a[c][j] = a[piv][j];
+        a[piv][j] = t;
+      }
+    const double d = a[c][c];
+    for (int j = 0; j < 2 * p; ++j) a[c][j] /= d;
+    for (int r = 0; r < p; ++r) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      for (int j = 0; j < 2 * p; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < p; ++i)
+    for (int j = 0; j < p; ++j) Binv[i * p + j] = a[i][p + j];
+}
+
+// Y[:, r] = Li X[:, r] (r < nr), X and Y column-major with leading dim ld.
+__device__ void tri_lower_mv(const double *Li, int64_t ld, int n,
+                             const double *X, double *Y, int nr) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double acc[kMaxP + 1];
+    for (int r = 0; r < nr; ++r) acc[r] = 0.0;
+    for (int k = 0; k <= i; ++k) {
+      const double l = Li[i + (int64_t)k * ld];
+      for (int r = 0; r < nr; ++r) acc[r] += l * X[k + (int64_t)r * ld];
+    }
+    for (int r = 0; r < nr; ++r) Y[i + (int64_t)r * ld] = acc[r];
+  }
+}
+
+// Y[:, r] = Li^T X[:, r]: one wave per output entry, lanes over rows.
+__device__ void tri_lower_tmv(const double *Li, int64_t ld, int n,
+                              const double *X, double *Y, int nr) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  for (int k = wave; k < n; k += nw) {
+    double acc[kMaxP + 1];
+    for (int r = 0; r < nr; ++r) acc[r] = 0.0;
+    for (int i = k + lane; i < n; i += 64) {
+      const double l = Li[i + (int64_t)k * ld];
+      for (int r = 0; r < nr; ++r) acc[r] += l * X[i + (int64_t)r * ld];
+    }
+    for (int r = 0; r < nr; ++r) {
+      const double v = wave_sum(acc[r]);
+      if (lane == 0) Y[k + (int64_t)r * ld] = v;
+    }
+  }
+}
+
+// Work array T0: [ld x 2 (kMaxP+1)] per step.
+__global__ __launch_bounds__(256) void k_dual(
+    const double *__restrict__ Li, int64_t ld, int64_t stride,
+    StepSc *__restrict__ sc, const double *__restrict__ zact, int64_t ldz,
+    const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ stn_drift, int m, int method,
+    double *__restrict__ V, int64_t ldv, double *__restrict__ T0,
+    double *__restrict__ T1) {
+  const int64_t ldt = ld;
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const double *L = Li + (int64_t)s * stride;
+  const double *z = zact + (int64_t)s * ldz;
+  const int32_t *my = idx + (int64_t)s * N;
+  // per-step work area: 2 (kMaxP + 1) columns of ld
+  double *X = T0 + (int64_t)s * ldt * 2 * (kMaxP + 1);
+  double *U = X + ldt * (kMaxP + 1);
+  double *alpha = V + (int64_t)s * ldv;
+  double *w = alpha + ldt;
+  const int p = (method == SPK_M_EDK) ? 1 + m : 1;
+  const int tid = threadIdx.x;
+  // X columns: 0..p-1 = F (ones, drifts), p = z
+  for (int i = tid; i < n; i += 256) {
+    X[i] = 1.0;
+    if (method == SPK_M_EDK)
+      for (int k = 0; k < m; ++k)
+        X[i + (int64_t)(1 + k) * ldt] = stn_drift[(int64_t)my[i] * m + k];
+    X[i + (int64_t)p * ldt] = z[i];
+  }
+  __syncthreads();
+  tri_lower_mv(L, ld, n, X, U, p + 1);  // U = Li [F z]   (ld of L = ld)
+  __syncthreads();
+  tri_lower_tmv(L, ld, n, U, X, p + 1);  // X = C^-1 [F z]
+  __syncthreads();
+  __shared__ double G[(kMaxP + 1) * (kMaxP + 1)];
+  // G[a][b] = U[:, a] . U[:, b]  (a, b <= p): B = U_F^T U_F, g = U_F^T U_z
+  for (int e = tid >> 6; e < (p + 1) * (p + 1); e += 4) {
+    const int a = e / (p + 1), b = e % (p + 1);
+    double acc = 0.0;
+    for (int i = (tid & 63); i < n; i += 64)
+      acc += U[i + (int64_t)a * ldt] * U[i + (int64_t)b * ldt];
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) G[e] = acc;
+  }
+  __syncthreads();
+  __shared__ double hcoef[kMaxP];
+  if (tid == 0) {
+    double B[kMaxP * kMaxP], Bi[kMaxP * kMaxP];
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b) B[a * p + b] = G[a * (p + 1) + b];
+    if (method == SPK_M_SK) {
+      // no constraint rows: alpha = C^-1 z, w = C^-1 1
+      hcoef[0] = 0.0;
+    } else {
+      small_inverse(B, p, Bi);
+      for (int a = 0; a < p; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < p; ++b) v += Bi[a * p + b] * G[b * (p + 1) + p];
+        hcoef[a] = v;
+      }
+    }
+    StepSc &o = sc[s];
+    o.b = G[0];  // 1^T C^-1 1
+    for (int a = 0; a < p; ++a) o.h[a] = hcoef[a];
+    o.a0 = (method == SPK_M_SK) ? 0.0 : hcoef[0];
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    double a = X[i + (int64_t)p * ldt];
+    if (method != SPK_M_SK)
+      for (int k = 0; k < p; ++k) a -= X[i + (int64_t)k * ldt] * hcoef[k];
+    alpha[i] = a;
+    w[i] = X[i];
+  }
+}
+
+// ---- explicit K^-1 for the plain path -------------------------------------------
+// Cinv = Li^T Li; W = Cinv F; B = F^T W; Q = W B^-1; P = Cinv - Q W^T.
+// The reference's K = [[G, F], [F^T, 0]] with G = c0 11^T - C has the
+// inverse [[-P, Q], [Q^T, B^-1 - c0 e0 e0^T]], written to M (ld rows).
+// SK has no constraint rows: M = Cinv.
+__global__ __launch_bounds__(256) void k_minv(
+    const double *__restrict__ Li, int64_t ld, int64_t stride,
+    StepSc *__restrict__ sc, const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ stn_drift, int m, int method,
+    double *__restrict__ M, double *__restrict__ T0) {
+  const int64_t ldt = ld;
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const double *L = Li + (int64_t)s * stride;
+  double *Ms = M + (int64_t)s * stride;
+  const int32_t *my = idx + (int64_t)s * N;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  __shared__ double Ta[32][65];
+  __shared__ double Tb[32][65];
+  const int nb = (n + 63) / 64;
+  // Cinv tiles (bi >= bj): sum over k >= bi*64 of L[k, i] L[k, j]
+  for (int bi = 0; bi < nb; ++bi) {
+    for (int bj = 0; bj <= bi; ++bj) {
+      double acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+      for (int k0 = bi * 64; k0 < n; k0 += 32) {
+        for (int e = tid; e < 32 * 64; e += 256) {
+          const int k = e & 31, c = e >> 5;
+          const int kk = k0 + k;
+          Ta[k][c] = (kk < n && bi * 64 + c < n)
+                         ? L[kk + (int64_t)(bi * 64 + c) * ld]
+                         : 0.0;
+          Tb[k][c] = (kk < n && bj * 64 + c < n)
+                         ? L[kk + (int64_t)(bj * 64 + c) * ld]
+                         : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < 32; ++k) {
+          double pa[4], pb[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) pa[a] = Ta[k][ty * 4 + a];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) pb[b] = Tb[k][tx * 4 + b];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int i = bi * 64 + ty * 4 + a;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int j = bj * 64 + tx * 4 + b;
+          if (i < n && j < n) {
+            Ms[i + (int64_t)j * ld] = acc[a][b];
+            Ms[j + (int64_t)i * ld] = acc[a][b];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (method == SPK_M_SK) return;
+  const int p = (method == SPK_M_EDK) ? 1 + m : 1;
+  // per-step work area: 2 (kMaxP + 1) columns of ld (W, then Q)
+  double *W = T0 + (int64_t)s * ldt * 2 * (kMaxP + 1);
+  // W = Cinv F
+  for (int i = tid; i < n; i += 256) {
+    double acc[kMaxP];
+    for (int r = 0; r < p; ++r) acc[r] = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double c = Ms[i + (int64_t)k * ld];
+      acc[0] += c;
+      for (int r = 1; r < p; ++r)
+        acc[r] += c * stn_drift[(int64_t)my[k] * m + (r - 1)];
+    }
+    for (int r = 0; r < p; ++r) W[i + (int64_t)r * ldt] = acc[r];
+  }
+  __syncthreads();
+  __shared__ double Bs[kMaxP * kMaxP], Bi[kMaxP * kMaxP];
+  for (int e = tid >> 6; e < p * p; e += 4) {
+    const int a = e / p, b = e % p;
+    double acc = 0.0;
+    for (int i = (tid & 63); i < n; i += 64) {
+      const double fa = a == 0 ? 1.0 : stn_drift[(int64_t)my[i] * m + a - 1];
+      acc += fa * W[i + (int64_t)b * ldt];
+    }
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) Bs[e] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) small_inverse(Bs, p, Bi);
+  __syncthreads();
+  // Q = W B^-1 (into columns p..2p-1 of the work array)
+  double *Q = W + (int64_t)p * ldt;
+  for (int i = tid; i < n; i += 256)
+    for (int a = 0; a < p; ++a) {
+      double v = 0.0;
+      for (int b = 0; b < p; ++b) v += W[i + (int64_t)b * ldt] * Bi[b * p + a];
+      Q[i + (int64_t)a * ldt] = v;
+    }
+  __syncthreads();
+  const double c0 = sc[s].c0;
+  for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    double v = Ms[i + (int64_t)j * ld];
+    for (int a = 0; a < p; ++a)
+      v -= Q[i + (int64_t)a * ldt] * W[j + (int64_t)a * ldt];
+    Ms[i + (int64_t)j * ld] = -v;
+  }
+  for (int i = tid; i < n; i += 256)
+    for (int a = 0; a < p; ++a) {
+      const double q = Q[i + (int64_t)a * ldt];
+      Ms[i + (int64_t)(n + a) * ld] = q;
+      Ms[(n + a) + (int64_t)i * ld] = q;
+    }
+  if (tid < p * p) {
+    const int a = tid / p, b = tid % p;
+    Ms[(n + a) + (int64_t)(n + b) * ld] =
+        Bi[a * p + b] - ((a == 0 && b == 0) ? c0 : 0.0);
+  }
+}
+
+// Li as float (column-major) for the FP32 contraction.
+__global__ void k_cast_tri(const double *__restrict__ Li, float *__restrict__ L32,
+                           int64_t ld, int64_t stride,
+                           const StepSc *__restrict__ sc) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int64_t tot = (int64_t)n * n;
+  const double *src = Li + (int64_t)s * stride;
+  float *dst = L32 + (int64_t)s * stride;
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    dst[i + (int64_t)j * ld] = (float)src[i + (int64_t)j * ld];
+  }
+}
+
+}  // namespace spk

@@ -1,0 +1,147 @@
+// Internal declarations shared by the libspk translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/spk.h"
+
+namespace spk {
+
+// ---- error state ------------------------------------------------------
+int set_error(int code, const char *fmt, ...);
+
+#define SPK_HIP(expr)                                                    \
+  do {                                                                   \
+    hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess)                                                \
+      return ::spk::set_error(SPK_EHIP, "%s failed: %s (%s:%d)", #expr,  \
+                              hipGetErrorString(e_), __FILE__, __LINE__);\
+  } while (0)
+
+#define SPK_TRY(expr)                                                    \
+  do {                                                                   \
+    int rc_ = (expr);                                                    \
+    if (rc_ != SPK_OK) return rc_;                                       \
+  } while (0)
+
+// ---- device workspace ----------------------------------------------------
+struct Buffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace spk
+
+struct spk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // named scratch buffers, grown on demand, freed by spk_ctx_trim/destroy
+  std::vector<std::pair<std::string, spk::Buffer>> bufs;
+  std::vector<hipEvent_t> events;
+  std::vector<std::string> stage_names;
+  int n_marks = 0;
+  std::vector<double> last_ms;
+};
+
+namespace spk {
+
+// Device buffer of at least `bytes`, cached under `name` in the context.
+int ws_get(spk_ctx *ctx, const char *name, size_t bytes, void **out);
+// Pinned-free helpers for the host-pointer API.
+int h2d(spk_ctx *ctx, void *dst, const void *src, size_t bytes);
+int d2h(spk_ctx *ctx, void *dst, const void *src, size_t bytes);
+int sync(spk_ctx *ctx);
+int check_launch(const char *what);
+// Stage timing (events recorded on the context stream).
+void mark_reset(spk_ctx *ctx);
+int mark(spk_ctx *ctx, const char *name);
+int mark_collect(spk_ctx *ctx);
+
+// Host-side parse of a variogram string (pyx:174-184).
+int parse_vg(const char *s, spk_vg *out);
+
+// ---- device helpers --------------------------------------------------------
+constexpr double kPi = 3.14159265358979311600;  // M_PI as a double
+
+// Euclidean distance with the reference's operation order and no fused
+// multiply-add: ((dx*dx) + (dy*dy)) ** 0.5 (pyx:141).
+__host__ __device__ inline double dist2(double x1, double y1, double x2,
+                                       double y2) {
+#pragma clang fp contract(off)
+  const double dx = x1 - x2;
+  const double dy = y1 - y2;
+  const double s = (dx * dx) + (dy * dy);
+  return sqrt(s);
+}
+
+// One variogram component gamma(h; r, s), pyx:38-83 (same operation order).
+__host__ __device__ inline double vg_comp(int type, double h, double r,
+                                         double s) {
+#pragma clang fp contract(off)
+  switch (type) {
+    case SPK_VG_RNG:
+      return h;
+    case SPK_VG_NUG:
+      return s;
+    case SPK_VG_SPH: {
+      if (h >= r) return s;
+      const double a = (1.5 * h) / r;
+      const double b = (h * h * h) / (2.0 * (r * r * r));
+      return s * (a - b);
+    }
+    case SPK_VG_EXP:
+      return s * (1.0 - exp((-3.0 * h) / r));
+    case SPK_VG_LIN:
+      if (h > r) return s;
+      return s * (h / r);
+    case SPK_VG_GAU:
+      return s * (1.0 - exp(-3.0 * ((h * h) / (r * r))));
+    case SPK_VG_POW:
+      return s * pow(h, r);
+    case SPK_VG_HOL: {
+      if (h == 0.0) return 0.0;
+      const double a = (kPi * h) / r;
+      return s * (1.0 - (sin(a) / a));
+    }
+    default:
+      return NAN;
+  }
+}
+
+// fill_vg_var_arr's per-entry value (pyx:162-216): components summed in
+// order from 0.0, covariance form s - gamma when covar_flag, then entries
+// <= min_vg_val set to 0.
+__host__ __device__ inline double vg_entry(const spk_vg &vg, double h,
+                                          int covar_flag, double min_vg_val) {
+#pragma clang fp contract(off)
+  double acc = 0.0;
+  const double mult = covar_flag ? 1.0 : 0.0;
+  const double sign = covar_flag ? -1.0 : 1.0;
+  for (int k = 0; k < vg.n; ++k)
+    acc += (mult * vg.sill[k]) + (sign * vg_comp(vg.type[k], h, vg.range[k],
+                                                 vg.sill[k]));
+  if (acc <= min_vg_val) acc = 0.0;
+  return acc;
+}
+
+// True when every component is bounded by its sill, so that
+// c0 = sum of sills turns the semivariogram matrix into a covariance.
+inline bool vg_bounded(const spk_vg &vg) {
+  for (int k = 0; k < vg.n; ++k) {
+    const int t = vg.type[k];
+    if (t == SPK_VG_RNG || t == SPK_VG_POW || t == SPK_VG_HOL) return false;
+  }
+  return true;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace spk

@@ -1,0 +1,45 @@
+// Shared declarations of the kriging kernels.
+#pragma once
+
+#include "spk_internal.h"
+
+namespace spk {
+
+constexpr int kMaxP = 8;  // constraint rows: 1 + up to 7 drifts
+
+// Per-step scalars kept on the device.
+struct StepSc {
+  int n;          // active stations
+  int status;     // SPK_ST_* (0 = regular kriging / IDW step)
+  double mean;    // mean of the active values (interp/steps.py:276)
+  double c0;      // covariance shift of the OK/EDK system
+  double b;       // 1^T C^-1 1
+  double a0;      // constant term of the dual-form estimate
+  double h[kMaxP];
+};
+
+__global__ void k_compact(const double *vals, int64_t N, int32_t *idx,
+                          double *zact, int64_t ldz, StepSc *sc,
+                          const spk_vg *vgs, int method, double thr,
+                          double min_vg_val);
+__global__ void k_assemble(const double *sx, const double *sy,
+                           const int32_t *idx, int64_t N, const StepSc *sc,
+                           const spk_vg *vgs, int covar_flag, double min_vg_val,
+                           double *A, int64_t ld, int64_t strideA);
+__global__ void k_potrf(double *A, int64_t ld, int64_t strideA, StepSc *sc,
+                        const spk_vg *vgs, int shift);
+__global__ void k_trtri(const double *L, double *X, int64_t ld, int64_t stride,
+                        const StepSc *sc);
+__global__ void k_dual(const double *Li, int64_t ld, int64_t stride,
+                       StepSc *sc, const double *zact, int64_t ldz,
+                       const int32_t *idx, int64_t N, const double *stn_drift,
+                       int m, int method, double *V, int64_t ldv, double *T0,
+                       double *T1);
+__global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
+                       StepSc *sc, const int32_t *idx, int64_t N,
+                       const double *stn_drift, int m, int method, double *M,
+                       double *T0);
+__global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
+                           int64_t stride, const StepSc *sc);
+
+}  // namespace spk

@@ -1,0 +1,232 @@
+"""GPU parity: libspk against the CPU oracle on seeded inputs.
+
+Tolerances (stated per test): distances and indices bit-exact; FP64
+variogram values 1e-13 relative; FP64 kriging estimates and variances
+1e-9 relative (the plain FP64 path and the batched FP64 path); the mixed
+path's estimates 1e-9 and its variances within the bound in the test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import spk_oracle as O
+from spinterps_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+VGS_ALL = ['0.1 Nug(0.0) + 0.9 Sph(25000.0)', '2 Exp(3000)', '1.5 Gau(8000.0)',
+           '1 Lin(5000) + 0.2 Nug(0)', '0.5 Pow(1.2)', '1 Hol(4000)',
+           '1 Rng(0)', '3 Sph(0)']
+
+
+def _rel(a, b, floor):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))
+
+
+def test_cyth_distances_bit_exact(gpu):
+    from spinterps_amd import cyth
+    rng = np.random.default_rng(11)
+    x1 = configs.ORIGIN_X + rng.uniform(0, 5e5, 300)
+    y1 = configs.ORIGIN_Y + rng.uniform(0, 5e5, 300)
+    x2 = configs.ORIGIN_X + rng.uniform(0, 5e5, 77)
+    y2 = configs.ORIGIN_Y + rng.uniform(0, 5e5, 77)
+    out = np.full((300, 77), np.nan)
+    cyth.fill_dists_2d_mat(x1, y1, x2, y2, out)
+    np.testing.assert_array_equal(out, O.dists_2d(x1, y1, x2, y2))
+    d1 = np.full(77, np.nan)
+    cyth.fill_dists_one_pt(x1[3], y1[3], x2, y2, d1)
+    np.testing.assert_array_equal(d1, out[3])
+    pts = rng.normal(size=(40, 3))
+    np.testing.assert_allclose(cyth.get_nd_dists(pts), O.get_nd_dists(pts),
+                               rtol=1e-15)
+
+
+@pytest.mark.parametrize('vg', VGS_ALL)
+@pytest.mark.parametrize('covar,diag,mvv', [(0, 1, 0.0), (1, 0, 0.0),
+                                            (0, 0, 0.3), (1, 1, -np.inf)])
+def test_cyth_vg_var_arr(gpu, vg, covar, diag, mvv):
+    from spinterps_amd import cyth
+    rng = np.random.default_rng(12)
+    x, y = rng.uniform(0, 2e4, 50), rng.uniform(0, 2e4, 50)
+    d = O.dists_2d(x, y, x, y)
+    got = np.full_like(d, np.nan)
+    exp = np.full_like(d, np.nan)
+    cyth.fill_vg_var_arr(d, got, covar, diag, vg, mvv)
+    O.fill_vg_var_arr(d, exp, covar, diag, vg, mvv)
+    np.testing.assert_allclose(got, exp, rtol=1e-13, atol=1e-300)
+
+
+def test_cyth_small_helpers(gpu):
+    from spinterps_amd import cyth
+    rng = np.random.default_rng(13)
+    arr = rng.normal(size=(20, 30))
+    ri = np.array([3, 0, 19, 7], dtype=np.int64)
+    ci = np.array([29, 1, 1, 5, 0], dtype=np.int64)
+    sub = np.full((6, 7), np.nan)
+    cyth.copy_2d_arr_at_idxs(arr, ri, ci, sub)
+    np.testing.assert_array_equal(sub[:4, :5], arr[np.ix_(ri, ci)])
+    assert np.isnan(sub[4:]).all() and np.isnan(sub[:, 5:]).all()
+    with pytest.raises(IndexError):
+        cyth.copy_2d_arr_at_idxs(arr, np.array([20]), ci, sub)
+    d = rng.uniform(0.1, 1.0, 100)
+    w = np.empty(100)
+    we = np.empty(100)
+    s = cyth.fill_wts_and_sum(d, w, 2.0)
+    se = O.fill_wts_and_sum(d, we, 2.0)
+    np.testing.assert_allclose(w, we, rtol=1e-15)
+    assert math.isclose(s, se, rel_tol=1e-13)
+    z = rng.normal(size=100)
+    assert math.isclose(cyth.get_mults_sum(w, z), O.get_mults_sum(w, z),
+                        rel_tol=1e-12)
+    h = np.linspace(0, 1e6, 10)
+    vals = np.zeros(10)
+    for nm, r, sl in [('Sph', 1e4, 100.0), ('Exp', 1e6, 10.0)]:
+        cyth.fill_theo_vg_vals(nm, h, r, sl, vals)
+    np.testing.assert_allclose(
+        vals, O.get_theo_vg_vals('100 Sph(10000) + 10 Exp(1000000)', h),
+        rtol=1e-14)
+    with pytest.raises(AssertionError):
+        cyth.fill_theo_vg_vals('Sph', h, -1.0, 1.0, vals)
+
+
+def test_neighbours_bit_exact(gpu):
+    from spinterps_amd import _lib
+    from spinterps_amd._lib import check, context, dptr, i64ptr, lib
+    rng = np.random.default_rng(14)
+    sx, sy = rng.uniform(0, 1e5, 150), rng.uniform(0, 1e5, 150)
+    cx, cy = rng.uniform(0, 1e5, 400), rng.uniform(0, 1e5, 400)
+    for k in (1, 7, 64, 150, 200):
+        kk = min(k, 150)
+        out = np.full((400, kk), -1, dtype=np.int64)
+        check(lib().spk_neb_idxs_nrst(context().h, dptr(cx), dptr(cy), 400,
+                                      dptr(sx), dptr(sy), 150, k, i64ptr(out)))
+        np.testing.assert_array_equal(out, O.neb_idxs_nrst(cx, cy, sx, sy, k))
+    for k, p in ((8, 4), (20, 6), (9, 9)):
+        out = np.full((400, k), -1, dtype=np.int64)
+        check(lib().spk_neb_idxs_pie(context().h, dptr(cx), dptr(cy), 400,
+                                     dptr(sx), dptr(sy), 150, k, p,
+                                     i64ptr(out)))
+        np.testing.assert_array_equal(out, O.neb_idxs_pie(cx, cy, sx, sy, k, p))
+
+
+def test_sel_equidist_single_point(gpu):
+    from spinterps_amd import cyth
+    sx = np.array([1.0, 0.0, -3.0, -1.0, -3.0, 2.0, 5.0])
+    sy = np.array([0.0, 2.0, 0.0, 2.0, -3.0, -2.0, 0.0])
+    res = []
+    for impl in (cyth.sel_equidist_refs, O.sel_equidist_refs):
+        a = [np.zeros(7), np.zeros(7), np.zeros(7, np.int64),
+             np.zeros(7, np.uint64), np.zeros(4, np.uint64)]
+        impl(0.0, 0.0, sx, sy, 4, -1.0, -1, *a)
+        res.append(a)
+    for g, e in zip(*res):
+        np.testing.assert_array_equal(g, e)
+
+
+# ---- step interpolation ---------------------------------------------------------
+
+def _small_case(n_side=20, steps=6, mask=False, seed=20240601):
+    w = configs.small(seed=seed, n_cells_side=n_side, cell=1e5 / n_side)
+    vals, vgs = configs.small_steps(w, 0, steps)
+    if mask:
+        rng = np.random.default_rng(99)
+        for t in range(steps):
+            vals[t, rng.choice(w.n_stns, 10 + 3 * t, replace=False)] = np.nan
+    return w, vals, vgs
+
+
+def _oracle_fields(w, vals, vgs, methods, **kw):
+    return O.interp_all(w.stn_x, w.stn_y, w.cell_x, w.cell_y, vals, vgs,
+                        methods, stn_drfts=w.stn_drift, cell_drfts=w.cell_drift,
+                        **kw)
+
+
+@pytest.mark.parametrize('method', ['IDW', 'NNB'])
+def test_idw_nnb_steps(gpu, method):
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w, vals, vgs = _small_case(mask=True)
+    prob = DeviceProblem.from_workload(w)
+    est, _, info = interp_steps(prob, vals, None, method, idw_exp=2.0)
+    _, f64, _ = _oracle_fields(w, vals, None, [(method, method, 2.0)])
+    got = est.cpu().numpy()
+    if method == 'NNB':
+        np.testing.assert_array_equal(got, f64[method])
+    else:
+        np.testing.assert_allclose(got, f64[method], rtol=1e-12)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp64b', 'mixed'])
+def test_ok_small_all_precisions(gpu, precision):
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w, vals, vgs = _small_case(mask=True)
+    prob = DeviceProblem.from_workload(w)
+    est, var, info = interp_steps(prob, vals, vgs, 'OK', precision=precision)
+    _, f64, _ = _oracle_fields(w, vals, vgs, [('OK', 'OK', None)])
+    e, v = est.cpu().numpy(), var.cpu().numpy()
+    assert (info.cpu().numpy() == 0).all()
+    scale = np.nanmax(np.abs(f64['OK']))
+    np.testing.assert_allclose(e, f64['OK'], rtol=1e-9, atol=1e-12 * scale)
+    vref = f64['EST_VARS_OK']
+    vfloor = 1e-3 * np.nanmax(vref)
+    err = _rel(v, vref, vfloor)
+    print(f'{precision}: est max rel {_rel(e, f64["OK"], 1e-12 * scale):.2e} '
+          f'var max rel (floor {vfloor:.3g}) {err:.2e}')
+    assert err <= (1e-9 if precision != 'mixed' else 1e-4)
+
+
+@pytest.mark.parametrize('method', ['SK', 'EDK'])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_sk_edk_daily_subset(gpu, method, precision):
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.daily(n_side=40, cell=12500.0, n_stns=300)
+    vals, vgs = configs.daily_steps(w, 0, 4)
+    prob = DeviceProblem.from_workload(w)
+    est, _, info = interp_steps(prob, vals, vgs, method, precision=precision,
+                                want_var=False)
+    _, f64, _ = _oracle_fields(w, vals, vgs, [(method, method, None)])
+    e = est.cpu().numpy()
+    scale = np.nanmax(np.abs(f64[method]))
+    np.testing.assert_allclose(e, f64[method], rtol=1e-9, atol=1e-12 * scale)
+
+
+def test_ok_daily_subset_plain_vs_oracle(gpu):
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.daily(n_side=30, cell=16000.0, n_stns=400)
+    vals, vgs = configs.daily_steps(w, 10, 13)
+    prob = DeviceProblem.from_workload(w)
+    _, f64, _ = _oracle_fields(w, vals, vgs, [('OK', 'OK', None)])
+    for prec in ('fp64', 'fp64b'):
+        est, var, info = interp_steps(prob, vals, vgs, 'OK', precision=prec)
+        e, v = est.cpu().numpy(), var.cpu().numpy()
+        np.testing.assert_allclose(e, f64['OK'], rtol=1e-9,
+                                   atol=1e-12 * np.nanmax(np.abs(f64['OK'])))
+        np.testing.assert_allclose(v, f64['EST_VARS_OK'], rtol=1e-9,
+                                   atol=1e-12 * np.nanmax(f64['EST_VARS_OK']))
+
+
+def test_special_steps(gpu):
+    """No station (NaN out), one station (its value, NaN variance), all
+    values below min_var_thr and nugget-only variograms (mean, variance
+    0): interp/steps.py:280-283, 325-331, 677-688."""
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w, vals, vgs = _small_case(steps=5)
+    vals = vals.copy()
+    vals[0, :] = np.nan
+    vals[1, 1:] = np.nan
+    vals[2] = np.minimum(vals[2], 0.5)
+    vgs = list(vgs)
+    vgs[3] = '1.0 Nug(0.0)'
+    prob = DeviceProblem.from_workload(w)
+    for prec in ('fp64', 'mixed'):
+        est, var, info = interp_steps(prob, vals, vgs, 'OK', precision=prec,
+                                      min_var_thr=1.0)
+        _, f64, prb = _oracle_fields(w, vals, vgs, [('OK', 'OK', None)],
+                                     min_var_thr=1.0)
+        e, v = est.cpu().numpy(), var.cpu().numpy()
+        np.testing.assert_array_equal(info.cpu().numpy()[:4], [1, 8, 2, 2])
+        np.testing.assert_allclose(e, f64['OK'], rtol=1e-9, equal_nan=True)
+        np.testing.assert_allclose(v[1:], f64['EST_VARS_OK'][1:], rtol=1e-4,
+                                   atol=1e-9, equal_nan=True)
+        assert np.isnan(e[0]).all() and np.isnan(v[0]).all()
+        assert prb == [0]
