@@ -26,12 +26,16 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// Result row held in slot r by lane group g = lane / 16 (column lane % 16):
+// f64 16x16x4 interleaves rows (g + 4 r), f32 16x16x4 keeps blocks (4 g + r)
+// (checked on gfx950 against a host matmul).
 template <>
 struct Mfma<double> {
   using V = d4_t;
   __device__ static inline V op(double a, double b, V c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static constexpr int row(int g, int r) { return g + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -39,6 +43,7 @@ struct Mfma<float> {
   __device__ static inline V op(float a, float b, V c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static constexpr int row(int g, int r) { return 4 * g + r; }
 };
 
 // np.isclose(x, 1.0) with rtol 1e-5, atol 1e-8 (interp/steps.py:418)
@@ -107,7 +112,7 @@ __global__ __launch_bounds__(256) void k_rhs_plain(
 // 32 x 32 sub-tile as 2 x 2 MFMA 16x16 tiles; k in chunks of 16.
 // A is column-major [P x P] (lda), B row-major [P x Cs] (ldb).
 // MFMA 16x16x4 operand layout: lane l supplies A[l % 16][l / 16] and
-// B[l / 16][l % 16]; the result holds D[4 (l / 16) + r][l % 16] in slot r.
+// B[l / 16][l % 16]; the result row in slot r is Mfma<T>::row(l / 16, r).
 template <typename T, bool PLAIN>
 __global__ __launch_bounds__(256) void k_gemm(
     const T *__restrict__ A, int64_t lda, int64_t strideA,
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(256) void k_gemm(
       for (int ti = 0; ti < 2; ++ti) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = r0 + wr * 32 + ti * 16 + 4 * lg + r;
+          const int row = r0 + wr * 32 + ti * 16 + MF::row(lg, r);
           const double v = (double)acc[ti][tj][r];
           if (PLAIN) {
             if (row < n) {

@@ -230,3 +230,35 @@ def test_special_steps(gpu):
                                    atol=1e-9, equal_nan=True)
         assert np.isnan(e[0]).all() and np.isnan(v[0]).all()
         assert prb == [0]
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_sk_kriging_branch(gpu, precision):
+    """SK keeps the reference's sum-of-weights test (interp/steps.py:418):
+    cells 1-5 cm from a station pass it (|sum lambda - 1| < 1e-5) while
+    their estimate still differs from the station value by ~1e-7 relative,
+    so both the kriging branch and the nearest-station fallback are
+    exercised."""
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.small(n_cells_side=6, cell=1e5 / 6)
+    rng = np.random.default_rng(21)
+    pick = rng.choice(w.n_stns, 40, replace=False)
+    ang = rng.uniform(0, 2 * np.pi, 40)
+    r = rng.uniform(0.01, 0.05, 40)
+    cx = np.concatenate([w.cell_x, w.stn_x[pick] + r * np.cos(ang)])
+    cy = np.concatenate([w.cell_y, w.stn_y[pick] + r * np.sin(ang)])
+    vals, _ = configs.small_steps(w, 0, 3)
+    vgs = ['0.2 Nug(0.0) + 1.0 Exp(30000.0)', '1.0 Gau(20000.0)',
+           '0.5 Sph(40000.0)']
+    prob = DeviceProblem(w.stn_x, w.stn_y, cx, cy)
+    est, _, info = interp_steps(prob, vals, vgs, 'SK', precision=precision,
+                                want_var=False)
+    f, f64, _ = O.interp_all(w.stn_x, w.stn_y, cx, cy, vals, vgs,
+                             [('SK', 'SK', None)])
+    e = est.cpu().numpy()
+    # the oracle takes the kriging branch for the near-station cells
+    near = slice(w.n_cells, None)
+    nnb = vals[:, pick]
+    assert (np.abs(f64['SK'][:, near] - nnb) > 1e-12 * np.abs(nnb)).any()
+    scale = np.nanmax(np.abs(f64["SK"]))
+    np.testing.assert_allclose(e, f64["SK"], rtol=1e-9, atol=1e-12 * scale)
