@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Headline benchmark: ordinary-kriging cell-estimates/s including the
+estimation variance, batched mixed-precision path vs the plain FP64 path.
+
+Workload (hourly): 2000 stations over 500 km x 500 km, a 1000 x 1000 grid at
+500 m, per step a random 200-2000 active stations and its own variogram
+'<f v> Nug(0.0) + <(1-f) v> {Sph|Exp|Gau}(5-80 km)'; lognormal values.
+One bench step = one batch of --batch time steps over the full grid through
+the batched mixed path, inputs resident in HBM.  Synthetic seeded data.
+
+Prints ONE JSON line (rank 0, N = 1).  Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--batch', type=int, default=64,
+                    help='time steps per bench step')
+    ap.add_argument('--plain-steps', type=int, default=2)
+    ap.add_argument('--grid', type=int, default=1000)
+    ap.add_argument('--cpu-cells', type=int, default=300)
+    ap.add_argument('--cpu-steps', type=int, default=3)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(time steps shard across GPUs with no exchange; see '
+              'DESIGN.md).  Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from spinterps_amd import _lib, configs
+    from spinterps_amd.interp.steps import (DeviceProblem, interp_steps,
+                                            prof_enable, prof_get, vg_device)
+
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    dev = torch.device('cuda', 0)
+    B, K, W, P = args.batch, args.steps, args.warmup, args.plain_steps
+    w = configs.hourly(n_side=args.grid, cell=5e5 / args.grid)
+    C = w.n_cells
+    log(f'hourly workload: N={w.n_stns} C={C} batch={B} steps={K} '
+        f'warmup={W} plain_steps={P}')
+    n_tot = (W + K) * B
+    t = time.time()
+    vals, vgs = configs.hourly_steps(w, 0, n_tot + P)
+    log(f'generated {n_tot + P} steps in {time.time() - t:.1f}s')
+    prob = DeviceProblem.from_workload(w, device=0)
+    vals_d = torch.as_tensor(vals, device=dev)
+    vg_d = vg_device(vgs, dev)
+    n_act = (~np.isnan(vals)).sum(1)
+    est = torch.empty((B, C), dtype=torch.float64, device=dev)
+    var = torch.empty((B, C), dtype=torch.float64, device=dev)
+
+    def batch(i):
+        s0 = i * B
+        vrec = _lib.VG_DTYPE.itemsize
+        return (vals_d[s0:s0 + B], vg_d[s0 * vrec:(s0 + B) * vrec])
+
+    # ---- warmup + timed region (mixed) --------------------------------
+    infos = []
+    for i in range(W):
+        v, g = batch(i)
+        interp_steps(prob, v, None, 'OK', precision='mixed', est=est, var=var,
+                     vg_dev=g)
+        torch.cuda.synchronize()
+        log(f'warmup {i + 1}/{W} done')
+    prof_enable(0, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(W, W + K):
+        v, g = batch(i)
+        _, _, info = interp_steps(prob, v, None, 'OK', precision='mixed',
+                                  est=est, var=var, vg_dev=g)
+        infos.append(info)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / K
+    value = B * C / ((t1 - t0) / K)
+    log(f'mixed: {ms_per_step:.1f} ms per batch of {B} steps, '
+        f'{value:.3e} cell-estimates/s')
+    kern = {}
+    for nm in ('kgen', 'gemm_tri_f32', 'gemm_tri_f64', 'potrf', 'trtri',
+               'dual', 'assemble', 'compact', 'special', 'cast_tri'):
+        ms, cnt = prof_get(nm)
+        kern[nm] = (ms, cnt)
+    prof_enable(0, False)
+    info_t = torch.cat(infos).cpu().numpy()
+    timed_n = n_act[W * B:(W + K) * B]
+    f32_steps = ((info_t & 63) & ~_lib.ST_ILLCOND) == 0
+    f64_steps = (info_t & 32) != 0
+    # algorithmic flops of W = L^-1 delta: n^2 per cell (n(n+1)/2 MACs)
+    fl32 = float(np.sum(timed_n[f32_steps].astype(np.float64) ** 2) * C)
+    fl64 = float(np.sum(timed_n[f64_steps].astype(np.float64) ** 2) * C)
+    ms32, c32 = kern['gemm_tri_f32']
+    ms64, c64 = kern['gemm_tri_f64']
+    ach32 = fl32 / (ms32 * 1e-3) / 1e12 if ms32 > 0 else 0.0
+    ach64 = fl64 / (ms64 * 1e-3) / 1e12 if ms64 > 0 else 0.0
+    # kgen: algorithmic HBM bytes = delta written (4 or 8 B) + cell coords
+    kb = float(np.sum(timed_n[f32_steps]) * C * 4 +
+               np.sum(timed_n[f64_steps]) * C * 8 + 16 * C * K * B)
+    mskg, ckg = kern['kgen']
+    kg_gbs = kb / (mskg * 1e-3) / 1e9 if mskg > 0 else 0.0
+    log('kernel totals (ms, launches): ' +
+        ', '.join(f'{k}={v[0]:.1f}/{v[1]}' for k, v in kern.items()))
+
+    # ---- plain FP64 path on P steps + error of mixed vs FP64 ------------
+    s0 = n_tot
+    pv = vals_d[s0:s0 + P]
+    vrec = _lib.VG_DTYPE.itemsize
+    pg = vg_d[s0 * vrec:(s0 + P) * vrec]
+    e64 = torch.empty((P, C), dtype=torch.float64, device=dev)
+    v64 = torch.empty((P, C), dtype=torch.float64, device=dev)
+    interp_steps(prob, pv[:1], None, 'OK', precision='fp64', est=e64[:1],
+                 var=v64[:1], vg_dev=pg[:vrec])   # warm the plain kernels
+    prof_enable(0, True)
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    _, _, pinfo = interp_steps(prob, pv, None, 'OK', precision='fp64',
+                               est=e64, var=v64, vg_dev=pg)
+    torch.cuda.synchronize()
+    tp1 = time.perf_counter()
+    msp, cp = prof_get('gemm_plain_f64')
+    prof_enable(0, False)
+    plain_rate = P * C / (tp1 - tp0)
+    log(f'plain fp64: {(tp1 - tp0) * 1e3 / P:.1f} ms per step, '
+        f'{plain_rate:.3e} cell-estimates/s')
+    pn = n_act[s0:s0 + P].astype(np.float64)
+    flp = float(np.sum(2 * (pn + 1) ** 2 * C))
+    achp = flp / (msp * 1e-3) / 1e12 if msp > 0 else 0.0
+    em = torch.empty((P, C), dtype=torch.float64, device=dev)
+    vm = torch.empty((P, C), dtype=torch.float64, device=dev)
+    _, _, minfo = interp_steps(prob, pv, None, 'OK', precision='mixed',
+                               est=em, var=vm, vg_dev=pg)
+    pi = pinfo.cpu().numpy()
+    comparable = ((pi & (_lib.ST_ILLCOND | _lib.ST_FAILED)) == 0)
+    errs_e, errs_v = [], []
+    for s in range(P):
+        if not comparable[s]:
+            continue
+        a, b = em[s], e64[s]
+        scale = b.abs().max()
+        re = ((a - b).abs() / torch.maximum(b.abs(), 1e-12 * scale))
+        rv = ((vm[s] - v64[s]).abs() / v64[s].abs())
+        errs_e.append(re.cpu().numpy())
+        errs_v.append(rv.cpu().numpy())
+    if errs_e:
+        ee = np.concatenate(errs_e)
+        ev = np.concatenate(errs_v)
+        err = {'est_max': float(ee.max()), 'est_median': float(np.median(ee)),
+               'var_max': float(ev.max()), 'var_median': float(np.median(ev)),
+               'steps_compared': int(comparable.sum()),
+               'steps_excluded_illcond': int((~comparable).sum())}
+    else:
+        err = {'steps_compared': 0,
+               'steps_excluded_illcond': int((~comparable).sum())}
+    log(f'mixed vs fp64: {err}')
+
+    # ---- CPU baseline (reference loop structure, bounded sample) -------
+    cpu = None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import spk_oracle as O
+        rng = np.random.default_rng(7)
+        cidx = rng.choice(C, args.cpu_cells, replace=False)
+        tot_setup, tot_cell, used = 0.0, 0.0, 0
+        ok_err = []
+        with threadpool_limits(1):
+            for s in range(args.cpu_steps):
+                st = n_tot + s if s < P else s
+                ts, tc, ecpu, vcpu = O.time_ok_step_loop(
+                    w.stn_x, w.stn_y, w.cell_x[cidx], w.cell_y[cidx],
+                    vals[st], vgs[st])
+                tot_setup += ts
+                tot_cell += tc
+                used += 1
+                if s < P and comparable[s]:
+                    g = em[s].cpu().numpy()[cidx]
+                    ok_err.append(float(np.max(np.abs(g - ecpu) /
+                                               np.maximum(np.abs(ecpu),
+                                                          1e-12))))
+                log(f'cpu sample step {s}: setup {ts:.2f}s, '
+                    f'{tc * 1e6:.0f} us/cell')
+        per_step = tot_setup / used + C * (tot_cell / used)
+        cpu = {'value': C / per_step, 'unit': 'cell-estimates/s', 'cores': 1,
+               'kind': 'port',
+               'sample': (f'{used} hourly steps x {args.cpu_cells} cells, '
+                          'numpy pinv per step + np.matmul per cell '
+                          '(interp/steps.py:350-434), one BLAS thread; '
+                          'rate = C / (setup + C * per-cell time)'),
+               'gpu_mixed_est_vs_oracle_max_rel': (max(ok_err) if ok_err
+                                                   else None)}
+        log(f'cpu baseline: {cpu}')
+
+    out = {
+        'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '
+                  'plain FP64 @<=1e-6 rel',
+        'value': value,
+        'unit': 'cell-estimates/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': W,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32+f64',
+        'data': 'synthetic (seeded hourly workload generator)',
+        'config': {'workload': 'hourly: 2000 stations, 1000x1000 grid at '
+                               '500 m, per-step variogram and 200-2000 '
+                               'active stations, OK estimate + variance',
+                   'time_steps_per_bench_step': B, 'n_cells': C,
+                   'n_stations': w.n_stns},
+        'plain_fp64': {'value': plain_rate, 'steps': P,
+                       'ratio_mixed_over_plain': value / plain_rate,
+                       'gemm_tflops': achp},
+        'error_mixed_vs_fp64': err,
+        'roofline': {'bound': 'mfma', 'achieved': ach32, 'peak': 157.3,
+                     'unit': 'TFLOP/s', 'frac': ach32 / 157.3,
+                     'traffic': None,
+                     'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA)'},
+        'rooflines': {
+            'gemm_tri_f64': {'bound': 'mfma', 'achieved': ach64,
+                             'peak': 78.6, 'unit': 'TFLOP/s',
+                             'frac': ach64 / 78.6},
+            'kgen': {'bound': 'hbm', 'achieved': kg_gbs, 'peak': 8000.0,
+                     'unit': 'GB/s', 'frac': kg_gbs / 8000.0},
+            'gemm_plain_f64': {'bound': 'mfma', 'achieved': achp,
+                               'peak': 78.6, 'unit': 'TFLOP/s',
+                               'frac': achp / 78.6}},
+        'steps_fp64_variance': int(f64_steps.sum()),
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

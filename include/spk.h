@@ -131,6 +131,10 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 #define SPK_ST_MEANFILL  2   /* interp flag off or nugget-only: mean fill  */
 #define SPK_ST_FAILED    4   /* factorisation failed: nearest-station fill */
 #define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
+#define SPK_ST_ILLCOND  16   /* condition estimate > 1e12: results are set
+                                by rounding (as the reference's pinv is) */
+#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: condition estimate > 1e5, so the
+                                variance contraction ran in FP64          */
 
 typedef struct spk_problem {
   int64_t n_stns;               /* N                                      */

@@ -702,3 +702,54 @@ def external_drift_kriging_md(xi, yi, zi, si, xk, yk, sk, model='1.0 Sph(2)'):
     zk = (lam * zi[None, :]).sum(axis=1)
     return dict(zk=zk, mus_arr=L[:, n:].T.copy(), lambdas=lam, rhss=R,
                 in_vars=K, in_vars_inv=Kinv)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline timing: the reference's per-step cost structure
+# ---------------------------------------------------------------------------
+
+
+def time_ok_step_loop(stn_x, stn_y, cell_x, cell_y, z_t, vg_str,
+                      min_vg_val=0.0):
+    """Time one OK step (estimate + variance) the way interp/steps.py runs it
+    for a step with its own variogram: semivariances of the cells against
+    the step's stations (steps.py:639-650 via pyx:146-226), the station
+    matrix and its pinv (steps.py:336-352), then one np.matmul per cell
+    and the isclose / estimate / variance of steps.py:415-434.
+
+    Returns (t_step_setup, t_per_cell, est, var) for the given cells; the
+    caller extrapolates to a full grid as t_setup + C * t_per_cell.
+    """
+    import time
+    act = ~np.isnan(z_t)
+    sx, sy, z = stn_x[act], stn_y[act], z_t[act]
+    n = z.size
+    t0 = time.perf_counter()
+    rr = dists_2d(sx, sy, sx, sy)
+    K = np.empty((n + 1, n + 1))
+    tmp = np.empty((n, n))
+    fill_vg_var_arr(rr, tmp, 0, 1, vg_str, min_vg_val)
+    K[:n, :n] = tmp
+    K[n, :n] = 1.0
+    K[:, n] = 1.0
+    K[n, n] = 0.0
+    Kinv = np.linalg.pinv(K)
+    t1 = time.perf_counter()
+    dr = dists_2d(cell_x, cell_y, sx, sy)
+    R = np.empty((cell_x.size, n + 1))
+    tmp2 = np.empty((cell_x.size, n))
+    fill_vg_var_arr(dr, tmp2, 0, 0, vg_str, min_vg_val)
+    R[:, :n] = tmp2
+    R[:, n] = 1.0
+    est = np.empty(cell_x.size)
+    var = np.empty(cell_x.size)
+    for i in range(cell_x.size):
+        lmds = np.matmul(Kinv, R[i])
+        if not np.isclose(lmds[:n].sum(), 1.0):
+            est[i] = z[np.argmin(dr[i])]
+            var[i] = 0.0
+        else:
+            est[i] = (lmds[:n] * z).sum()
+            var[i] = (lmds * R[i]).sum() + lmds[n]
+    t2 = time.perf_counter()
+    return t1 - t0, (t2 - t1) / cell_x.size, est, var

@@ -27,6 +27,7 @@ P_FP64, P_MIXED, P_FP64B = range(3)
 PRECISIONS = {'fp64': P_FP64, 'mixed': P_MIXED, 'fp64b': P_FP64B}
 
 ST_NOSTNS, ST_MEANFILL, ST_FAILED, ST_ONESTN = 1, 2, 4, 8
+ST_ILLCOND, ST_FP64VAR = 16, 32
 
 
 class SpkVg(ctypes.Structure):

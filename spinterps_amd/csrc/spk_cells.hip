@@ -121,12 +121,16 @@ __global__ __launch_bounds__(256) void k_gemm(
     const double *__restrict__ zact, int64_t ldz,
     const int32_t *__restrict__ nnb, int64_t ldn,
     const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
-    double *__restrict__ est, double *__restrict__ var, int64_t ldo) {
+    double *__restrict__ est, double *__restrict__ var, int64_t ldo,
+    int sel) {
   using MF = Mfma<T>;
   using V = typename MF::V;
   const int sl = blockIdx.y;
   const StepSc st = sc[sl];
   if (st.status) return;
+  // sel 1: only steps without SPK_ST_FP64VAR, sel 2: only steps with it
+  if (sel == 1 && (st.flags & SPK_ST_FP64VAR)) return;
+  if (sel == 2 && !(st.flags & SPK_ST_FP64VAR)) return;
   const int n = st.n;
   const int P = PLAIN ? n + p : n;
   A += (int64_t)sl * strideA;
@@ -241,10 +245,9 @@ __global__ __launch_bounds__(256) void k_gemm(
           if (want_var) vo[col] = 0.0;
         }
       } else {
-        // OK variance lambda.g + 2 mu (interp/steps.py:432-434) in
-        // covariance form: c0 - k^T C^-1 k + (a - 1)(a - 2) / b
-        const double a = abuf[(int64_t)sl * ldab + col];
-        vo[col] = st.c0 - vs + (a - 1.0) * (a - 2.0) / st.b;
+        // OK variance lambda.g + 2 mu (interp/steps.py:432-434):
+        // base (FP64 terms) minus the deflated quadratic form
+        vo[col] = abuf[(int64_t)sl * ldab + col] - vs;
       }
     }
   }
@@ -252,10 +255,17 @@ __global__ __launch_bounds__(256) void k_gemm(
 
 // ---- batched path: fused distance -> covariance -------------------------------
 // For each active station i and cell c: k_ic = c0 - G(d_ic) (OK/EDK) or the
-// covariance sum (SK); the estimate k.alpha (+ constraint terms) and, for OK
-// with variance, a = k.w are accumulated in FP64 and k is stored as KT for the
-// contraction.  SK keeps the reference's sum-of-weights test (k.w == 1).
-template <typename KT>
+// covariance sum (SK).  The estimate k.alpha (+ constraint terms) is summed
+// in FP64.  SK keeps the reference's sum-of-weights test (k.w == 1).
+// For the OK variance the contraction input is deflated by the nearest
+// active station j(c): delta_ic = G(d_ij) - G(d_ic) = k_ic - C_ij, formed in
+// FP64.  With q = k^T C^-1 k = C_jj + 2 delta_j + |L^-1 delta|^2 the variance
+// lambda.g + 2 mu becomes
+//     2 g_jc - G_jj - |L^-1 delta|^2 + (a - 1)(a - 2) / b,   a = k.w,
+// whose first terms are exact in FP64 and whose last GEMM term is small
+// near stations, where the variance is small.  base = everything but the
+// GEMM term goes to abuf; delta goes to K32 or, for steps flagged
+// SPK_ST_FP64VAR (or the FP64 batched path), to K64.
 __global__ __launch_bounds__(256) void k_kgen(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N,
@@ -264,8 +274,9 @@ __global__ __launch_bounds__(256) void k_kgen(
     const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
     int64_t C, const double *__restrict__ cell_drift,
     const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
-    double min_vg_val, int want_k, KT *__restrict__ K, int64_t ldk,
-    double *__restrict__ abuf, double *__restrict__ est, int64_t ldo) {
+    double min_vg_val, int want_var, int all64, float *__restrict__ K32,
+    double *__restrict__ K64, int64_t ldk, double *__restrict__ abuf,
+    double *__restrict__ est, int64_t ldo) {
   const int sl = blockIdx.y;
   const StepSc st = sc[sl];
   if (st.status) return;
@@ -282,9 +293,36 @@ __global__ __launch_bounds__(256) void k_kgen(
   if (tid == 0) vg = vgs[sl];
   const int cflag = method == SPK_M_SK ? 1 : 0;
   const double c0 = st.c0;
-  KT *Ks = K ? K + (int64_t)sl * ld * ldk : nullptr;
-  double e = 0.0, a = 0.0, best = INFINITY;
+  const bool var_on = want_var && method == SPK_M_OK;
+  const bool use64 = all64 || (st.flags & SPK_ST_FP64VAR);
+  float *K32s = (var_on && !use64) ? K32 + (int64_t)sl * ld * ldk : nullptr;
+  double *K64s = (var_on && use64) ? K64 + (int64_t)sl * ld * ldk : nullptr;
+  double best = INFINITY;
   int bi = 0;
+  // pass 1 (variance only): the nearest active station j(c)
+  if (var_on) {
+    for (int base = 0; base < n; base += 256) {
+      __syncthreads();
+      if (base + tid < n) {
+        const int g = my[base + tid];
+        stx[tid] = sx[g];
+        sty[tid] = sy[g];
+      }
+      __syncthreads();
+      const int cnt = min(256, n - base);
+      if (valid)
+        for (int j = 0; j < cnt; ++j) {
+          const double d = dist2(x, y, stx[j], sty[j]);
+          if (d < best) {
+            best = d;
+            bi = base + j;
+          }
+        }
+    }
+  }
+  const double jx = var_on ? sx[my[bi]] : 0.0;
+  const double jy = var_on ? sy[my[bi]] : 0.0;
+  double e = 0.0, a = 0.0, g_jc = 0.0;
   for (int base = 0; base < n; base += 256) {
     __syncthreads();
     if (base + tid < n) {
@@ -299,7 +337,7 @@ __global__ __launch_bounds__(256) void k_kgen(
     if (valid) {
       for (int j = 0; j < cnt; ++j) {
         const double d = dist2(x, y, stx[j], sty[j]);
-        if (d < best) {
+        if (!var_on && d < best) {
           best = d;
           bi = base + j;
         }
@@ -307,7 +345,16 @@ __global__ __launch_bounds__(256) void k_kgen(
         const double k = cflag ? g : c0 - g;
         e += k * sal[j];
         a += k * sw[j];
-        if (want_k) Ks[(int64_t)(base + j) * ldk + c] = (KT)k;
+        if (var_on) {
+          const double dj = dist2(stx[j], sty[j], jx, jy);
+          const double gij = vg_entry(vg, dj, 0, min_vg_val);
+          const double delta = gij - g;
+          if (base + j == bi) g_jc = g;
+          if (K32s)
+            K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
+          else
+            K64s[(int64_t)(base + j) * ldk + c] = delta;
+        }
       }
     }
   }
@@ -316,7 +363,11 @@ __global__ __launch_bounds__(256) void k_kgen(
   const double *z = zact + (int64_t)sl * ldz;
   if (method == SPK_M_OK) {
     eo[c] = e + st.a0;
-    if (abuf) abuf[(int64_t)sl * ldk + c] = a;
+    if (var_on) {
+      const double G0 = vg_entry(vg, 0.0, 0, min_vg_val);
+      abuf[(int64_t)sl * ldk + c] =
+          2.0 * g_jc - G0 + (a - 1.0) * (a - 2.0) / st.b;
+    }
   } else if (method == SPK_M_EDK) {
     double v = e + st.h[0];
     for (int k = 0; k < m; ++k) v += st.h[1 + k] * cell_drift[(int64_t)k * C + c];
@@ -462,7 +513,7 @@ __global__ __launch_bounds__(256) void k_special(
 __global__ void k_status_out(const StepSc *__restrict__ sc, int S,
                              int32_t *__restrict__ info) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < S) info[s] = sc[s].status;
+  if (s < S) info[s] = sc[s].status | sc[s].flags;
 }
 
 __global__ void k_clip_cast(const double *__restrict__ in,
@@ -700,23 +751,29 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   if (krig && prec == SPK_P_FP64) {
     Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
   } else if (krig && want_var) {
-    const int64_t eb = prec == SPK_P_MIXED ? 4 : 8;
-    Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / (S * ld * eb)));
+    const int64_t eb = prec == SPK_P_MIXED ? 12 : 8;
+    Cs = std::min<int64_t>(C, (int64_t)(2 * rhs_budget / (S * ld * eb)));
   }
   Cs = std::max<int64_t>(64, Cs / 64 * 64);
   if (Cs > C) Cs = C;
   const int64_t ldk = round_up(Cs, 64);
-  double *R = nullptr, *abuf = nullptr;
-  void *Kbuf = nullptr;
+  double *R = nullptr, *abuf = nullptr, *K64 = nullptr;
+  float *K32 = nullptr;
   int32_t *nnb = nullptr;
   if (krig && prec == SPK_P_FP64) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
   } else if (krig && want_var) {
-    const int64_t eb = prec == SPK_P_MIXED ? 4 : 8;
-    SPK_TRY(ws_get(ctx, "K", S * ld * ldk * eb, &Kbuf));
+    // FP64 buffer for the batched FP64 path and for the steps the mixed
+    // path promotes to FP64 (SPK_ST_FP64VAR)
+    SPK_TRY(ws_get(ctx, "K64", S * ld * ldk * 8, (void **)&K64));
+    if (prec == SPK_P_MIXED)
+      SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
     SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
   }
+  // SPK_P_MIXED: condition estimate above which a step's variance runs in
+  // FP64; the batched FP64 path runs every step in FP64
+  const double fp64var_thr = prec == SPK_P_MIXED ? 1e5 : INFINITY;
 
   for (int64_t t0 = 0; t0 < T; t0 += S) {
     const int Sb = (int)std::min<int64_t>(S, T - t0);
@@ -762,7 +819,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           ProfScope ps(ctx, "minv");
           hipLaunchKernelGGL(k_minv, dim3(Sb), dim3(256), 0, sm, Li, ld,
                              strideM, sc, idx, N, prob->stn_drift, m, method,
-                             M, T0);
+                             M, T0, gb, mvg);
         }
         SPK_TRY(check_launch("k_minv"));
         for (int s = 0; s < Sb; ++s) {
@@ -786,7 +843,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                  method, want_var ? 1 : 0, zact + s * ldz, ldz,
                                  nnb, ldk, (const double *)nullptr,
                                  (int64_t)0, cn, eb + s * C + cb,
-                                 vbo ? vbo + s * C + cb : nullptr, C);
+                                 vbo ? vbo + s * C + cb : nullptr, C, 0);
             }
             SPK_TRY(check_launch("k_gemm<double,plain>"));
           }
@@ -796,7 +853,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           ProfScope ps(ctx, "dual");
           hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sm, Li, ld,
                              strideM, sc, zact, ldz, idx, N, prob->stn_drift,
-                             m, method, V, 2 * ld, T0, nullptr);
+                             m, method, V, 2 * ld, T0, gb, mvg, fp64var_thr);
         }
         SPK_TRY(check_launch("k_dual"));
         if (prec == SPK_P_MIXED && want_var) {
@@ -809,24 +866,14 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           const int64_t cn = std::min<int64_t>(Cs, C - cb);
           const double *cd =
               prob->cell_drift ? prob->cell_drift + cb : nullptr;
-          if (prec == SPK_P_MIXED) {
-            ProfScope ps(ctx, "kgen_f32");
-            hipLaunchKernelGGL((k_kgen<float>),
-                               dim3((unsigned)ceil_div(cn, 256), Sb),
+          {
+            ProfScope ps(ctx, "kgen");
+            hipLaunchKernelGGL(k_kgen, dim3((unsigned)ceil_div(cn, 256), Sb),
                                dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc,
                                gb, cx + cb, cy + cb, cn, C, cd, V, 2 * ld, ld,
                                method, m, mvg, want_var ? 1 : 0,
-                               (float *)Kbuf, ldk, want_var ? abuf : nullptr,
-                               eb + cb, C);
-          } else {
-            ProfScope ps(ctx, "kgen_f64");
-            hipLaunchKernelGGL((k_kgen<double>),
-                               dim3((unsigned)ceil_div(cn, 256), Sb),
-                               dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc,
-                               gb, cx + cb, cy + cb, cn, C, cd, V, 2 * ld, ld,
-                               method, m, mvg, want_var ? 1 : 0,
-                               (double *)Kbuf, ldk, want_var ? abuf : nullptr,
-                               eb + cb, C);
+                               prec == SPK_P_MIXED ? 0 : 1, K32, K64, ldk,
+                               abuf, eb + cb, C);
           }
           SPK_TRY(check_launch("k_kgen"));
           if (want_var) {
@@ -835,19 +882,20 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
               hipLaunchKernelGGL((k_gemm<float, false>),
                                  dim3((unsigned)ceil_div(cn, 64), Sb),
                                  dim3(256), 0, sm, L32, ld, strideM,
-                                 (const float *)Kbuf, ldk, ld * ldk, sc, p,
+                                 (const float *)K32, ldk, ld * ldk, sc, p,
                                  method, 1, zact, ldz, (const int32_t *)nullptr,
                                  (int64_t)0, abuf, ldk, cn, eb + cb,
-                                 vbo + cb, C);
-            } else {
+                                 vbo + cb, C, 1);
+            }
+            {
               ProfScope ps(ctx, "gemm_tri_f64");
               hipLaunchKernelGGL((k_gemm<double, false>),
                                  dim3((unsigned)ceil_div(cn, 64), Sb),
                                  dim3(256), 0, sm, Li, ld, strideM,
-                                 (const double *)Kbuf, ldk, ld * ldk, sc, p,
+                                 (const double *)K64, ldk, ld * ldk, sc, p,
                                  method, 1, zact, ldz, (const int32_t *)nullptr,
                                  (int64_t)0, abuf, ldk, cn, eb + cb,
-                                 vbo + cb, C);
+                                 vbo + cb, C, prec == SPK_P_MIXED ? 2 : 0);
             }
             SPK_TRY(check_launch("k_gemm<tri>"));
           }

@@ -86,6 +86,8 @@ __global__ __launch_bounds__(256) void k_compact(
       if (!red_any[0] || nug) st = SPK_ST_MEANFILL;
     }
     o.status = st;
+    o.flags = 0;
+    o.kappa = 0.0;
     o.c0 = 0.0;
     o.b = 0.0;
     o.a0 = 0.0;
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(256) void k_dual(
     const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ stn_drift, int m, int method,
     double *__restrict__ V, int64_t ldv, double *__restrict__ T0,
-    double *__restrict__ T1) {
+    const spk_vg *__restrict__ vgs, double min_vg_val, double fp64var_thr) {
   const int64_t ldt = ld;
   const int s = blockIdx.x;
   if (sc[s].status) return;
@@ -537,6 +539,36 @@ __global__ __launch_bounds__(256) void k_dual(
   __syncthreads();
   tri_lower_tmv(L, ld, n, U, X, p + 1);  // X = C^-1 [F z]
   __syncthreads();
+  // condition estimate: max diag(C^-1) (squared column norms of Li) times
+  // the diagonal of C
+  {
+    __shared__ double red[4];
+    const int lane = tid & 63, wave = tid >> 6;
+    double mx = 0.0;
+    for (int k = wave; k < n; k += 4) {
+      double acc = 0.0;
+      for (int i = k + lane; i < n; i += 64) {
+        const double l = L[i + (int64_t)k * ld];
+        acc += l * l;
+      }
+      mx = fmax(mx, wave_sum(acc));
+    }
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      const double dinv = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      const spk_vg &vg = vgs[s];
+      const double cii = (method == SPK_M_SK)
+                             ? vg_entry(vg, 0.0, 1, min_vg_val)
+                             : sc[s].c0 - vg_entry(vg, 0.0, 0, min_vg_val);
+      const double kap = dinv * fabs(cii);
+      sc[s].kappa = kap;
+      int fl = 0;
+      if (!(kap <= 1e12)) fl |= SPK_ST_ILLCOND;
+      if (!(kap <= fp64var_thr)) fl |= SPK_ST_FP64VAR;
+      sc[s].flags = fl;
+    }
+  }
   __shared__ double G[(kMaxP + 1) * (kMaxP + 1)];
   // G[a][b] = U[:, a] . U[:, b]  (a, b <= p): B = U_F^T U_F, g = U_F^T U_z
   for (int e = tid >> 6; e < (p + 1) * (p + 1); e += 4) {
@@ -588,7 +620,8 @@ __global__ __launch_bounds__(256) void k_minv(
     const double *__restrict__ Li, int64_t ld, int64_t stride,
     StepSc *__restrict__ sc, const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ stn_drift, int m, int method,
-    double *__restrict__ M, double *__restrict__ T0) {
+    double *__restrict__ M, double *__restrict__ T0,
+    const spk_vg *__restrict__ vgs, double min_vg_val) {
   const int64_t ldt = ld;
   const int s = blockIdx.x;
   if (sc[s].status) return;
@@ -648,6 +681,27 @@ __global__ __launch_bounds__(256) void k_minv(
     }
   }
   __syncthreads();
+  {
+    __shared__ double red[256];
+    double mx = 0.0;
+    for (int i = tid; i < n; i += 256) mx = fmax(mx, Ms[i + (int64_t)i * ld]);
+    red[tid] = mx;
+    __syncthreads();
+    for (int w2 = 128; w2 > 0; w2 >>= 1) {
+      if (tid < w2) red[tid] = fmax(red[tid], red[tid + w2]);
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const spk_vg &vg = vgs[s];
+      const double cii = (method == SPK_M_SK)
+                             ? vg_entry(vg, 0.0, 1, min_vg_val)
+                             : sc[s].c0 - vg_entry(vg, 0.0, 0, min_vg_val);
+      const double kap = red[0] * fabs(cii);
+      sc[s].kappa = kap;
+      sc[s].flags = (kap <= 1e12) ? 0 : SPK_ST_ILLCOND;
+    }
+    __syncthreads();
+  }
   if (method == SPK_M_SK) return;
   const int p = (method == SPK_M_EDK) ? 1 + m : 1;
   // per-step work area: 2 (kMaxP + 1) columns of ld (W, then Q)

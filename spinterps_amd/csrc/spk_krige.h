@@ -10,7 +10,10 @@ constexpr int kMaxP = 8;  // constraint rows: 1 + up to 7 drifts
 // Per-step scalars kept on the device.
 struct StepSc {
   int n;          // active stations
-  int status;     // SPK_ST_* (0 = regular kriging / IDW step)
+  int status;     // SPK_ST_* that skip the step's regular kernels (0 = run)
+  int flags;      // informational SPK_ST_* bits (ill-conditioned, FP64 var)
+  int pad_;
+  double kappa;   // max diag(C^-1) * max diag(C), a condition estimate
   double mean;    // mean of the active values (interp/steps.py:276)
   double c0;      // covariance shift of the OK/EDK system
   double b;       // 1^T C^-1 1
@@ -34,11 +37,12 @@ __global__ void k_dual(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const double *zact, int64_t ldz,
                        const int32_t *idx, int64_t N, const double *stn_drift,
                        int m, int method, double *V, int64_t ldv, double *T0,
-                       double *T1);
+                       const spk_vg *vgs, double min_vg_val,
+                       double fp64var_thr);
 __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const int32_t *idx, int64_t N,
                        const double *stn_drift, int m, int method, double *M,
-                       double *T0);
+                       double *T0, const spk_vg *vgs, double min_vg_val);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
 

@@ -164,7 +164,7 @@ def test_ok_small_all_precisions(gpu, precision):
     est, var, info = interp_steps(prob, vals, vgs, 'OK', precision=precision)
     _, f64, _ = _oracle_fields(w, vals, vgs, [('OK', 'OK', None)])
     e, v = est.cpu().numpy(), var.cpu().numpy()
-    assert (info.cpu().numpy() == 0).all()
+    assert ((info.cpu().numpy() & 15) == 0).all()
     scale = np.nanmax(np.abs(f64['OK']))
     np.testing.assert_allclose(e, f64['OK'], rtol=1e-9, atol=1e-12 * scale)
     vref = f64['EST_VARS_OK']
