@@ -133,7 +133,8 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 #define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
 #define SPK_ST_ILLCOND  16   /* condition estimate > 1e12: results are set
                                 by rounding (as the reference's pinv is) */
-#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: condition estimate > 1e5, so the
+#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: condition estimate above 2e4
+                                (1e3 with a Gaussian component), so the
                                 variance contraction ran in FP64          */
 
 typedef struct spk_problem {
@@ -160,12 +161,14 @@ typedef struct spk_opts {
 
 /* Interpolate n_steps steps.  vals: device [n_steps x N], NaN = station
  * missing at that step.  vgs: device [n_steps] (kriging only).  est, var:
- * device [n_steps x C] float64 (var may be NULL).  info: device [n_steps].
+ * device [n_steps x C] float64 (var may be NULL).  info: device [n_steps]
+ * SPK_ST_* bits.  kappa: device [n_steps] condition estimate
+ * max diag(C^-1) * max diag(C) of each kriging system, or NULL.
  * Outputs of steps without stations are left untouched. */
 int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          const spk_opts *opts, int64_t n_steps,
                          const double *vals, const spk_vg *vgs, double *est,
-                         double *var, int32_t *info);
+                         double *var, int32_t *info, double *kappa);
 
 /* interp/steps.py:466-476 + the float32 cast of steps.py:815-831:
  * out32[i] = (float)clip(in64[i]) on device; NaN passes through.

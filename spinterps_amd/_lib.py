@@ -97,7 +97,8 @@ SIGNATURES = {
                                  c_int64, c_int64, c_int64, _PI64]),
     'spk_interp_steps_dev': (c_int, [c_void_p, POINTER(SpkProblem),
                                      POINTER(SpkOpts), c_int64, c_void_p,
-                                     c_void_p, c_void_p, c_void_p, c_void_p]),
+                                     c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
     'spk_clip_cast_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                   c_double, c_int, c_double]),
     'spk_last_timings': (c_int, [c_void_p, _PD, c_int, POINTER(c_int)]),

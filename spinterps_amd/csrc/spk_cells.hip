@@ -108,12 +108,27 @@ __global__ __launch_bounds__(256) void k_rhs_plain(
 }
 
 // ---- MFMA contraction --------------------------------------------------------
-// 256 threads = 4 waves; block tile 64 rows x 64 cells; wave (wr, wc) owns a
-// 32 x 32 sub-tile as 2 x 2 MFMA 16x16 tiles; k in chunks of 16.
-// A is column-major [P x P] (lda), B row-major [P x Cs] (ldb).
+// Block tile: 64 rows x 128 cells, k in chunks of BK, 256 threads = 4 waves;
+// wave w owns cells [32 w, 32 w + 32) of the tile over all 64 rows as 4 x 2
+// MFMA 16x16 tiles, so the per-cell row reductions of the epilogue need no
+// cross-wave step.  The next k chunk is loaded into registers while the
+// current one is multiplied out of LDS (two LDS buffers, one barrier per
+// chunk).  A is column-major [P x P] (lda), B row-major [P x ldb cells];
+// ldb is a multiple of 128 and every 128-cell tile lies inside it.
 // MFMA 16x16x4 operand layout: lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; the result row in slot r is Mfma<T>::row(l / 16, r).
-template <typename T, bool PLAIN>
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double2;  // 16-byte loads: two doubles
+};
+
+template <typename T, bool PLAIN, int BK>
 __global__ __launch_bounds__(256) void k_gemm(
     const T *__restrict__ A, int64_t lda, int64_t strideA,
     const T *__restrict__ B, int64_t ldb, int64_t strideB,
@@ -122,10 +137,29 @@ __global__ __launch_bounds__(256) void k_gemm(
     const int32_t *__restrict__ nnb, int64_t ldn,
     const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
     double *__restrict__ est, double *__restrict__ var, int64_t ldo,
-    int sel) {
+    int sel, double *__restrict__ qpart, int nrb) {
   using MF = Mfma<T>;
   using V = typename MF::V;
-  const int sl = blockIdx.y;
+  constexpr int BM = 64, BN = 128;
+  // PLAIN: grid (cell tiles, steps), each block walks all row blocks.
+  // TRI: grid (row blocks, cell tiles, steps), one row block per block;
+  // blocks on the same XCD (linear id mod 8) take consecutive row blocks of
+  // one cell tile so that the tile's K rows stay in that XCD's L2.
+  int rbi = 0, cti = blockIdx.x;
+  if (!PLAIN) {
+    const int G = gridDim.x * gridDim.y;
+    int lin = blockIdx.x + blockIdx.y * gridDim.x;
+    if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
+    rbi = lin % gridDim.x;
+    cti = lin / gridDim.x;
+  }
+  constexpr int VW = 16 / sizeof(T);         // elements per 16-byte load
+  constexpr int PAD = VW;
+  constexpr int A_LD = BM * BK / VW / 256;   // vector loads per thread
+  constexpr int B_LD = BN * BK / VW / 256;
+  static_assert(A_LD >= 1 && B_LD >= 1, "tile too small");
+  using VT = typename Vec4<T>::type;
+  const int sl = PLAIN ? blockIdx.y : blockIdx.z;
   const StepSc st = sc[sl];
   if (st.status) return;
   // sel 1: only steps without SPK_ST_FP64VAR, sel 2: only steps with it
@@ -133,64 +167,135 @@ __global__ __launch_bounds__(256) void k_gemm(
   if (sel == 2 && !(st.flags & SPK_ST_FP64VAR)) return;
   const int n = st.n;
   const int P = PLAIN ? n + p : n;
+  const int64_t c0 = cti * (int64_t)BN;
+  if (c0 >= Cs) return;
+  const int r_lo = PLAIN ? 0 : rbi * BM;
+  if (r_lo >= P) return;
+  const int r_hi = PLAIN ? P : min(P, r_lo + BM);
   A += (int64_t)sl * strideA;
   B += (int64_t)sl * strideB;
   const double *z = zact + (int64_t)sl * ldz;
-  constexpr int PAD = sizeof(T) == 8 ? 2 : 4;
-  __shared__ T As[16][64 + PAD];
-  __shared__ T Bs[16][64 + PAD];
-  __shared__ double red[4][2][16][4];
+  __shared__ __align__(16) T As[2][BK][BM + PAD];
+  __shared__ __align__(16) T Bs[2][BK][BN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int64_t c0 = blockIdx.x * (int64_t)64;
   const int li = lane & 15, lg = lane >> 4;
 
   double acc_e[2] = {0.0, 0.0}, acc_l[2] = {0.0, 0.0};
   double acc_v[2] = {0.0, 0.0}, acc_m[2] = {0.0, 0.0};
 
-  for (int r0 = 0; r0 < P; r0 += 64) {
-    V acc[2][2];
+  VT ra[A_LD], rb[B_LD];
+  // A tile element group q: rows r..r+VW-1 of column k
+  auto load_tiles = [&](int r0, int k0, int kend) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < A_LD; ++q) {
+      const int e = tid + q * 256;
+      const int r = (e % (BM / VW)) * VW, k = e / (BM / VW);
+      const int gk = k0 + k;
+      if (gk < kend) {
+        ra[q] = *reinterpret_cast<const VT *>(A + (r0 + r) +
+                                              (int64_t)gk * lda);
+        // rows beyond P are never used but may hold stale values
+        T *v = reinterpret_cast<T *>(&ra[q]);
+#pragma unroll
+        for (int u = 0; u < VW; ++u)
+          if (r0 + r + u >= P) v[u] = T(0);
+      } else {
+        ra[q] = VT{};
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_LD; ++q) {
+      const int e = tid + q * 256;
+      const int c = (e % (BN / VW)) * VW, k = e / (BN / VW);
+      const int gk = k0 + k;
+      if (gk < kend)
+        rb[q] = *reinterpret_cast<const VT *>(B + (int64_t)gk * ldb + c0 + c);
+      else
+        rb[q] = VT{};
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < A_LD; ++q) {
+      const int e = tid + q * 256;
+      const int r = (e % (BM / VW)) * VW, k = e / (BM / VW);
+      *reinterpret_cast<VT *>(&As[buf][k][r]) = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < B_LD; ++q) {
+      const int e = tid + q * 256;
+      const int c = (e % (BN / VW)) * VW, k = e / (BN / VW);
+      *reinterpret_cast<VT *>(&Bs[buf][k][c]) = rb[q];
+    }
+  };
+
+  // FP32 MFMA sums only BK products per chunk; chunk sums are added in FP64
+  constexpr bool ACC64 = sizeof(T) == 4;
+  for (int r0 = r_lo; r0 < r_hi; r0 += BM) {
+    V acc[4][2];
+    double acc64[ACC64 ? 4 : 1][2][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
-    const int kend = PLAIN ? P : min(P, r0 + 64);
-    for (int k0 = 0; k0 < kend; k0 += 16) {
+    if (ACC64) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = tid + q * 256;
-        const int r = e & 63, k = e >> 6;
-        const int gr = r0 + r, gk = k0 + k;
-        As[k][r] = (gr < P && gk < kend) ? A[gr + (int64_t)gk * lda] : T(0);
-        const int64_t gc = c0 + r;
-        Bs[k][r] = (gk < kend && gc < Cs) ? B[(int64_t)gk * ldb + gc] : T(0);
-      }
-      __syncthreads();
+      for (int a = 0; a < (ACC64 ? 4 : 1); ++a)
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+    }
+    const int kend = PLAIN ? P : min(P, r0 + BM);
+    const int nch = (kend + BK - 1) / BK;
+    load_tiles(r0, 0, kend);
+    __syncthreads();  // previous row block's readers are done with LDS
+    store_tiles(0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+      const int buf = ch & 1;
+      if (ch + 1 < nch) load_tiles(r0, (ch + 1) * BK, kend);
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; ++kk) {
         const int ka = kk * 4 + lg;
-        const T a0 = As[ka][wr * 32 + li];
-        const T a1 = As[ka][wr * 32 + 16 + li];
-        const T b0 = Bs[ka][wc * 32 + li];
-        const T b1 = Bs[ka][wc * 32 + 16 + li];
-        acc[0][0] = MF::op(a0, b0, acc[0][0]);
-        acc[0][1] = MF::op(a0, b1, acc[0][1]);
-        acc[1][0] = MF::op(a1, b0, acc[1][0]);
-        acc[1][1] = MF::op(a1, b1, acc[1][1]);
+        T af[4], bf[2];
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) af[ti] = As[buf][ka][ti * 16 + li];
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+          bf[tj] = Bs[buf][ka][wave * 32 + tj * 16 + li];
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < 2; ++tj)
+            acc[ti][tj] = MF::op(af[ti], bf[tj], acc[ti][tj]);
       }
+      if (ACC64) {
+#pragma unroll
+        for (int ti = 0; ti < (ACC64 ? 4 : 1); ++ti)
+#pragma unroll
+          for (int tj = 0; tj < 2; ++tj) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              acc64[ti][tj][r] += (double)acc[ti][tj][r];
+            acc[ti][tj] = V{0, 0, 0, 0};
+          }
+      }
+      if (ch + 1 < nch) store_tiles(buf ^ 1);
       __syncthreads();
     }
     // epilogue for this row block
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj) {
-      const int64_t col = c0 + wc * 32 + tj * 16 + li;
+      const int64_t col = c0 + wave * 32 + tj * 16 + li;
       if (col >= Cs) continue;
 #pragma unroll
-      for (int ti = 0; ti < 2; ++ti) {
+      for (int ti = 0; ti < 4; ++ti) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = r0 + wr * 32 + ti * 16 + MF::row(lg, r);
-          const double v = (double)acc[ti][tj][r];
+          const int row = r0 + ti * 16 + MF::row(lg, r);
+          const double v = ACC64 ? acc64[ACC64 ? ti : 0][tj][r]
+                                 : (double)acc[ti][tj][r];
           if (PLAIN) {
             if (row < n) {
               acc_e[tj] += v * z[row];
@@ -206,7 +311,7 @@ __global__ __launch_bounds__(256) void k_gemm(
       }
     }
   }
-  // reduce the four lane groups that share a column, then the two waves
+  // the four lane groups of a wave share each column
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj) {
     acc_e[tj] += __shfl_xor(acc_e[tj], 16, 64);
@@ -217,40 +322,46 @@ __global__ __launch_bounds__(256) void k_gemm(
     acc_v[tj] += __shfl_xor(acc_v[tj], 32, 64);
     acc_m[tj] += __shfl_xor(acc_m[tj], 16, 64);
     acc_m[tj] += __shfl_xor(acc_m[tj], 32, 64);
-    if (lg == 0) {
-      red[wave][tj][li][0] = acc_e[tj];
-      red[wave][tj][li][1] = acc_l[tj];
-      red[wave][tj][li][2] = acc_v[tj];
-      red[wave][tj][li][3] = acc_m[tj];
-    }
   }
-  __syncthreads();
-  if (tid < 64) {
-    const int wcc = tid >> 5, tj = (tid >> 4) & 1, l = tid & 15;
-    const int64_t col = c0 + wcc * 32 + tj * 16 + l;
-    if (col < Cs) {
-      const int w0 = wcc, w1 = 2 + wcc;  // wr = 0 / 1 with the same wc
-      const double e = red[w0][tj][l][0] + red[w1][tj][l][0];
-      const double ls = red[w0][tj][l][1] + red[w1][tj][l][1];
-      const double vs = red[w0][tj][l][2] + red[w1][tj][l][2];
-      const double mu = red[w0][tj][l][3] + red[w1][tj][l][3];
-      double *eo = est + (int64_t)sl * ldo;
-      double *vo = var ? var + (int64_t)sl * ldo : nullptr;
-      if (PLAIN) {
-        if (isclose1(ls)) {
-          eo[col] = e;
-          if (want_var) vo[col] = vs + mu;
-        } else {
-          eo[col] = z[nnb[(int64_t)sl * ldn + col]];
-          if (want_var) vo[col] = 0.0;
-        }
+  if (lg != 0) return;
+  double *eo = est + (int64_t)sl * ldo;
+  double *vo = var ? var + (int64_t)sl * ldo : nullptr;
+#pragma unroll
+  for (int tj = 0; tj < 2; ++tj) {
+    const int64_t col = c0 + wave * 32 + tj * 16 + li;
+    if (col >= Cs) continue;
+    if (PLAIN) {
+      if (isclose1(acc_l[tj])) {
+        eo[col] = acc_e[tj];
+        if (want_var) vo[col] = acc_v[tj] + acc_m[tj];
       } else {
-        // OK variance lambda.g + 2 mu (interp/steps.py:432-434):
-        // base (FP64 terms) minus the deflated quadratic form
-        vo[col] = abuf[(int64_t)sl * ldab + col] - vs;
+        eo[col] = z[nnb[(int64_t)sl * ldn + col]];
+        if (want_var) vo[col] = 0.0;
       }
+    } else {
+      // this row block's share of |L^-1 delta|^2
+      qpart[((int64_t)sl * nrb + rbi) * ldab + col] = acc_v[tj];
     }
   }
+}
+
+// OK variance lambda.g + 2 mu (interp/steps.py:432-434): the FP64 base
+// terms minus the deflated quadratic form, summed over row blocks in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_var_reduce(
+    const StepSc *__restrict__ sc, const double *__restrict__ abuf,
+    const double *__restrict__ qpart, int64_t ldq, int nrb, int64_t Cs,
+    double *__restrict__ var, int64_t ldo) {
+  const int sl = blockIdx.y;
+  const StepSc *st = sc + sl;
+  if (st->status) return;
+  const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (c >= Cs) return;
+  const int nb = (st->n + 63) / 64;
+  const double *q = qpart + (int64_t)sl * nrb * ldq + c;
+  double acc = 0.0;
+  for (int r = 0; r < nb; ++r) acc += q[(int64_t)r * ldq];
+  var[(int64_t)sl * ldo + c] = abuf[(int64_t)sl * ldq + c] - acc;
 }
 
 // ---- batched path: fused distance -> covariance -------------------------------
@@ -275,26 +386,26 @@ __global__ __launch_bounds__(256) void k_kgen(
     int64_t C, const double *__restrict__ cell_drift,
     const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
     double min_vg_val, int want_var, int all64, float *__restrict__ K32,
-    double *__restrict__ K64, int64_t ldk, double *__restrict__ abuf,
-    double *__restrict__ est, int64_t ldo) {
+    double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
+    double *__restrict__ abuf, double *__restrict__ est, int64_t ldo) {
   const int sl = blockIdx.y;
-  const StepSc st = sc[sl];
-  if (st.status) return;
-  const int n = st.n;
+  const StepSc *stp = sc + sl;
+  if (stp->status) return;
+  const int n = stp->n;
   const int32_t *my = idx + (int64_t)sl * N;
   const double *alpha = V + (int64_t)sl * ldv;
   const double *w = alpha + ld;
   __shared__ double stx[256], sty[256], sal[256], sw[256];
-  __shared__ spk_vg vg;
+  __shared__ VgFast vf;
   const int tid = threadIdx.x;
   const int64_t c = blockIdx.x * (int64_t)256 + tid;
   const bool valid = c < Cs;
   const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
-  if (tid == 0) vg = vgs[sl];
+  if (tid == 0) vg_fast_init(vgs[sl], vf);
   const int cflag = method == SPK_M_SK ? 1 : 0;
-  const double c0 = st.c0;
+  const double c0 = stp->c0;
   const bool var_on = want_var && method == SPK_M_OK;
-  const bool use64 = all64 || (st.flags & SPK_ST_FP64VAR);
+  const bool use64 = all64 || (stp->flags & SPK_ST_FP64VAR);
   float *K32s = (var_on && !use64) ? K32 + (int64_t)sl * ld * ldk : nullptr;
   double *K64s = (var_on && use64) ? K64 + (int64_t)sl * ld * ldk : nullptr;
   double best = INFINITY;
@@ -320,8 +431,9 @@ __global__ __launch_bounds__(256) void k_kgen(
         }
     }
   }
-  const double jx = var_on ? sx[my[bi]] : 0.0;
-  const double jy = var_on ? sy[my[bi]] : 0.0;
+  // column j(c) of the step's full semivariogram matrix
+  const double *gcol =
+      var_on ? Gfull + (int64_t)sl * ld * ld + (int64_t)bi * ld : nullptr;
   double e = 0.0, a = 0.0, g_jc = 0.0;
   for (int base = 0; base < n; base += 256) {
     __syncthreads();
@@ -335,25 +447,31 @@ __global__ __launch_bounds__(256) void k_kgen(
     __syncthreads();
     const int cnt = min(256, n - base);
     if (valid) {
-      for (int j = 0; j < cnt; ++j) {
-        const double d = dist2(x, y, stx[j], sty[j]);
-        if (!var_on && d < best) {
-          best = d;
-          bi = base + j;
-        }
-        const double g = vg_entry(vg, d, cflag, min_vg_val);
-        const double k = cflag ? g : c0 - g;
-        e += k * sal[j];
-        a += k * sw[j];
-        if (var_on) {
-          const double dj = dist2(stx[j], sty[j], jx, jy);
-          const double gij = vg_entry(vg, dj, 0, min_vg_val);
-          const double delta = gij - g;
+      if (var_on) {
+        for (int j = 0; j < cnt; ++j) {
+          const double d = dist2(x, y, stx[j], sty[j]);
+          const double g = vg_fast_entry(vf, d, 0, min_vg_val);
+          const double k = c0 - g;
+          e += k * sal[j];
+          a += k * sw[j];
+          const double delta = gcol[base + j] - g;
           if (base + j == bi) g_jc = g;
           if (K32s)
             K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
           else
             K64s[(int64_t)(base + j) * ldk + c] = delta;
+        }
+      } else {
+        for (int j = 0; j < cnt; ++j) {
+          const double d = dist2(x, y, stx[j], sty[j]);
+          if (d < best) {
+            best = d;
+            bi = base + j;
+          }
+          const double g = vg_fast_entry(vf, d, cflag, min_vg_val);
+          const double k = cflag ? g : c0 - g;
+          e += k * sal[j];
+          a += k * sw[j];
         }
       }
     }
@@ -362,15 +480,16 @@ __global__ __launch_bounds__(256) void k_kgen(
   double *eo = est + (int64_t)sl * ldo;
   const double *z = zact + (int64_t)sl * ldz;
   if (method == SPK_M_OK) {
-    eo[c] = e + st.a0;
+    eo[c] = e + stp->a0;
     if (var_on) {
-      const double G0 = vg_entry(vg, 0.0, 0, min_vg_val);
+      const double G0 = gcol[bi];  // G_jj: the semivariogram at lag 0
       abuf[(int64_t)sl * ldk + c] =
-          2.0 * g_jc - G0 + (a - 1.0) * (a - 2.0) / st.b;
+          2.0 * g_jc - G0 + (a - 1.0) * (a - 2.0) / stp->b;
     }
   } else if (method == SPK_M_EDK) {
-    double v = e + st.h[0];
-    for (int k = 0; k < m; ++k) v += st.h[1 + k] * cell_drift[(int64_t)k * C + c];
+    double v = e + stp->h[0];
+    for (int k = 0; k < m; ++k)
+      v += stp->h[1 + k] * cell_drift[(int64_t)k * C + c];
     eo[c] = v;
   } else {  // SK
     eo[c] = isclose1(a) ? e : z[bi];
@@ -511,9 +630,13 @@ __global__ __launch_bounds__(256) void k_special(
 }
 
 __global__ void k_status_out(const StepSc *__restrict__ sc, int S,
-                             int32_t *__restrict__ info) {
+                             int32_t *__restrict__ info,
+                             double *__restrict__ kappa) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < S) info[s] = sc[s].status | sc[s].flags;
+  if (s < S) {
+    info[s] = sc[s].status | sc[s].flags;
+    if (kappa) kappa[s] = sc[s].kappa;
+  }
 }
 
 __global__ void k_clip_cast(const double *__restrict__ in,
@@ -563,17 +686,19 @@ struct ProfScope {
   spk_ctx *ctx;
   Prof &pf;
   const char *name;
+  hipStream_t st;
   hipEvent_t a{}, b{};
-  ProfScope(spk_ctx *c, const char *nm) : ctx(c), pf(prof_of(c)), name(nm) {
+  ProfScope(spk_ctx *c, const char *nm, hipStream_t s = nullptr)
+      : ctx(c), pf(prof_of(c)), name(nm), st(s ? s : c->stream) {
     if (pf.on) {
       a = prof_event(pf);
-      hipEventRecord(a, ctx->stream);
+      hipEventRecord(a, st);
     }
   }
   ~ProfScope() {
     if (pf.on) {
       b = prof_event(pf);
-      hipEventRecord(b, ctx->stream);
+      hipEventRecord(b, st);
       pf.pending.push_back({name, a, b});
     }
   }
@@ -583,6 +708,7 @@ static int prof_collect(spk_ctx *ctx) {
   Prof &pf = prof_of(ctx);
   if (pf.pending.empty()) return SPK_OK;
   SPK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
   for (auto &r : pf.pending) {
     float ms = 0.f;
     SPK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
@@ -652,7 +778,7 @@ int spk_clip_cast_dev(spk_ctx *ctx, const double *in64, float *out32,
 int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          const spk_opts *opts, int64_t T, const double *vals,
                          const spk_vg *vgs, double *est, double *var,
-                         int32_t *info) {
+                         int32_t *info, double *kappa) {
   if (!ctx || !prob || !opts) return set_error(SPK_EARG, "null argument");
   if (T < 0) return set_error(SPK_ESHAPE, "negative step count");
   if (T == 0) return SPK_OK;
@@ -701,225 +827,302 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int64_t ldz = round_up(N, 64);
   const int p = method == SPK_M_OK ? 1 : method == SPK_M_EDK ? 1 + m : 0;
   const int64_t ld = round_up(N + (p > 0 ? p : 1), 64);
-
-  // batch size
-  int64_t S;
-  if (!krig) {
-    S = std::min<int64_t>(T, 512);
-  } else {
-    const int64_t per_step =
-        ld * ld * 8 * 2 + (prec == SPK_P_FP64 ? ld * ld * 8
-                                              : (prec == SPK_P_MIXED && want_var
-                                                     ? ld * ld * 4
-                                                     : 0));
-    const int64_t budget = (int64_t)6 << 30;
-    S = std::max<int64_t>(1, std::min<int64_t>(budget / per_step, 64));
-    S = std::min<int64_t>(S, T);
-  }
-  int32_t *idx;
-  double *zact;
-  StepSc *sc;
-  int *bad;
-  SPK_TRY(ws_get(ctx, "idx", S * N * sizeof(int32_t), (void **)&idx));
-  SPK_TRY(ws_get(ctx, "zact", S * ldz * sizeof(double), (void **)&zact));
-  SPK_TRY(ws_get(ctx, "stepsc", S * sizeof(StepSc), (void **)&sc));
-  SPK_TRY(ws_get(ctx, "bad", 64, (void **)&bad));
-  SPK_HIP(hipMemsetAsync(bad, 0, sizeof(int), sm));
+  const int64_t strideM = ld * ld;
   const double thr = opts->min_var_thr;
   const double mvg = opts->min_vg_val;
+  int *bad;
+  SPK_TRY(ws_get(ctx, "bad", 64, (void **)&bad));
+  SPK_HIP(hipMemsetAsync(bad, 0, sizeof(int), sm));
 
-  // work buffers for kriging
-  double *A = nullptr, *Li = nullptr, *M = nullptr, *V = nullptr;
-  double *T0 = nullptr;
-  float *L32 = nullptr;
-  const int64_t strideM = ld * ld;
-  if (krig) {
-    SPK_TRY(ws_get(ctx, "A", S * strideM * 8, (void **)&A));
-    SPK_TRY(ws_get(ctx, "Li", S * strideM * 8, (void **)&Li));
-    SPK_TRY(ws_get(ctx, "T0", S * ld * 2 * (kMaxP + 1) * 8, (void **)&T0));
-    if (prec == SPK_P_FP64) {
-      SPK_TRY(ws_get(ctx, "M", S * strideM * 8, (void **)&M));
-    } else {
-      SPK_TRY(ws_get(ctx, "V", S * 2 * ld * 8, (void **)&V));
-      if (prec == SPK_P_MIXED && want_var)
-        SPK_TRY(ws_get(ctx, "L32", S * strideM * 4, (void **)&L32));
-    }
-  }
-  // cell stripes for the materialised right-hand sides / covariances
-  int64_t Cs = C;
-  const size_t rhs_budget = (size_t)4 << 30;
-  if (krig && prec == SPK_P_FP64) {
-    Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
-  } else if (krig && want_var) {
-    const int64_t eb = prec == SPK_P_MIXED ? 12 : 8;
-    Cs = std::min<int64_t>(C, (int64_t)(2 * rhs_budget / (S * ld * eb)));
-  }
-  Cs = std::max<int64_t>(64, Cs / 64 * 64);
-  if (Cs > C) Cs = C;
-  const int64_t ldk = round_up(Cs, 64);
-  double *R = nullptr, *abuf = nullptr, *K64 = nullptr;
-  float *K32 = nullptr;
-  int32_t *nnb = nullptr;
-  if (krig && prec == SPK_P_FP64) {
-    SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
-    SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
-  } else if (krig && want_var) {
-    // FP64 buffer for the batched FP64 path and for the steps the mixed
-    // path promotes to FP64 (SPK_ST_FP64VAR)
-    SPK_TRY(ws_get(ctx, "K64", S * ld * ldk * 8, (void **)&K64));
-    if (prec == SPK_P_MIXED)
-      SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
-    SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
-  }
-  // SPK_P_MIXED: condition estimate above which a step's variance runs in
-  // FP64; the batched FP64 path runs every step in FP64
-  const double fp64var_thr = prec == SPK_P_MIXED ? 1e5 : INFINITY;
-
-  for (int64_t t0 = 0; t0 < T; t0 += S) {
-    const int Sb = (int)std::min<int64_t>(S, T - t0);
-    const double *vb = vals + t0 * N;
-    const spk_vg *gb = krig ? vgs + t0 : nullptr;
-    double *eb = est + t0 * C;
-    double *vbo = want_var ? var + t0 * C : nullptr;
-    {
-      ProfScope ps(ctx, "compact");
-      hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm, vb, N, idx,
-                         zact, ldz, sc, gb, method, thr, mvg);
-    }
-    SPK_TRY(check_launch("k_compact"));
-    if (!krig) {
-      ProfScope ps(ctx, "idw_nnb");
-      hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
-                         dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
-                         cy, C, method, opts->idw_exp, eb, bad);
-      SPK_TRY(check_launch("k_idw_nnb"));
-    } else {
-      const int tiles = (int)ceil_div(N, 64);
+  // ---- IDW / nearest neighbour: no systems ---------------------------------
+  if (!krig) {
+    const int64_t S = std::min<int64_t>(T, 512);
+    int32_t *idx;
+    double *zact;
+    StepSc *sc;
+    SPK_TRY(ws_get(ctx, "idx0", S * N * sizeof(int32_t), (void **)&idx));
+    SPK_TRY(ws_get(ctx, "zact0", S * ldz * sizeof(double), (void **)&zact));
+    SPK_TRY(ws_get(ctx, "sc0", S * sizeof(StepSc), (void **)&sc));
+    for (int64_t t0 = 0; t0 < T; t0 += S) {
+      const int Sb = (int)std::min<int64_t>(S, T - t0);
+      double *eb = est + t0 * C;
+      hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm, vals + t0 * N,
+                         N, idx, zact, ldz, sc, (const spk_vg *)nullptr,
+                         method, thr, mvg);
       {
-        ProfScope ps(ctx, "assemble");
-        hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles, Sb), dim3(256), 0,
-                           sm, sx, sy, idx, N, sc, gb,
-                           method == SPK_M_SK ? 1 : 0, mvg, A, ld, strideM);
+        ProfScope ps(ctx, "idw_nnb");
+        hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
+                           dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
+                           cy, C, method, opts->idw_exp, eb, bad);
       }
-      SPK_TRY(check_launch("k_assemble"));
-      {
-        ProfScope ps(ctx, "potrf");
-        hipLaunchKernelGGL(k_potrf, dim3(Sb), dim3(256), 0, sm, A, ld, strideM,
-                           sc, gb, method == SPK_M_SK ? 0 : 1);
-      }
-      SPK_TRY(check_launch("k_potrf"));
-      {
-        ProfScope ps(ctx, "trtri");
-        hipLaunchKernelGGL(k_trtri, dim3(Sb), dim3(256), 0, sm, A, Li, ld,
-                           strideM, sc);
-      }
-      SPK_TRY(check_launch("k_trtri"));
-      if (prec == SPK_P_FP64) {
-        {
-          ProfScope ps(ctx, "minv");
-          hipLaunchKernelGGL(k_minv, dim3(Sb), dim3(256), 0, sm, Li, ld,
-                             strideM, sc, idx, N, prob->stn_drift, m, method,
-                             M, T0, gb, mvg);
-        }
-        SPK_TRY(check_launch("k_minv"));
-        for (int s = 0; s < Sb; ++s) {
-          for (int64_t cb = 0; cb < C; cb += Cs) {
-            const int64_t cn = std::min<int64_t>(Cs, C - cb);
-            {
-              ProfScope ps(ctx, "rhs_plain");
-              hipLaunchKernelGGL(
-                  k_rhs_plain, dim3((unsigned)ceil_div(cn, 256)), dim3(256), 0,
-                  sm, sx, sy, idx + s * N, sc + s, gb + s, cx + cb, cy + cb, cn,
-                  C, prob->cell_drift ? prob->cell_drift + cb : nullptr,
-                  method, m, method == SPK_M_SK ? 1 : 0, mvg, R, ldk, nnb);
-            }
-            SPK_TRY(check_launch("k_rhs_plain"));
-            {
-              ProfScope ps(ctx, "gemm_plain_f64");
-              hipLaunchKernelGGL((k_gemm<double, true>),
-                                 dim3((unsigned)ceil_div(cn, 64), 1),
-                                 dim3(256), 0, sm, M + s * strideM, ld,
-                                 (int64_t)0, R, ldk, (int64_t)0, sc + s, p,
-                                 method, want_var ? 1 : 0, zact + s * ldz, ldz,
-                                 nnb, ldk, (const double *)nullptr,
-                                 (int64_t)0, cn, eb + s * C + cb,
-                                 vbo ? vbo + s * C + cb : nullptr, C, 0);
-            }
-            SPK_TRY(check_launch("k_gemm<double,plain>"));
-          }
-        }
-      } else {
-        {
-          ProfScope ps(ctx, "dual");
-          hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sm, Li, ld,
-                             strideM, sc, zact, ldz, idx, N, prob->stn_drift,
-                             m, method, V, 2 * ld, T0, gb, mvg, fp64var_thr);
-        }
-        SPK_TRY(check_launch("k_dual"));
-        if (prec == SPK_P_MIXED && want_var) {
-          ProfScope ps(ctx, "cast_tri");
-          hipLaunchKernelGGL(k_cast_tri, dim3(1024, Sb), dim3(256), 0, sm, Li,
-                             L32, ld, strideM, sc);
-        }
-        SPK_TRY(check_launch("k_cast_tri"));
-        for (int64_t cb = 0; cb < C; cb += Cs) {
-          const int64_t cn = std::min<int64_t>(Cs, C - cb);
-          const double *cd =
-              prob->cell_drift ? prob->cell_drift + cb : nullptr;
-          {
-            ProfScope ps(ctx, "kgen");
-            hipLaunchKernelGGL(k_kgen, dim3((unsigned)ceil_div(cn, 256), Sb),
-                               dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc,
-                               gb, cx + cb, cy + cb, cn, C, cd, V, 2 * ld, ld,
-                               method, m, mvg, want_var ? 1 : 0,
-                               prec == SPK_P_MIXED ? 0 : 1, K32, K64, ldk,
-                               abuf, eb + cb, C);
-          }
-          SPK_TRY(check_launch("k_kgen"));
-          if (want_var) {
-            if (prec == SPK_P_MIXED) {
-              ProfScope ps(ctx, "gemm_tri_f32");
-              hipLaunchKernelGGL((k_gemm<float, false>),
-                                 dim3((unsigned)ceil_div(cn, 64), Sb),
-                                 dim3(256), 0, sm, L32, ld, strideM,
-                                 (const float *)K32, ldk, ld * ldk, sc, p,
-                                 method, 1, zact, ldz, (const int32_t *)nullptr,
-                                 (int64_t)0, abuf, ldk, cn, eb + cb,
-                                 vbo + cb, C, 1);
-            }
-            {
-              ProfScope ps(ctx, "gemm_tri_f64");
-              hipLaunchKernelGGL((k_gemm<double, false>),
-                                 dim3((unsigned)ceil_div(cn, 64), Sb),
-                                 dim3(256), 0, sm, Li, ld, strideM,
-                                 (const double *)K64, ldk, ld * ldk, sc, p,
-                                 method, 1, zact, ldz, (const int32_t *)nullptr,
-                                 (int64_t)0, abuf, ldk, cn, eb + cb,
-                                 vbo + cb, C, prec == SPK_P_MIXED ? 2 : 0);
-            }
-            SPK_TRY(check_launch("k_gemm<tri>"));
-          }
-        }
-      }
-    }
-    {
-      ProfScope ps(ctx, "special");
       hipLaunchKernelGGL(k_special, dim3((unsigned)ceil_div(C, 256), Sb),
                          dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
-                         cy, C, eb, vbo);
+                         cy, C, eb, (double *)nullptr);
+      hipLaunchKernelGGL(k_status_out, dim3((unsigned)ceil_div(Sb, 256)),
+                         dim3(256), 0, sm, sc, Sb, info + t0,
+                         kappa ? kappa + t0 : nullptr);
+      SPK_TRY(check_launch("idw/nnb"));
     }
-    SPK_TRY(check_launch("k_special"));
-    hipLaunchKernelGGL(k_status_out, dim3((unsigned)ceil_div(Sb, 256)),
-                       dim3(256), 0, sm, sc, Sb, info + t0);
-    SPK_TRY(check_launch("k_status_out"));
-  }
-  if (!krig) {
     int hbad = 0;
     SPK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, sm));
     SPK_HIP(hipStreamSynchronize(sm));
     if (hbad)
       return set_error(SPK_EARG, "IDW weight sum below 1e-14 "
                        "(interp/steps.py:305)");
+    return SPK_OK;
+  }
+
+  // ---- kriging --------------------------------------------------------------
+  // Batches of S steps.  The systems of batch b+1 are factorised on a second
+  // stream while the cells of batch b are contracted on the caller's stream;
+  // the per-batch system workspaces are double-buffered.
+  const bool batched = prec != SPK_P_FP64;
+  const int64_t per_step =
+      strideM * 8 * 2 +
+      (prec == SPK_P_FP64 ? strideM * 8
+                          : (want_var ? strideM * 8 : 0) +
+                                (prec == SPK_P_MIXED && want_var ? strideM * 4
+                                                                 : 0));
+  const int64_t budget = (int64_t)6 << 30;
+  int64_t S = std::max<int64_t>(1, std::min<int64_t>(budget / per_step, 64));
+  S = std::min<int64_t>(S, T);
+  const int nsets = T > S ? 2 : 1;
+  struct Set {
+    int32_t *idx;
+    double *zact, *A, *Li, *M, *V, *T0, *Gm;
+    float *L32;
+    StepSc *sc;
+  } set[2] = {};
+  for (int k = 0; k < nsets; ++k) {
+    char nm[32];
+    Set &q = set[k];
+    auto get = [&](const char *base, size_t bytes, void **out) {
+      snprintf(nm, sizeof(nm), "%s%d", base, k);
+      return ws_get(ctx, nm, bytes, out);
+    };
+    SPK_TRY(get("idx", S * N * sizeof(int32_t), (void **)&q.idx));
+    SPK_TRY(get("zact", S * ldz * sizeof(double), (void **)&q.zact));
+    SPK_TRY(get("sc", S * sizeof(StepSc), (void **)&q.sc));
+    SPK_TRY(get("A", S * strideM * 8, (void **)&q.A));
+    SPK_TRY(get("Li", S * strideM * 8, (void **)&q.Li));
+    SPK_TRY(get("T0", S * ld * 2 * (kMaxP + 1) * 8, (void **)&q.T0));
+    if (!batched) {
+      SPK_TRY(get("M", S * strideM * 8, (void **)&q.M));
+    } else {
+      SPK_TRY(get("V", S * 2 * ld * 8, (void **)&q.V));
+      if (want_var) SPK_TRY(get("Gm", S * strideM * 8, (void **)&q.Gm));
+      if (prec == SPK_P_MIXED && want_var)
+        SPK_TRY(get("L32", S * strideM * 4, (void **)&q.L32));
+    }
+  }
+  // cell stripes for the materialised right-hand sides / deflated covariances
+  int64_t Cs = C;
+  const size_t rhs_budget = (size_t)4 << 30;
+  if (!batched) {
+    Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
+  } else if (want_var) {
+    const int64_t eb = prec == SPK_P_MIXED ? 12 : 8;
+    Cs = std::min<int64_t>(C, (int64_t)(2 * rhs_budget / (S * ld * eb)));
+  }
+  Cs = std::max<int64_t>(128, Cs / 128 * 128);
+  if (Cs > C) Cs = C;
+  const int64_t ldk = round_up(Cs, 128);
+  const int nrb = (int)ceil_div(N, 64);
+  double *R = nullptr, *abuf = nullptr, *K64 = nullptr, *qpart = nullptr;
+  float *K32 = nullptr;
+  int32_t *nnb = nullptr;
+  if (!batched) {
+    SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
+    SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
+  } else if (want_var) {
+    // FP64 buffer for the batched FP64 path and for the steps the mixed
+    // path promotes to FP64 (SPK_ST_FP64VAR)
+    SPK_TRY(ws_get(ctx, "K64", S * ld * ldk * 8, (void **)&K64));
+    if (prec == SPK_P_MIXED)
+      SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
+    SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
+    SPK_TRY(ws_get(ctx, "qpart", S * nrb * ldk * 8, (void **)&qpart));
+  }
+  // SPK_P_MIXED: condition estimates above which a step's variance runs in
+  // FP64 (systems with a Gaussian component lose accuracy in FP32 much
+  // sooner); the batched FP64 path runs every step in FP64
+  const double fp64var_thr = prec == SPK_P_MIXED ? 2e4 : INFINITY;
+  const double fp64var_thr_gau = prec == SPK_P_MIXED ? 1e3 : INFINITY;
+
+  // second stream + events for the overlap
+  if (!ctx->aux_stream)
+    SPK_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+  for (int k = 0; k < 4; ++k)
+    if (!ctx->ev[k])
+      SPK_HIP(hipEventCreateWithFlags(&ctx->ev[k], hipEventDisableTiming));
+  hipStream_t sf = nsets > 1 ? ctx->aux_stream : sm;
+  hipEvent_t ev_fact[2] = {ctx->ev[0], ctx->ev[1]};
+  hipEvent_t ev_done[2] = {ctx->ev[2], ctx->ev[3]};
+  // the factor stream must see the re-centred coordinates
+  if (sf != sm) {
+    SPK_HIP(hipEventRecord(ev_done[0], sm));
+    SPK_HIP(hipStreamWaitEvent(sf, ev_done[0], 0));
+  }
+
+  auto factorise = [&](int64_t t0, int Sb, Set &q) -> int {
+    const spk_vg *gb = vgs + t0;
+    hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sf, vals + t0 * N,
+                       N, q.idx, q.zact, ldz, q.sc, gb, method, thr, mvg);
+    SPK_TRY(check_launch("k_compact"));
+    const int tiles = (int)ceil_div(N, 64);
+    {
+      ProfScope ps(ctx, "assemble", sf);
+      hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles, Sb), dim3(256), 0, sf,
+                         sx, sy, q.idx, N, q.sc, gb,
+                         method == SPK_M_SK ? 1 : 0, mvg, q.A, ld, strideM,
+                         q.Gm);
+    }
+    SPK_TRY(check_launch("k_assemble"));
+    {
+      ProfScope ps(ctx, "potrf", sf);
+      hipLaunchKernelGGL(k_potrf, dim3(Sb), dim3(256), 0, sf, q.A, ld, strideM,
+                         q.sc, gb, method == SPK_M_SK ? 0 : 1);
+    }
+    SPK_TRY(check_launch("k_potrf"));
+    {
+      ProfScope ps(ctx, "trtri", sf);
+      hipLaunchKernelGGL(k_trtri, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
+                         strideM, q.sc);
+    }
+    SPK_TRY(check_launch("k_trtri"));
+    if (!batched) {
+      ProfScope ps(ctx, "minv", sf);
+      hipLaunchKernelGGL(k_minv, dim3(Sb), dim3(256), 0, sf, q.Li, ld, strideM,
+                         q.sc, q.idx, N, prob->stn_drift, m, method, q.M, q.T0,
+                         gb, mvg);
+    } else {
+      {
+        ProfScope ps(ctx, "dual", sf);
+        hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
+                           strideM, q.sc, q.zact, ldz, q.idx, N,
+                           prob->stn_drift, m, method, q.V, 2 * ld, q.T0, gb,
+                           mvg, fp64var_thr, fp64var_thr_gau);
+      }
+      if (q.L32) {
+        ProfScope ps(ctx, "cast_tri", sf);
+        hipLaunchKernelGGL(k_cast_tri, dim3(1024, Sb), dim3(256), 0, sf, q.Li,
+                           q.L32, ld, strideM, q.sc);
+      }
+    }
+    SPK_TRY(check_launch("factorise"));
+    return SPK_OK;
+  };
+
+  auto contract = [&](int64_t t0, int Sb, Set &q) -> int {
+    const spk_vg *gb = vgs + t0;
+    double *eb = est + t0 * C;
+    double *vbo = want_var ? var + t0 * C : nullptr;
+    if (!batched) {
+      for (int s = 0; s < Sb; ++s) {
+        for (int64_t cb = 0; cb < C; cb += Cs) {
+          const int64_t cn = std::min<int64_t>(Cs, C - cb);
+          {
+            ProfScope ps(ctx, "rhs_plain");
+            hipLaunchKernelGGL(
+                k_rhs_plain, dim3((unsigned)ceil_div(cn, 256)), dim3(256), 0,
+                sm, sx, sy, q.idx + s * N, q.sc + s, gb + s, cx + cb, cy + cb,
+                cn, C, prob->cell_drift ? prob->cell_drift + cb : nullptr,
+                method, m, method == SPK_M_SK ? 1 : 0, mvg, R, ldk, nnb);
+          }
+          SPK_TRY(check_launch("k_rhs_plain"));
+          {
+            ProfScope ps(ctx, "gemm_plain_f64");
+            hipLaunchKernelGGL((k_gemm<double, true, 16>),
+                               dim3((unsigned)ceil_div(cn, 128), 1), dim3(256),
+                               0, sm, q.M + s * strideM, ld, (int64_t)0, R, ldk,
+                               (int64_t)0, q.sc + s, p, method,
+                               want_var ? 1 : 0, q.zact + s * ldz, ldz, nnb,
+                               ldk, (const double *)nullptr, (int64_t)0, cn,
+                               eb + s * C + cb,
+                               vbo ? vbo + s * C + cb : nullptr, C, 0,
+                               (double *)nullptr, 0);
+          }
+          SPK_TRY(check_launch("k_gemm<double,plain>"));
+        }
+      }
+    } else {
+      for (int64_t cb = 0; cb < C; cb += Cs) {
+        const int64_t cn = std::min<int64_t>(Cs, C - cb);
+        const double *cd = prob->cell_drift ? prob->cell_drift + cb : nullptr;
+        {
+          ProfScope ps(ctx, "kgen");
+          hipLaunchKernelGGL(k_kgen, dim3((unsigned)ceil_div(cn, 256), Sb),
+                             dim3(256), 0, sm, sx, sy, q.idx, N, q.zact, ldz,
+                             q.sc, gb, cx + cb, cy + cb, cn, C, cd, q.V, 2 * ld,
+                             ld, method, m, mvg, want_var ? 1 : 0,
+                             prec == SPK_P_MIXED ? 0 : 1, K32, K64, ldk, q.Gm,
+                             abuf, eb + cb, C);
+        }
+        SPK_TRY(check_launch("k_kgen"));
+        if (!want_var) continue;
+        const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
+        if (prec == SPK_P_MIXED) {
+          ProfScope ps(ctx, "gemm_tri_f32");
+          hipLaunchKernelGGL((k_gemm<float, false, 32>), grid, dim3(256), 0, sm,
+                             q.L32, ld, strideM, (const float *)K32, ldk,
+                             ld * ldk, q.sc, p, method, 1, q.zact, ldz,
+                             (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
+                             cn, eb + cb, vbo + cb, C, 1, qpart, nrb);
+        }
+        {
+          ProfScope ps(ctx, "gemm_tri_f64");
+          hipLaunchKernelGGL((k_gemm<double, false, 16>), grid, dim3(256), 0,
+                             sm, q.Li, ld, strideM, (const double *)K64, ldk,
+                             ld * ldk, q.sc, p, method, 1, q.zact, ldz,
+                             (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
+                             cn, eb + cb, vbo + cb, C,
+                             prec == SPK_P_MIXED ? 2 : 0, qpart, nrb);
+        }
+        SPK_TRY(check_launch("k_gemm<tri>"));
+        {
+          ProfScope ps(ctx, "var_reduce");
+          hipLaunchKernelGGL(k_var_reduce, dim3((unsigned)ceil_div(cn, 256), Sb),
+                             dim3(256), 0, sm, q.sc, abuf, qpart, ldk, nrb, cn,
+                             vbo + cb, C);
+        }
+        SPK_TRY(check_launch("k_var_reduce"));
+      }
+    }
+    {
+      ProfScope ps(ctx, "special");
+      hipLaunchKernelGGL(k_special, dim3((unsigned)ceil_div(C, 256), Sb),
+                         dim3(256), 0, sm, sx, sy, q.idx, N, q.zact, ldz, q.sc,
+                         cx, cy, C, eb, vbo);
+    }
+    SPK_TRY(check_launch("k_special"));
+    hipLaunchKernelGGL(k_status_out, dim3((unsigned)ceil_div(Sb, 256)),
+                       dim3(256), 0, sm, q.sc, Sb, info + t0,
+                       kappa ? kappa + t0 : nullptr);
+    SPK_TRY(check_launch("k_status_out"));
+    return SPK_OK;
+  };
+
+  const int64_t nb = ceil_div(T, S);
+  auto bsize = [&](int64_t b) { return (int)std::min<int64_t>(S, T - b * S); };
+  SPK_TRY(factorise(0, bsize(0), set[0]));
+  if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[0], sf));
+  for (int64_t b = 0; b < nb; ++b) {
+    const int k = (int)(b & 1);
+    if (b + 1 < nb) {
+      const int k2 = k ^ 1;
+      // set k2 was last read by the contraction of batch b - 1
+      if (b >= 1 && sf != sm) SPK_HIP(hipStreamWaitEvent(sf, ev_done[k2], 0));
+      SPK_TRY(factorise((b + 1) * S, bsize(b + 1), set[nsets > 1 ? k2 : 0]));
+      if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[k2], sf));
+    }
+    if (sf != sm) SPK_HIP(hipStreamWaitEvent(sm, ev_fact[k], 0));
+    SPK_TRY(contract(b * S, bsize(b), set[nsets > 1 ? k : 0]));
+    if (sf != sm) SPK_HIP(hipEventRecord(ev_done[k], sm));
+  }
+  // nothing may remain queued on the factor stream after return
+  if (sf != sm) {
+    SPK_HIP(hipEventRecord(ev_fact[0], sf));
+    SPK_HIP(hipStreamWaitEvent(sm, ev_fact[0], 0));
   }
   return SPK_OK;
 }

@@ -36,8 +36,9 @@ int ws_get(spk_ctx *ctx, const char *name, size_t bytes, void **out) {
         *out = kv.second.ptr;
         return SPK_OK;
       }
-      // The stream may still use the old buffer.
+      // The streams may still use the old buffer.
       SPK_HIP(hipStreamSynchronize(ctx->stream));
+      if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
       SPK_HIP(hipFree(kv.second.ptr));
       kv.second.ptr = nullptr;
       kv.second.bytes = 0;
@@ -347,6 +348,7 @@ int spk_ctx_trim(spk_ctx *ctx) {
   if (!ctx) return set_error(SPK_EARG, "null ctx");
   SPK_HIP(hipSetDevice(ctx->device));
   SPK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
   for (auto &kv : ctx->bufs)
     if (kv.second.ptr) SPK_HIP(hipFree(kv.second.ptr));
   ctx->bufs.clear();
@@ -357,6 +359,9 @@ int spk_ctx_destroy(spk_ctx *ctx) {
   if (!ctx) return SPK_OK;
   int rc = spk_ctx_trim(ctx);
   for (auto ev : ctx->events) hipEventDestroy(ev);
+  for (auto ev : ctx->ev)
+    if (ev) hipEventDestroy(ev);
+  if (ctx->aux_stream) hipStreamDestroy(ctx->aux_stream);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return rc;

@@ -103,7 +103,8 @@ __global__ __launch_bounds__(256) void k_assemble(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N, const StepSc *__restrict__ sc,
     const spk_vg *__restrict__ vgs, int covar_flag, double min_vg_val,
-    double *__restrict__ A, int64_t ld, int64_t strideA) {
+    double *__restrict__ A, int64_t ld, int64_t strideA,
+    double *__restrict__ Gfull) {
   const int s = blockIdx.y;
   if (sc[s].status) return;
   const int n = sc[s].n;
@@ -136,6 +137,11 @@ __global__ __launch_bounds__(256) void k_assemble(
     const double v = vg_entry(vg, d, covar_flag, min_vg_val);
     As[i + (int64_t)j * ld] = v;
     As[j + (int64_t)i * ld] = v;
+    if (Gfull) {
+      double *Gs = Gfull + (int64_t)s * strideA;
+      Gs[i + (int64_t)j * ld] = v;
+      Gs[j + (int64_t)i * ld] = v;
+    }
   }
 }
 
@@ -443,6 +449,15 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Sum over the 256-thread block (all threads receive the result).
+__device__ inline double block_sum(double v, double *red4) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red4[0] + red4[1] + red4[2] + red4[3];
+}
+
 __device__ void small_inverse(double *B, int p, double *Binv) {
   // Gauss-Jordan with partial pivoting on a p x p (p <= kMaxP) row-major.
   double a[kMaxP][2 * kMaxP];
@@ -506,12 +521,14 @@ __device__ void tri_lower_tmv(const double *Li, int64_t ld, int n,
 
 // Work array T0: [ld x 2 (kMaxP+1)] per step.
 __global__ __launch_bounds__(256) void k_dual(
-    const double *__restrict__ Li, int64_t ld, int64_t stride,
+    const double *__restrict__ Lc, const double *__restrict__ Li, int64_t ld,
+    int64_t stride,
     StepSc *__restrict__ sc, const double *__restrict__ zact, int64_t ldz,
     const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ stn_drift, int m, int method,
     double *__restrict__ V, int64_t ldv, double *__restrict__ T0,
-    const spk_vg *__restrict__ vgs, double min_vg_val, double fp64var_thr) {
+    const spk_vg *__restrict__ vgs, double min_vg_val, double fp64var_thr,
+    double fp64var_thr_gau) {
   const int64_t ldt = ld;
   const int s = blockIdx.x;
   if (sc[s].status) return;
@@ -526,6 +543,59 @@ __global__ __launch_bounds__(256) void k_dual(
   double *w = alpha + ldt;
   const int p = (method == SPK_M_EDK) ? 1 + m : 1;
   const int tid = threadIdx.x;
+  // Condition estimate kappa = lambda_max(C) lambda_max(C^-1) by power
+  // iteration on C = L L^T and C^-1 = Li^T Li (Rayleigh quotients, 12
+  // rounds each; X and U serve as scratch before their real use below).
+  {
+    __shared__ double red4[4];
+    double lam[2] = {0.0, 0.0};
+    for (int which = 0; which < 2; ++which) {
+      const double *T = which == 0 ? Lc + (int64_t)s * stride : L;
+      for (int i = tid; i < n; i += 256) {
+        // fixed start vector with both smooth and oscillating parts
+        const unsigned h = (unsigned)i * 2654435761u;
+        X[i] = 1.0 + 0.5 * (double)((h >> 8) & 0xffff) / 65536.0 -
+               (which ? 0.5 * (double)(i & 1) : 0.0);
+      }
+      __syncthreads();
+      for (int it = 0; it < 12; ++it) {
+        double nv = 0.0;
+        for (int i = tid; i < n; i += 256) nv += X[i] * X[i];
+        nv = block_sum(nv, red4);
+        const double inv = 1.0 / sqrt(nv);
+        for (int i = tid; i < n; i += 256) X[i] *= inv;
+        __syncthreads();
+        if (which == 0) {
+          tri_lower_tmv(T, ld, n, X, U, 1);  // U = L^T x
+          __syncthreads();
+          double q = 0.0;
+          for (int i = tid; i < n; i += 256) q += U[i] * U[i];
+          lam[0] = block_sum(q, red4);  // x^T C x
+          tri_lower_mv(T, ld, n, U, X, 1);  // x = L L^T x
+        } else {
+          tri_lower_mv(T, ld, n, X, U, 1);  // U = Li x
+          __syncthreads();
+          double q = 0.0;
+          for (int i = tid; i < n; i += 256) q += U[i] * U[i];
+          lam[1] = block_sum(q, red4);  // x^T C^-1 x
+          tri_lower_tmv(T, ld, n, U, X, 1);  // x = Li^T Li x
+        }
+        __syncthreads();
+      }
+    }
+    if (tid == 0) {
+      const double kap = lam[0] * lam[1];
+      sc[s].kappa = kap;
+      int fl = 0;
+      if (!(kap <= 1e12)) fl |= SPK_ST_ILLCOND;
+      bool gau = false;
+      for (int k = 0; k < vgs[s].n; ++k) gau |= vgs[s].type[k] == SPK_VG_GAU;
+      if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr)))
+        fl |= SPK_ST_FP64VAR;
+      sc[s].flags = fl;
+    }
+    __syncthreads();
+  }
   // X columns: 0..p-1 = F (ones, drifts), p = z
   for (int i = tid; i < n; i += 256) {
     X[i] = 1.0;
@@ -539,36 +609,6 @@ __global__ __launch_bounds__(256) void k_dual(
   __syncthreads();
   tri_lower_tmv(L, ld, n, U, X, p + 1);  // X = C^-1 [F z]
   __syncthreads();
-  // condition estimate: max diag(C^-1) (squared column norms of Li) times
-  // the diagonal of C
-  {
-    __shared__ double red[4];
-    const int lane = tid & 63, wave = tid >> 6;
-    double mx = 0.0;
-    for (int k = wave; k < n; k += 4) {
-      double acc = 0.0;
-      for (int i = k + lane; i < n; i += 64) {
-        const double l = L[i + (int64_t)k * ld];
-        acc += l * l;
-      }
-      mx = fmax(mx, wave_sum(acc));
-    }
-    if (lane == 0) red[wave] = mx;
-    __syncthreads();
-    if (tid == 0) {
-      const double dinv = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      const spk_vg &vg = vgs[s];
-      const double cii = (method == SPK_M_SK)
-                             ? vg_entry(vg, 0.0, 1, min_vg_val)
-                             : sc[s].c0 - vg_entry(vg, 0.0, 0, min_vg_val);
-      const double kap = dinv * fabs(cii);
-      sc[s].kappa = kap;
-      int fl = 0;
-      if (!(kap <= 1e12)) fl |= SPK_ST_ILLCOND;
-      if (!(kap <= fp64var_thr)) fl |= SPK_ST_FP64VAR;
-      sc[s].flags = fl;
-    }
-  }
   __shared__ double G[(kMaxP + 1) * (kMaxP + 1)];
   // G[a][b] = U[:, a] . U[:, b]  (a, b <= p): B = U_F^T U_F, g = U_F^T U_z
   for (int e = tid >> 6; e < (p + 1) * (p + 1); e += 4) {

@@ -43,6 +43,10 @@ struct spk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // second stream (default priority) for factorisations that overlap the
+  // cell contractions, and the events that order the two
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // named scratch buffers, grown on demand, freed by spk_ctx_trim/destroy
   std::vector<std::pair<std::string, spk::Buffer>> bufs;
   std::vector<hipEvent_t> events;
@@ -128,6 +132,80 @@ __host__ __device__ inline double vg_entry(const spk_vg &vg, double h,
   for (int k = 0; k < vg.n; ++k)
     acc += (mult * vg.sill[k]) + (sign * vg_comp(vg.type[k], h, vg.range[k],
                                                  vg.sill[k]));
+  if (acc <= min_vg_val) acc = 0.0;
+  return acc;
+}
+
+// A variogram with per-component constants folded in (reciprocals instead
+// of divisions) for the batched path's bulk cell evaluation.  Values agree
+// with vg_entry to a few ulp; the plain FP64 path keeps vg_entry.
+struct VgFast {
+  int n;
+  int type[SPK_VG_MAX];
+  double s[SPK_VG_MAX], r[SPK_VG_MAX], c1[SPK_VG_MAX], c2[SPK_VG_MAX];
+};
+
+__host__ __device__ inline void vg_fast_init(const spk_vg &vg, VgFast &f) {
+  f.n = vg.n;
+  for (int k = 0; k < vg.n; ++k) {
+    const double r = vg.range[k];
+    f.type[k] = vg.type[k];
+    f.s[k] = vg.sill[k];
+    f.r[k] = r;
+    f.c1[k] = 0.0;
+    f.c2[k] = 0.0;
+    switch (vg.type[k]) {
+      case SPK_VG_SPH:
+        f.c1[k] = 1.5 / r;
+        f.c2[k] = 0.5 / (r * r * r);
+        break;
+      case SPK_VG_EXP:
+        f.c1[k] = -3.0 / r;
+        break;
+      case SPK_VG_GAU:
+        f.c1[k] = -3.0 / (r * r);
+        break;
+      case SPK_VG_LIN:
+        f.c1[k] = 1.0 / r;
+        break;
+      case SPK_VG_HOL:
+        f.c1[k] = kPi / r;
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+__device__ inline double vg_fast_entry(const VgFast &f, double h,
+                                       int covar_flag, double min_vg_val) {
+  double acc = 0.0;
+  for (int k = 0; k < f.n; ++k) {
+    const double s = f.s[k];
+    double g;
+    switch (f.type[k]) {
+      case SPK_VG_RNG: g = h; break;
+      case SPK_VG_NUG: g = s; break;
+      case SPK_VG_SPH:
+        g = (h >= f.r[k]) ? s : s * (h * f.c1[k] - (h * h * h) * f.c2[k]);
+        break;
+      case SPK_VG_EXP: g = s * (1.0 - exp(h * f.c1[k])); break;
+      case SPK_VG_GAU: g = s * (1.0 - exp((h * h) * f.c1[k])); break;
+      case SPK_VG_LIN: g = (h > f.r[k]) ? s : s * (h * f.c1[k]); break;
+      case SPK_VG_POW: g = s * pow(h, f.r[k]); break;
+      case SPK_VG_HOL: {
+        if (h == 0.0) {
+          g = 0.0;
+        } else {
+          const double a = h * f.c1[k];
+          g = s * (1.0 - sin(a) / a);
+        }
+        break;
+      }
+      default: g = NAN;
+    }
+    acc += covar_flag ? (s - g) : g;
+  }
   if (acc <= min_vg_val) acc = 0.0;
   return acc;
 }

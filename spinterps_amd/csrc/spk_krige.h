@@ -28,17 +28,19 @@ __global__ void k_compact(const double *vals, int64_t N, int32_t *idx,
 __global__ void k_assemble(const double *sx, const double *sy,
                            const int32_t *idx, int64_t N, const StepSc *sc,
                            const spk_vg *vgs, int covar_flag, double min_vg_val,
-                           double *A, int64_t ld, int64_t strideA);
+                           double *A, int64_t ld, int64_t strideA,
+                           double *Gfull);
 __global__ void k_potrf(double *A, int64_t ld, int64_t strideA, StepSc *sc,
                         const spk_vg *vgs, int shift);
 __global__ void k_trtri(const double *L, double *X, int64_t ld, int64_t stride,
                         const StepSc *sc);
-__global__ void k_dual(const double *Li, int64_t ld, int64_t stride,
+__global__ void k_dual(const double *Lc, const double *Li, int64_t ld,
+                       int64_t stride,
                        StepSc *sc, const double *zact, int64_t ldz,
                        const int32_t *idx, int64_t N, const double *stn_drift,
                        int m, int method, double *V, int64_t ldv, double *T0,
                        const spk_vg *vgs, double min_vg_val,
-                       double fp64var_thr);
+                       double fp64var_thr, double fp64var_thr_gau);
 __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const int32_t *idx, int64_t N,
                        const double *stn_drift, int m, int method, double *M,

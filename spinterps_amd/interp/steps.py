@@ -115,13 +115,16 @@ def _ctx_for(device):
 def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
                  precision='fp64', want_var=True, idw_exp=2.0,
                  min_vg_val=0.0, min_var_thr=-np.inf, est=None, var=None,
-                 vg_dev=None):
+                 vg_dev=None, kappa=None):
     """Interpolate T steps.  vals [T x N] (NaN = missing station).
 
     Returns (est [T x C] float64, var [T x C] float64 or None,
     info [T] int32) as device tensors.  info holds SPK_ST_* flags per step:
     1 no station (outputs left NaN), 2 mean fill, 4 failed system
-    (nearest-station fill), 8 single station.
+    (nearest-station fill), 8 single station, 16 ill-conditioned
+    (condition estimate > 1e12), 32 variance promoted to FP64 (mixed).
+    ``kappa``: optional float64 device tensor [T] that receives each
+    system's condition estimate.
     """
     dev = prob.device
     v = _t64(vals, dev)
@@ -158,7 +161,8 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     check(lib().spk_interp_steps_dev(
         ctx.h, ctypes.byref(pst), ctypes.byref(opts), T, v.data_ptr(),
         vg_dev.data_ptr() if krig else None, est.data_ptr(),
-        var.data_ptr() if wv else None, info.data_ptr()))
+        var.data_ptr() if wv else None, info.data_ptr(),
+        kappa.data_ptr() if kappa is not None else None))
     return est, (var if wv else None), info
 
 

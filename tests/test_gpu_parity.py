@@ -262,3 +262,51 @@ def test_sk_kriging_branch(gpu, precision):
     assert (np.abs(f64['SK'][:, near] - nnb) > 1e-12 * np.abs(nnb)).any()
     scale = np.nanmax(np.abs(f64["SK"]))
     np.testing.assert_allclose(e, f64["SK"], rtol=1e-9, atol=1e-12 * scale)
+
+
+def test_multi_batch_overlap_hourly_small(gpu):
+    """More steps than one batch (64): systems of batch b+1 are factorised
+    on the second stream while batch b is contracted.  Every step of the
+    batched FP64 path must match the plain FP64 path (1e-9), the mixed
+    path must stay within its bound, and sampled steps match the oracle."""
+    import torch
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.hourly(n_side=24, cell=5e5 / 24, n_stns=160)
+    vals, vgs = configs.hourly_steps(w, 0, 150, min_active=20)
+    prob = DeviceProblem.from_workload(w)
+    out = {}
+    for prec in ('fp64', 'fp64b', 'mixed'):
+        kap = torch.zeros(150, dtype=torch.float64, device='cuda')
+        e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=prec,
+                                  kappa=kap)
+        out[prec] = (e.cpu().numpy(), v.cpu().numpy(), info.cpu().numpy(),
+                     kap.cpu().numpy())
+    kap = out['fp64b'][3]
+    good = (kap < 1e8) & ((out['fp64'][2] & 4) == 0)
+    assert good.sum() > 100
+    e64, v64 = out['fp64'][0][good], out['fp64'][1][good]
+    # two FP64 formulations differ by ~1e-16 x condition: 1e-9 for
+    # kappa < 1e5, 1e-7 up to 1e8; the mixed path stays within 1e-6
+    well = kap[good] < 1e5
+    for prec in ('fp64b', 'mixed'):
+        e, v = out[prec][0][good], out[prec][1][good]
+        floor = 1e-3 * np.abs(e64).mean(axis=1, keepdims=True)
+        assert np.max(np.abs(e - e64) / np.maximum(np.abs(e64), floor)) < 1e-8
+        rv = np.abs(v - v64) / np.abs(v64)
+        rw, ri = rv[well].max(), rv[~well].max(initial=0.0)
+        print(f'{prec}: var max rel vs fp64 {rw:.2e} (kappa < 1e5), '
+              f'{ri:.2e} (1e5 <= kappa < 1e8)')
+        if prec == 'fp64b':
+            assert rw < 1e-9 and ri < 1e-7
+        else:
+            worst = np.where(good)[0][np.argmax(rv.max(axis=1))]
+            assert max(rw, ri) < 1e-6, (worst, kap[worst], vgs[worst],
+                                        out[prec][2][worst])
+    for t in np.where(good)[0][[0, 50, 100]]:
+        _, ref, _ = O.interp_all(w.stn_x, w.stn_y, w.cell_x, w.cell_y,
+                                 vals[t:t + 1], vgs[t:t + 1],
+                                 [('OK', 'OK', None)])
+        np.testing.assert_allclose(out['fp64b'][0][t], ref['OK'][0],
+                                   rtol=1e-9, atol=1e-12 * np.abs(ref['OK']).max())
+        np.testing.assert_allclose(out['fp64b'][1][t], ref['EST_VARS_OK'][0],
+                                   rtol=1e-9)
