@@ -177,8 +177,45 @@ int spk_clip_cast_dev(spk_ctx *ctx, const double *in64, float *out32,
                       int64_t n, int has_min, double min_cut, int has_max,
                       double max_cut);
 
+/* ---- variogram estimation and fit (variograms/vgs.py, fitvgs.py) ---- */
+/* All station pairs (i > j) sorted by distance (stable).  pi, pj, pd:
+ * device [N (N - 1) / 2].  vgs.py:345-391. */
+int spk_vgfit_pairs_dev(spk_ctx *ctx, const double *x, const double *y,
+                        int64_t N, int32_t *pi, int32_t *pj, double *pd);
+/* Empirical variograms of T steps: vals device [T x N] (NaN = no value);
+ * pairs with both values, the mdr distance clip (vgs.py:597-659), bins of nk
+ * consecutive pairs with the robust (classic = 0) or classic estimator
+ * (vgs.py:399-431, 495-520).  bin_h, bin_g, bin_n: device [T x max_bins];
+ * nbins: device [T]. */
+int spk_vgfit_evg_dev(spk_ctx *ctx, const int32_t *pi, const int32_t *pj,
+                      const double *pd, int64_t M, int64_t N,
+                      const double *vals, int64_t T, double mdr, int nk,
+                      int classic, int max_bins, double *bin_h, double *bin_g,
+                      int32_t *bin_n, int32_t *nbins);
+/* Differential-evolution fit (vgs.py:661-764) of n_cands candidate models
+ * to each step's bins.  cands: HOST [n_cands x 6] int32 = {n_comp,
+ * type[4] (SPK_VG_*), first_fixed}; first_fixed pins the first component
+ * to the first bin (vgs.py:735-739).  var: device [T] (the step variance
+ * bounds the sills at 2 var).  params: device [T x n_cands x 8] as (range,
+ * sill) per component; fun, nit: device [T x n_cands].  range_ub is the
+ * upper range bound (1e9 in vgs.py:747); popmul 50 and tol 0.01 are the
+ * reference's settings. */
+int spk_vgfit_de_dev(spk_ctx *ctx, const double *bin_h, const double *bin_g,
+                     const int32_t *nbins, int max_bins, const double *var,
+                     int64_t T, const int32_t *cands, int n_cands, int maxiter,
+                     int popmul, double tol, double range_ub, uint64_t seed,
+                     double *params, double *fun, int32_t *nit);
+
 /* Device timing of the last spk_interp_steps_dev call, per stage (ms). */
 int spk_last_timings(spk_ctx *ctx, double *ms, int max_n, int *n_out);
+
+/* Kernel timing with hipEvents around the library's heavy launches:
+ * enable/disable, clear, and read the total ms and launch count of one
+ * kernel family ("gemm_tri_f32", "kgen", ...). */
+int spk_prof_enable(spk_ctx *ctx, int on);
+int spk_prof_reset(spk_ctx *ctx);
+int spk_prof_get(spk_ctx *ctx, const char *name, double *total_ms,
+                 int64_t *count);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,84 @@
+"""GPU variogram estimation and fit against the CPU oracle.
+
+Bins: pair counts exact, mean distances and estimator values 1e-12
+relative.  Fits: differential evolution is stochastic (unseeded in the
+reference), so a fit is compared by its objective on bounds that contain
+the optimum: the GPU objective must be within 1e-3 of SciPy's (or below).
+"""
+import numpy as np
+import pytest
+
+from oracle import vgfit_oracle as V
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(seed, n=260, n_act=200, r=25e3, model='Exp'):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 2e5, n)
+    y = rng.uniform(0, 2e5, n)
+    d = np.hypot(x[:, None] - x[None], y[:, None] - y[None])
+    if model == 'Exp':
+        c = np.exp(-3 * d / r)
+    else:
+        c = np.exp(-3 * (d / r) ** 2)
+    c += 1e-6 * np.eye(n)
+    z = np.linalg.cholesky(c) @ rng.standard_normal(n) * 2.0 + 5.0
+    z[rng.choice(n, n - n_act, replace=False)] = np.nan
+    return x, y, z
+
+
+@pytest.mark.parametrize('evg', ['robust', 'classic'])
+@pytest.mark.parametrize('mdr', [0.5, 0.7])
+def test_bins_match_oracle(gpu, evg, mdr):
+    from spinterps_amd.variograms.fitvgs import BatchVariogramFitter
+    fields = [_field(s) for s in range(3)]
+    x, y = fields[0][0], fields[0][1]
+    rng = np.random.default_rng(9)
+    vals = np.stack([_field(s)[2] for s in range(3)])
+    f = BatchVariogramFitter(x, y)
+    bh, bg, bn, nb = f.empirical(vals, mdr, 50, evg)
+    for t in range(3):
+        ok = ~np.isnan(vals[t])
+        ref = V.empirical(x[ok], y[ok], vals[t][ok], mdr, 50, evg)
+        k = int(nb[t])
+        assert k == ref.shape[0]
+        np.testing.assert_array_equal(bn[t, :k].cpu().numpy(), ref[:, 2])
+        np.testing.assert_allclose(bh[t, :k].cpu().numpy(), ref[:, 0],
+                                   rtol=1e-12)
+        np.testing.assert_allclose(bg[t, :k].cpu().numpy(), ref[:, 1],
+                                   rtol=1e-12)
+
+
+def test_de_objective_matches_scipy(gpu):
+    from spinterps_amd.variograms.fitvgs import BatchVariogramFitter
+    x, y, _ = _field(0)
+    vals = np.stack([_field(s)[2] for s in (0, 1)] +
+                    [_field(s, model='Gau', r=40e3)[2] for s in (2, 3)])
+    f = BatchVariogramFitter(x, y)
+    kw = dict(mdr=0.5, nk=50, perm_r_list=[1, 2], fit_vgs=['Sph', 'Exp'],
+              fil_nug_vg='Nug', range_ub=2e5)
+    out, det = f.fit(vals, seed=5, return_details=True, **kw)
+    cands = det['cands']
+    for k, t in enumerate(det['steps']):
+        ok = ~np.isnan(vals[t])
+        strs, ref = V.fit(x[ok], y[ok], vals[t][ok], seed=11, **kw)
+        for c, o in enumerate(cands):
+            fg = det['fun'][k, c]
+            fc = ref['funs'][o][0]
+            assert fg <= fc * (1 + 1e-3) + 1e-12, (t, o, fg, fc)
+        assert out[t] is not None and len(out[t]) == 1
+
+
+def test_fit_strings_parse_and_skip_rules(gpu):
+    from spinterps_amd import _lib
+    from spinterps_amd.variograms.fitvgs import BatchVariogramFitter
+    x, y, z = _field(4)
+    vals = np.stack([z, np.full_like(z, np.nan), np.where(np.isnan(z), z, 3.0)])
+    f = BatchVariogramFitter(x, y)
+    out = f.fit(vals, mdr=0.7, perm_r_list=[1, 2], fit_vgs=['Sph', 'Exp'],
+                fil_nug_vg='Nug', seed=1, opt_iters=2000)
+    assert out[1] is None            # no values
+    assert out[2] is None            # no spread (fitvgs.py:176-180)
+    v = _lib.parse_vg(out[0][0])
+    assert 1 <= v.n <= 2
