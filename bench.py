@@ -40,6 +40,10 @@ def main():
     ap.add_argument('--cpu-cells', type=int, default=200)
     ap.add_argument('--cpu-steps', type=int, default=6)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--vg-fields', type=int, default=512,
+                    help='simulated fields for the variogram-fit leg '
+                         '(0 skips it)')
+    ap.add_argument('--cpu-vg-fields', type=int, default=4)
     ap.add_argument('--calib', action='store_true',
                     help='also run k_clip_cast over 2^28 float64 values '
                          '(2 GiB read, 1 GiB written) as the reference '
@@ -253,6 +257,8 @@ def main():
         del x
         log('calibration copy done: k_clip_cast, 2 GiB read + 1 GiB write')
 
+    vgfit = vgfit_leg(args, log) if args.vg_fields > 0 else None
+
     out = {
         'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '
                   'plain FP64 @<=1e-6 rel',
@@ -292,8 +298,77 @@ def main():
                                'frac': achp / 78.6}},
         'steps_fp64_variance': int(f64_steps.sum()),
         'cpu_baseline': cpu,
+        'variograms_fitted': vgfit,
     }
     print(json.dumps(out), flush=True)
+
+
+def vgfit_leg(args, log):
+    """Variograms fitted/s on simulated fields (1500-station pool over
+    500 km x 500 km, 300-1500 active stations, random Sph/Exp/Gau with a
+    nugget): the plain GPU fit (all 7 candidates of fitvgs.py's settings),
+    the proposal-seeded fit with and without the 2 % verification, and the
+    SciPy restatement on the host as the CPU baseline."""
+    import numpy as np
+    import torch
+
+    from spinterps_amd.interp.steps import DeviceProblem
+    from spinterps_amd.variograms.fitvgs import BatchVariogramFitter
+    from spinterps_amd.variograms.learned import (LearnedVariogramFitter,
+                                                  load_proposal_net,
+                                                  training_fields)
+    from spinterps_amd.variograms.train_proposal import FIT_KW, station_pool
+    F = args.vg_fields
+    x, y = station_pool()
+    prob = DeviceProblem(x, y, x[:1], y[:1])
+    parts = []
+    for c0 in range(0, F, 128):
+        fl, info, _ = training_fields(prob, min(128, F - c0),
+                                      seed=777000 + c0)
+        parts.append(fl.cpu().numpy())
+    vals = np.concatenate(parts)
+    fitter = BatchVariogramFitter(x, y)
+    lf = LearnedVariogramFitter(x, y, load_proposal_net(), fitter=fitter)
+    fitter.fit(vals[:8], seed=1, **FIT_KW)  # warm-up
+    lf.fit(vals[:8], verify=False, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fitter.fit(vals, seed=2, **FIT_KW)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lf.fit(vals, verify=False, seed=2)
+    torch.cuda.synchronize()
+    t_fast = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, det = lf.fit(vals, verify=True, seed=2, return_details=True)
+    torch.cuda.synchronize()
+    t_ver = time.perf_counter() - t0
+    ratio = det['learned_fun'] / det['plain_fun']
+    log(f'vgfit: plain {F / t_plain:.1f}/s, learned {F / t_fast:.1f}/s, '
+        f'verified {F / t_ver:.1f}/s, fallback {det["fallback"].mean():.3f}')
+    res = {'fields': F, 'unit': 'variograms fitted/s',
+           'plain_gpu': F / t_plain, 'learned_gpu': F / t_fast,
+           'learned_verified_gpu': F / t_ver,
+           'fallback_fraction': float(det['fallback'].mean()),
+           'learned_over_plain_error_median': float(np.median(ratio)),
+           'learned_over_plain_error_p90': float(np.percentile(ratio, 90))}
+    if not args.no_cpu and args.cpu_vg_fields > 0:
+        from threadpoolctl import threadpool_limits
+        from oracle import vgfit_oracle as V
+        t0 = time.perf_counter()
+        n = min(args.cpu_vg_fields, F)
+        with threadpool_limits(1):
+            for t in range(n):
+                V.fit_step(x, y, vals[t], seed=t, **FIT_KW)
+        dt = time.perf_counter() - t0
+        res['cpu_baseline'] = {
+            'value': n / dt, 'unit': 'variograms fitted/s', 'cores': 1,
+            'kind': 'port',
+            'sample': f'{n} fields, scipy differential_evolution as '
+                      'vgs.py:758-764 calls it, one BLAS thread'}
+        log(f'vgfit cpu: {n / dt:.3f}/s')
+    return res
 
 
 if __name__ == '__main__':

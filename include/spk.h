@@ -206,6 +206,70 @@ int spk_vgfit_de_dev(spk_ctx *ctx, const double *bin_h, const double *bin_g,
                      int popmul, double tol, double range_ub, uint64_t seed,
                      double *params, double *fun, int32_t *nit);
 
+/* spk_vgfit_de_dev with a seeded start population (the learned proposal's
+ * refinement).  Replaces the Latin-hypercube start of the same
+ * differential evolution (vgs.py:758-764, scipy 'latinhypercube' init):
+ * member 0 of (step t, candidate c) is init[(t * n_cands + c) * 8 + 0..D-1]
+ * (clipped to the bounds); the others stratify, per dimension, the
+ * log-box init * exp([-spread, spread]) clipped to the bounds.  spread in
+ * [0, 20]; pinned dimensions stay pinned. */
+int spk_vgfit_de_seeded_dev(spk_ctx *ctx, const double *bin_h,
+                            const double *bin_g, const int32_t *nbins,
+                            int max_bins, const double *var, int64_t T,
+                            const int32_t *cands, int n_cands, int maxiter,
+                            int popmul, double tol, double range_ub,
+                            uint64_t seed, const double *init, double spread,
+                            double *params, double *fun, int32_t *nit);
+
+/* ---- learned variogram proposal ----------------------------------------
+ * No counterpart function in the reference: the training data generator,
+ * features and network that propose a model for the fit of
+ * variograms/fitvgs.py:143-278.  All pointers are device memory. */
+
+/* Unconditional Gaussian fields: for each step t the stations whose
+ * mask_vals[t, i] is not NaN get mean[t] + (L xi)_i with L L^T the
+ * covariance of vgs[t] (sum over components of sill - gamma, the nugget
+ * sill on the diagonal, plus 1e-6 of the sill sum as jitter), xi standard
+ * normal from a counter-based generator keyed on (seed, t, station rank).
+ * out [T x N]: NaN at masked stations.  info [T]: SPK_ST_* (FAILED when
+ * the Cholesky failed; the step's row stays NaN). */
+int spk_simulate_fields_dev(spk_ctx *ctx, const spk_problem *prob, int64_t T,
+                            const double *mask_vals, const spk_vg *vgs,
+                            const double *means, uint64_t seed, double *out,
+                            int32_t *info);
+
+/* Lag-bin features of each step from the distance-sorted pairs of
+ * spk_vgfit_pairs_dev: nlag equal-width bins over (0, mdr * largest active
+ * pair distance]; feat [T x 2 nlag] float: classic semivariance
+ * (half mean squared difference) / step variance, then nlag times the
+ * share of pairs per bin; scale [T x 2]: (mdr * largest distance,
+ * variance).  nlag <= 64. */
+int spk_vg_features_dev(spk_ctx *ctx, const int32_t *pi, const int32_t *pj,
+                        const double *pd, int64_t M, int64_t N,
+                        const double *vals, int64_t T, double mdr, int nlag,
+                        float *feat, double *scale);
+
+/* Multilayer perceptron 40 -> 128 -> 128 -> 128 -> 6 (GELU hidden
+ * activations), float32 parameters in one flat array
+ * [W1 b1 W2 b2 W3 b3 W4 b4], W_l row-major [out x in].  Outputs: logits of
+ * (Sph, Exp, Gau), log(sill / variance), log(range / lag scale), nugget
+ * fraction. */
+int64_t spk_mlp_n_params(void);
+int spk_mlp_init_dev(spk_ctx *ctx, uint64_t seed, float *params);
+/* X [B x 40] -> out [B x 6]. */
+int spk_mlp_forward_dev(spk_ctx *ctx, const float *params, const float *X,
+                        int B, float *out);
+/* Mean over the batch of softmax cross-entropy (cls [B] in 0..2) plus
+ * reg_w * squared error of the 3 regression outputs against reg [B x 3]
+ * (NaN targets ignored); writes loss [1] and grads (layout of params). */
+int spk_mlp_loss_grad_dev(spk_ctx *ctx, const float *params, const float *X,
+                          int B, const int32_t *cls, const float *reg,
+                          float reg_w, float *grads, float *loss);
+/* Adam update, step counted from 1 (bias correction 1 - beta^step). */
+int spk_mlp_adam_dev(spk_ctx *ctx, float *params, const float *grads,
+                     float *m, float *v, int64_t step, float lr, float b1,
+                     float b2, float eps);
+
 /* Device timing of the last spk_interp_steps_dev call, per stage (ms). */
 int spk_last_timings(spk_ctx *ctx, double *ms, int max_n, int *n_out);
 

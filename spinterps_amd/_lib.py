@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import ctypes
 import threading
-from ctypes import (POINTER, c_char_p, c_double, c_int, c_int32, c_int64,
+from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64,
                     c_uint64, c_void_p)
 from pathlib import Path
 
@@ -116,6 +116,28 @@ SIGNATURES = {
                                  POINTER(c_int32), c_int, c_int, c_int,
                                  c_double, c_double, c_uint64, c_void_p,
                                  c_void_p, c_void_p]),
+    'spk_vgfit_de_seeded_dev': (c_int, [c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_int, c_void_p, c_int64,
+                                        POINTER(c_int32), c_int, c_int,
+                                        c_int, c_double, c_double, c_uint64,
+                                        c_void_p, c_double, c_void_p,
+                                        c_void_p, c_void_p]),
+    'spk_simulate_fields_dev': (c_int, [c_void_p, c_void_p, c_int64,
+                                        c_void_p, c_void_p, c_void_p,
+                                        c_uint64, c_void_p, c_void_p]),
+    'spk_vg_features_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_int64, c_int64, c_void_p, c_int64,
+                                    c_double, c_int, c_void_p, c_void_p]),
+    'spk_mlp_n_params': (c_int64, []),
+    'spk_mlp_init_dev': (c_int, [c_void_p, c_uint64, c_void_p]),
+    'spk_mlp_forward_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_int,
+                                    c_void_p]),
+    'spk_mlp_loss_grad_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_int,
+                                      c_void_p, c_void_p, c_float, c_void_p,
+                                      c_void_p]),
+    'spk_mlp_adam_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_int64, c_float, c_float, c_float,
+                                 c_float]),
 }
 
 _lib = None

@@ -45,6 +45,9 @@ __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const int32_t *idx, int64_t N,
                        const double *stn_drift, int m, int method, double *M,
                        double *T0, const spk_vg *vgs, double min_vg_val);
+__global__ void k_recenter(const double *in, double *out, int64_t n, double o);
+__global__ void k_status_out(const StepSc *sc, int S, int32_t *info,
+                             double *kappa);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
 

@@ -261,7 +261,8 @@ __global__ __launch_bounds__(256) void k_de(
     const int32_t *__restrict__ nbins, int max_bins,
     const double *__restrict__ var, const int32_t *__restrict__ cand,
     int ncand, int maxiter, int popmul, double tol, double range_ub,
-    unsigned long long seed, double *__restrict__ out_par,
+    unsigned long long seed, const double *__restrict__ init, double spread,
+    double *__restrict__ out_par,
     double *__restrict__ out_fun, int32_t *__restrict__ out_nit) {
   const int s = blockIdx.y, ci = blockIdx.x;
   const int tid = threadIdx.x;
@@ -326,8 +327,27 @@ __global__ __launch_bounds__(256) void k_de(
       }
     }
     __syncthreads();
-    for (int i = tid; i < P; i += 256)
-      pop[i * kDeMaxD + d] = (perm[i] + rng.uni()) / P;
+    if (!init) {
+      for (int i = tid; i < P; i += 256)
+        pop[i * kDeMaxD + d] = (perm[i] + rng.uni()) / P;
+    } else {
+      // seeded start: member 0 is the given point; the others stratify a
+      // log-box of half-width `spread` around it, clipped to the bounds
+      const double w = ub[d] - lb[d];
+      const double c = fmin(fmax(init[((int64_t)s * ncand + ci) * kDeMaxD + d],
+                                 lb[d]), ub[d]);
+      for (int i = tid; i < P; i += 256) {
+        double u = 0.0;
+        if (w > 0.0) {
+          double v = c;
+          if (i > 0)
+            v = c * exp(spread * (2.0 * (perm[i] + rng.uni()) / P - 1.0));
+          v = fmin(fmax(v, lb[d]), ub[d]);
+          u = (v - lb[d]) / w;
+        }
+        pop[i * kDeMaxD + d] = u;
+      }
+    }
     __syncthreads();
   }
   for (int i = tid; i < P; i += 256)
@@ -461,35 +481,62 @@ int spk_vgfit_evg_dev(spk_ctx *ctx, const int32_t *pi, const int32_t *pj,
   return SPK_OK;
 }
 
-int spk_vgfit_de_dev(spk_ctx *ctx, const double *bin_h, const double *bin_g,
+static int de_launch(spk_ctx *ctx, const double *bin_h, const double *bin_g,
                      const int32_t *nbins, int max_bins, const double *var,
                      int64_t T, const int32_t *cands, int n_cands, int maxiter,
                      int popmul, double tol, double range_ub, uint64_t seed,
-                     double *params, double *fun, int32_t *nit) {
+                     const double *init, double spread, double *params,
+                     double *fun, int32_t *nit) {
   if (!ctx || !bin_h || !bin_g || !nbins || !var || !cands || !params ||
       !fun || !nit)
     return set_error(SPK_EARG, "null argument");
   if (n_cands <= 0 || n_cands > 64) return set_error(SPK_EARG, "bad n_cands");
   if (popmul <= 0 || maxiter <= 0) return set_error(SPK_EARG, "bad DE sizes");
+  for (int c = 0; c < n_cands; ++c)
+    if (cands[c * 6] < 1 || cands[c * 6] > 4)
+      return set_error(SPK_EARG, "candidate %d: 1..4 components", c);
+  if (init && !(spread >= 0.0 && spread <= 20.0))
+    return set_error(SPK_EARG, "spread outside [0, 20]");
   if (T <= 0) return SPK_OK;
   SPK_HIP(hipSetDevice(ctx->device));
   int32_t *dc;
   SPK_TRY(ws_get(ctx, "vgcand", n_cands * 6 * 4, (void **)&dc));
   SPK_TRY(h2d(ctx, dc, cands, n_cands * 6 * 4));
-  for (int c = 0; c < n_cands; ++c)
-    if (cands[c * 6] < 1 || cands[c * 6] > 4)
-      return set_error(SPK_EARG, "candidate %d: 1..4 components", c);
   for (int64_t t0 = 0; t0 < T; t0 += 65535) {
     const int Tb = (int)std::min<int64_t>(65535, T - t0);
     hipLaunchKernelGGL(k_de, dim3(n_cands, Tb), dim3(256), 0, ctx->stream,
                        bin_h + t0 * max_bins, bin_g + t0 * max_bins,
                        nbins + t0, max_bins, var + t0, dc, n_cands, maxiter,
                        popmul, tol, range_ub, (unsigned long long)seed,
+                       init ? init + t0 * n_cands * kDeMaxD : nullptr, spread,
                        params + t0 * n_cands * kDeMaxD, fun + t0 * n_cands,
                        nit + t0 * n_cands);
     SPK_TRY(check_launch("k_de"));
   }
   return SPK_OK;
+}
+
+int spk_vgfit_de_dev(spk_ctx *ctx, const double *bin_h, const double *bin_g,
+                     const int32_t *nbins, int max_bins, const double *var,
+                     int64_t T, const int32_t *cands, int n_cands, int maxiter,
+                     int popmul, double tol, double range_ub, uint64_t seed,
+                     double *params, double *fun, int32_t *nit) {
+  return de_launch(ctx, bin_h, bin_g, nbins, max_bins, var, T, cands, n_cands,
+                   maxiter, popmul, tol, range_ub, seed, nullptr, 0.0, params,
+                   fun, nit);
+}
+
+int spk_vgfit_de_seeded_dev(spk_ctx *ctx, const double *bin_h,
+                            const double *bin_g, const int32_t *nbins,
+                            int max_bins, const double *var, int64_t T,
+                            const int32_t *cands, int n_cands, int maxiter,
+                            int popmul, double tol, double range_ub,
+                            uint64_t seed, const double *init, double spread,
+                            double *params, double *fun, int32_t *nit) {
+  if (!init) return set_error(SPK_EARG, "null argument");
+  return de_launch(ctx, bin_h, bin_g, nbins, max_bins, var, T, cands, n_cands,
+                   maxiter, popmul, tol, range_ub, seed, init, spread, params,
+                   fun, nit);
 }
 
 }  // extern "C"

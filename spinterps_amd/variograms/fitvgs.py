@@ -175,5 +175,6 @@ class BatchVariogramFitter:
             details.append(best)
         if return_details:
             return out, dict(steps=steps, bins=bins, params=par, fun=fun,
-                             cands=cands, best=details)
+                             nit=nit.cpu().numpy(), cands=cands,
+                             best=details)
         return out
