@@ -174,7 +174,9 @@ def main():
         a, b = em[s], e64[s]
         floor = 1e-3 * b.abs().mean()
         re = ((a - b).abs() / torch.maximum(b.abs(), floor)).cpu().numpy()
-        rv = ((vm[s] - v64[s]).abs() / v64[s].abs()).cpu().numpy()
+        vfloor = 1e-3 * v64[s].abs().mean()
+        rv = ((vm[s] - v64[s]).abs() /
+              torch.maximum(v64[s].abs(), vfloor)).cpu().numpy()
         per_step.append({'kappa': float(kp[s]), 'fp64_var': bool(mi[s] & 32),
                          'n': int(n_act[s0 + s]),
                          'est_max': float(re.max()),
@@ -183,6 +185,7 @@ def main():
             ee_all.append(re)
             ev_all.append(rv)
     err = {'est_floor': '1e-3 x mean |estimate| of the step',
+           'var_floor': '1e-3 x mean variance of the step',
            'steps_compared': int(comparable.sum()),
            'steps_excluded': int((~comparable).sum()),
            'per_step': per_step}

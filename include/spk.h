@@ -133,9 +133,10 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 #define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
 #define SPK_ST_ILLCOND  16   /* condition estimate > 1e12: results are set
                                 by rounding (as the reference's pinv is) */
-#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: condition estimate above 2e4
-                                (1e3 with a Gaussian component), so the
-                                variance contraction ran in FP64          */
+#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: condition estimate above the
+                                FP32 limit (50 by default, see
+                                spk_opts.fp64var_kappa), so the variance
+                                contraction ran in FP64                   */
 
 typedef struct spk_problem {
   int64_t n_stns;               /* N                                      */
@@ -157,6 +158,9 @@ typedef struct spk_opts {
   double idw_exp;
   double min_vg_val;            /* interp/data.py default 0.0             */
   double min_var_thr;           /* interp_steps_flags (steps.py:760-765)  */
+  double fp64var_kappa;         /* SPK_P_MIXED: condition estimate above
+                                   which a step's variance contraction runs
+                                   in FP64; 0 = the library's policy      */
 } spk_opts;
 
 /* Interpolate n_steps steps.  vals: device [n_steps x N], NaN = station

@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void k_gemm(
 __global__ __launch_bounds__(256) void k_var_reduce(
     const StepSc *__restrict__ sc, const double *__restrict__ abuf,
     const double *__restrict__ qpart, int64_t ldq, int nrb, int64_t Cs,
-    double *__restrict__ var, int64_t ldo) {
+    const int32_t *__restrict__ perm, double *__restrict__ var, int64_t ldo) {
   const int sl = blockIdx.y;
   const StepSc *st = sc + sl;
   if (st->status) return;
@@ -361,22 +361,164 @@ __global__ __launch_bounds__(256) void k_var_reduce(
   const double *q = qpart + (int64_t)sl * nrb * ldq + c;
   double acc = 0.0;
   for (int r = 0; r < nb; ++r) acc += q[(int64_t)r * ldq];
-  var[(int64_t)sl * ldo + c] = abuf[(int64_t)sl * ldq + c] - acc;
+  var[(int64_t)sl * ldo + (perm ? (int64_t)perm[c] : c)] =
+      abuf[(int64_t)sl * ldq + c] - acc;
+}
+
+// ---- FP64 triangular contraction, 128 x 128 tiles -----------------------------
+// q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell, L^-1 lower
+// triangular (column-major, lda), delta row-major [n x ldb].  Block tile
+// 128 rows x 128 cells, k in chunks of 16 (K runs to the tile's last row);
+// 4 waves in 2 x 2, each 64 rows x 64 cells = 4 x 4 MFMA f64 16x16x4 tiles
+// held in 64 accumulator registers.  The next chunk is loaded into
+// registers while the current one is multiplied out of LDS.  Each wave
+// writes the partial norm of its 64 rows to qpart[(step * nrb + rb) * ldq]
+// with rb = 2 * (row tile) + (wave row), so the 64-row partials of the
+// generic kernel and this one have the same layout.
+__global__ __launch_bounds__(256, 2) void k_tri64(
+    const double *__restrict__ A, int64_t lda, int64_t strideA,
+    const double *__restrict__ B, int64_t ldb, int64_t strideB,
+    const StepSc *__restrict__ sc, int64_t Cs, double *__restrict__ qpart,
+    int64_t ldq, int nrb, int sel) {
+  constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
+  const int G = gridDim.x * gridDim.y;
+  int lin = blockIdx.x + blockIdx.y * gridDim.x;
+  if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
+  const int rti = lin % gridDim.x, cti = lin / gridDim.x;
+  const int sl = blockIdx.z;
+  const StepSc st = sc[sl];
+  if (st.status) return;
+  if (sel == 1 && (st.flags & SPK_ST_FP64VAR)) return;
+  if (sel == 2 && !(st.flags & SPK_ST_FP64VAR)) return;
+  const int n = st.n;
+  const int r0 = rti * BM;
+  if (r0 >= n) return;
+  const int64_t c0 = cti * (int64_t)BN;
+  if (c0 >= Cs) return;
+  const int kend = min(n, r0 + BM);
+  A += (int64_t)sl * strideA;
+  B += (int64_t)sl * strideB;
+  __shared__ __align__(16) double As[2][BK][BM + PAD];
+  __shared__ __align__(16) double Bs[2][BK][BN + PAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  // global -> register staging: A chunk = 16 columns x 128 rows, B chunk =
+  // 16 rows x 128 cells; each thread moves 4 double2 of each
+  double2 ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;          // 0..1023 double2 slots
+      const int r = (e & 63) * 2, k = e >> 6;  // A: row pair r, column k
+      const int gk = k0 + k;
+      // rows past n are zero and never read (lda is a multiple of 64 above
+      // n, so a pair starting below n lies inside the column)
+      if (gk < kend && r0 + r < n) {
+        double2 v = *reinterpret_cast<const double2 *>(
+            A + (r0 + r) + (int64_t)gk * lda);
+        if (r0 + r + 1 >= n) v.y = 0.0;
+        ra[q] = v;
+      } else {
+        ra[q] = double2{0.0, 0.0};
+      }
+      const int cc = (e & 63) * 2, kb = e >> 6;  // B: row kb, cell pair cc
+      const int gkb = k0 + kb;
+      rb[q] = gkb < kend ? *reinterpret_cast<const double2 *>(
+                               B + (int64_t)gkb * ldb + c0 + cc)
+                         : double2{0.0, 0.0};
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;
+      const int r = (e & 63) * 2, k = e >> 6;
+      *reinterpret_cast<double2 *>(&As[buf][k][r]) = ra[q];
+      *reinterpret_cast<double2 *>(&Bs[buf][k][r]) = rb[q];
+    }
+  };
+  d4_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4_t{0, 0, 0, 0};
+  const int nch = (kend + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) load((ch + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int ka = kk * 4 + lg;
+      double af[4], bf[4];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) af[ti] = As[buf][ka][wr * 64 + ti * 16 + li];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) bf[tj] = Bs[buf][ka][wc * 64 + tj * 16 + li];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ti], bf[tj],
+                                                             acc[ti][tj], 0,
+                                                             0, 0);
+    }
+    if (ch + 1 < nch) store(buf ^ 1);
+    __syncthreads();
+  }
+  // squared norms of this wave's 64 rows, per cell
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = r0 + wr * 64 + ti * 16 + lg + 4 * r;
+      if (row < n) {
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) part[tj] += acc[ti][tj][r] * acc[ti][tj][r];
+      }
+    }
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    part[tj] += __shfl_xor(part[tj], 16, 64);
+    part[tj] += __shfl_xor(part[tj], 32, 64);
+  }
+  const int prow = 2 * rti + wr;
+  if (lg != 0 || prow >= nrb) return;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    const int64_t col = c0 + wc * 64 + tj * 16 + li;
+    if (col < Cs) qpart[((int64_t)sl * nrb + prow) * ldq + col] = part[tj];
+  }
 }
 
 // ---- batched path: fused distance -> covariance -------------------------------
 // For each active station i and cell c: k_ic = c0 - G(d_ic) (OK/EDK) or the
 // covariance sum (SK).  The estimate k.alpha (+ constraint terms) is summed
 // in FP64.  SK keeps the reference's sum-of-weights test (k.w == 1).
-// For the OK variance the contraction input is deflated by the nearest
-// active station j(c): delta_ic = G(d_ij) - G(d_ic) = k_ic - C_ij, formed in
-// FP64.  With q = k^T C^-1 k = C_jj + 2 delta_j + |L^-1 delta|^2 the variance
-// lambda.g + 2 mu becomes
-//     2 g_jc - G_jj - |L^-1 delta|^2 + (a - 1)(a - 2) / b,   a = k.w,
-// whose first terms are exact in FP64 and whose last GEMM term is small
-// near stations, where the variance is small.  base = everything but the
-// GEMM term goes to abuf; delta goes to K32 or, for steps flagged
-// SPK_ST_FP64VAR (or the FP64 batched path), to K64.
+//
+// OK variance.  With q = k^T C^-1 k and a = k.w the reference's
+// lambda.g + 2 mu equals c0 - q + (a - 1)(a - 2) / b.  q is split with a
+// local fit of k on the columns of C at the cell's nearest active stations
+// J (up to kJ): for any beta and delta = k - C_J beta,
+//     q = beta^T C_JJ beta + 2 beta^T delta_J + |L^-1 delta|^2,
+// because C^-1 C_J = E_J.  beta solves C_JJ beta = k_J in FP64, so delta is
+// the residual of local kriging, small near stations (where the variance is
+// small) and wherever J screens the cell; that keeps the FP32 contraction
+// term small next to the variance.  All other terms are FP64; abuf receives
+// c0 - (beta terms) + (a-1)(a-2)/b and delta goes to K32 or, for steps
+// flagged SPK_ST_FP64VAR (or the FP64 batched path), to K64.  Any beta gives
+// the same variance in exact arithmetic, so J is cut where C_JJ's Cholesky
+// pivot falls below 1e-8 of its diagonal.
+// Cells come in a spatially compact order, so the 256 cells of a block share
+// few distinct J stations: up to kSlots of their G columns are staged in
+// LDS per station chunk (a cell's stations without a slot leave its J).
+// Cells are stripe-local indices c; with perm the outputs and the cell
+// drifts are addressed at the original index perm[c].
+constexpr int kJ = 4, kSlots = 16;
 __global__ __launch_bounds__(256) void k_kgen(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N,
@@ -384,6 +526,7 @@ __global__ __launch_bounds__(256) void k_kgen(
     const StepSc *__restrict__ sc, const spk_vg *__restrict__ vgs,
     const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
     int64_t C, const double *__restrict__ cell_drift,
+    const int32_t *__restrict__ perm,
     const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
     double min_vg_val, int want_var, int all64, float *__restrict__ K32,
     double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
@@ -396,22 +539,44 @@ __global__ __launch_bounds__(256) void k_kgen(
   const double *alpha = V + (int64_t)sl * ldv;
   const double *w = alpha + ld;
   __shared__ double stx[256], sty[256], sal[256], sw[256];
+  __shared__ double sG[kSlots][256];
+  __shared__ int slot_j[kSlots];
+  __shared__ int n_slots;
   __shared__ VgFast vf;
   const int tid = threadIdx.x;
   const int64_t c = blockIdx.x * (int64_t)256 + tid;
   const bool valid = c < Cs;
   const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
-  if (tid == 0) vg_fast_init(vgs[sl], vf);
+  if (tid == 0) {
+    vg_fast_init(vgs[sl], vf);
+    n_slots = 0;
+  }
+  if (tid < kSlots) slot_j[tid] = -1;
   const int cflag = method == SPK_M_SK ? 1 : 0;
   const double c0 = stp->c0;
   const bool var_on = want_var && method == SPK_M_OK;
   const bool use64 = all64 || (stp->flags & SPK_ST_FP64VAR);
   float *K32s = (var_on && !use64) ? K32 + (int64_t)sl * ld * ldk : nullptr;
   double *K64s = (var_on && use64) ? K64 + (int64_t)sl * ld * ldk : nullptr;
-  double best = INFINITY;
-  int bi = 0;
-  // pass 1 (variance only): the nearest active station j(c)
+  const double *Gs = var_on ? Gfull + (int64_t)sl * ld * ld : nullptr;
+  double beta[kJ];
+  int slt[kJ];
+#pragma unroll
+  for (int q = 0; q < kJ; ++q) {
+    beta[q] = 0.0;
+    slt[q] = -1;
+  }
+  double qloc = 0.0, sbeta = 0.0;
   if (var_on) {
+    // pass 1: the cell's kJ nearest active stations (ascending distance,
+    // ties to the lower index)
+    double bd[kJ];
+    int bj[kJ];
+#pragma unroll
+    for (int q = 0; q < kJ; ++q) {
+      bd[q] = INFINITY;
+      bj[q] = -1;
+    }
     for (int base = 0; base < n; base += 256) {
       __syncthreads();
       if (base + tid < n) {
@@ -423,18 +588,126 @@ __global__ __launch_bounds__(256) void k_kgen(
       const int cnt = min(256, n - base);
       if (valid)
         for (int j = 0; j < cnt; ++j) {
-          const double d = dist2(x, y, stx[j], sty[j]);
-          if (d < best) {
-            best = d;
-            bi = base + j;
+          const double dx = x - stx[j], dy = y - sty[j];
+          double d2 = dx * dx + dy * dy;
+          if (d2 < bd[kJ - 1]) {
+            int id = base + j;
+#pragma unroll
+            for (int q = 0; q < kJ; ++q) {  // insertion, strict: stable
+              if (d2 < bd[q]) {
+                const double td = bd[q];
+                const int ti = bj[q];
+                bd[q] = d2;
+                bj[q] = id;
+                d2 = td;
+                id = ti;
+              }
+            }
           }
         }
     }
+    // slots of the block's table of staged G columns
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < kJ; ++q) {
+        if (bj[q] < 0) continue;
+        for (int k = 0; k < kSlots; ++k) {
+          const int cur = atomicCAS(&slot_j[k], -1, bj[q]);
+          if (cur == -1 || cur == bj[q]) {
+            slt[q] = k;
+            if (cur == -1) atomicAdd(&n_slots, 1);
+            break;
+          }
+        }
+      }
+    }
+    // local system on the slotted stations, in distance order: Cholesky of
+    // C_JJ = c0 - G_JJ, cut at the first small pivot
+    double Lj[kJ][kJ], kJv[kJ];
+    int mm = 0;
+#pragma unroll
+    for (int q = 0; q < kJ; ++q) {
+      kJv[q] = 0.0;
+#pragma unroll
+      for (int r = 0; r < kJ; ++r) Lj[q][r] = 0.0;
+    }
+    if (valid) {
+      bool ok = true;
+#pragma unroll
+      for (int q = 0; q < kJ; ++q) {
+        if (!ok || slt[q] < 0) {
+          ok = false;
+          slt[q] = -1;
+          continue;
+        }
+        const int jq = bj[q];
+        const double dq = c0 - Gs[(int64_t)jq * ld + jq];
+#pragma unroll
+        for (int r = 0; r < kJ; ++r) {
+          if (r < q) {
+            double v = c0 - Gs[(int64_t)bj[r] * ld + jq];
+#pragma unroll
+            for (int t = 0; t < kJ; ++t)
+              if (t < r) v -= Lj[q][t] * Lj[r][t];
+            Lj[q][r] = v / Lj[r][r];
+          }
+        }
+        double v = dq;
+#pragma unroll
+        for (int t = 0; t < kJ; ++t)
+          if (t < q) v -= Lj[q][t] * Lj[q][t];
+        if (v > 1e-8 * dq && dq > 0.0) {
+          Lj[q][q] = sqrt(v);
+          const int g = my[jq];
+          kJv[q] = c0 - vg_fast_entry(vf, dist2(x, y, sx[g], sy[g]), 0,
+                                      min_vg_val);
+          mm = q + 1;
+        } else {
+          ok = false;
+          slt[q] = -1;
+        }
+      }
+      // beta = C_JJ^-1 k_J; q_loc = 2 beta.k_J - beta^T C_JJ beta
+      double t[kJ];
+#pragma unroll
+      for (int q = 0; q < kJ; ++q) {
+        t[q] = 0.0;
+        if (q < mm) {
+          double v = kJv[q];
+#pragma unroll
+          for (int r = 0; r < kJ; ++r)
+            if (r < q) v -= Lj[q][r] * t[r];
+          t[q] = v / Lj[q][q];
+        }
+      }
+      double tt = 0.0;
+#pragma unroll
+      for (int q = 0; q < kJ; ++q) tt += t[q] * t[q];
+#pragma unroll
+      for (int q = kJ - 1; q >= 0; --q) {
+        if (q < mm) {
+          double v = t[q];
+#pragma unroll
+          for (int r = 0; r < kJ; ++r)
+            if (r > q && r < mm) v -= Lj[r][q] * beta[r];
+          beta[q] = v / Lj[q][q];
+        }
+      }
+      double bk = 0.0;
+#pragma unroll
+      for (int q = 0; q < kJ; ++q) {
+        bk += beta[q] * kJv[q];
+        sbeta += beta[q];
+      }
+      qloc = 2.0 * bk - tt;
+    }
   }
-  // column j(c) of the step's full semivariogram matrix
-  const double *gcol =
-      var_on ? Gfull + (int64_t)sl * ld * ld + (int64_t)bi * ld : nullptr;
-  double e = 0.0, a = 0.0, g_jc = 0.0;
+  double best = INFINITY;
+  int bi = 0;
+  double e = 0.0, a = 0.0;
+  // delta_i = k_i - sum_q (c0 - G_{i,J_q}) beta_q
+  //         = c0 (1 - sum beta) - g_i + sum_q G_{i,J_q} beta_q
+  const double dconst = c0 * (1.0 - sbeta);
   for (int base = 0; base < n; base += 256) {
     __syncthreads();
     if (base + tid < n) {
@@ -444,8 +717,15 @@ __global__ __launch_bounds__(256) void k_kgen(
       sal[tid] = alpha[base + tid];
       sw[tid] = w[base + tid];
     }
-    __syncthreads();
     const int cnt = min(256, n - base);
+    if (var_on) {
+      const int ns = n_slots;
+      for (int e2 = tid; e2 < ns * 256; e2 += 256) {
+        const int k = e2 >> 8, u = e2 & 255;
+        if (u < cnt) sG[k][u] = Gs[(int64_t)slot_j[k] * ld + base + u];
+      }
+    }
+    __syncthreads();
     if (valid) {
       if (var_on) {
         for (int j = 0; j < cnt; ++j) {
@@ -454,8 +734,10 @@ __global__ __launch_bounds__(256) void k_kgen(
           const double k = c0 - g;
           e += k * sal[j];
           a += k * sw[j];
-          const double delta = gcol[base + j] - g;
-          if (base + j == bi) g_jc = g;
+          double delta = dconst - g;
+#pragma unroll
+          for (int q = 0; q < kJ; ++q)
+            if (slt[q] >= 0) delta += sG[slt[q]][j] * beta[q];
           if (K32s)
             K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
           else
@@ -477,22 +759,21 @@ __global__ __launch_bounds__(256) void k_kgen(
     }
   }
   if (!valid) return;
+  const int64_t oc = perm ? (int64_t)perm[c] : c;
   double *eo = est + (int64_t)sl * ldo;
   const double *z = zact + (int64_t)sl * ldz;
   if (method == SPK_M_OK) {
-    eo[c] = e + stp->a0;
-    if (var_on) {
-      const double G0 = gcol[bi];  // G_jj: the semivariogram at lag 0
+    eo[oc] = e + stp->a0;
+    if (var_on)
       abuf[(int64_t)sl * ldk + c] =
-          2.0 * g_jc - G0 + (a - 1.0) * (a - 2.0) / stp->b;
-    }
+          c0 - qloc + (a - 1.0) * (a - 2.0) / stp->b;
   } else if (method == SPK_M_EDK) {
     double v = e + stp->h[0];
     for (int k = 0; k < m; ++k)
-      v += stp->h[1 + k] * cell_drift[(int64_t)k * C + c];
-    eo[c] = v;
+      v += stp->h[1 + k] * cell_drift[(int64_t)k * C + oc];
+    eo[oc] = v;
   } else {  // SK
-    eo[c] = isclose1(a) ? e : z[bi];
+    eo[oc] = isclose1(a) ? e : z[bi];
   }
 }
 
@@ -943,11 +1224,26 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
     SPK_TRY(ws_get(ctx, "qpart", S * nrb * ldk * 8, (void **)&qpart));
   }
-  // SPK_P_MIXED: condition estimates above which a step's variance runs in
-  // FP64 (systems with a Gaussian component lose accuracy in FP32 much
-  // sooner); the batched FP64 path runs every step in FP64
-  const double fp64var_thr = prec == SPK_P_MIXED ? 2e4 : INFINITY;
-  const double fp64var_thr_gau = prec == SPK_P_MIXED ? 1e3 : INFINITY;
+  // SPK_P_MIXED: condition estimate above which a step's variance runs in
+  // FP64.  The FP32 contraction loses about u * kappa^(1/2) to cancellation
+  // in L^-1 delta: measured on the hourly workload it stays within 1e-6
+  // (relative, floor 1e-3 x the step's mean variance) up to kappa ~ 90 and
+  // exceeds it from kappa ~ 130; the batched FP64 path runs every step in
+  // FP64.
+  const double kap = opts->fp64var_kappa;
+  const double fp64var_thr =
+      prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 50.0) : INFINITY;
+  const double fp64var_thr_gau = fp64var_thr;
+
+  // spatially compact cell order for the batched cell kernels
+  int32_t *perm = nullptr;
+  double *pcx = nullptr, *pcy = nullptr;
+  if (batched && C >= 1024) {
+    SPK_TRY(ws_get(ctx, "cell_perm", C * sizeof(int32_t), (void **)&perm));
+    SPK_TRY(ws_get(ctx, "cell_pxy", 2 * C * sizeof(double), (void **)&pcx));
+    pcy = pcx + C;
+    SPK_TRY(cell_order(ctx, cx, cy, C, perm, pcx, pcy));
+  }
 
   // second stream + events for the overlap
   if (!ctx->aux_stream)
@@ -1048,15 +1344,22 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     } else {
       for (int64_t cb = 0; cb < C; cb += Cs) {
         const int64_t cn = std::min<int64_t>(Cs, C - cb);
-        const double *cd = prob->cell_drift ? prob->cell_drift + cb : nullptr;
+        // with a cell order, stripes are ranges of the permuted cells and
+        // outputs / drifts are addressed through perm
+        const double *cd = prob->cell_drift
+                               ? (perm ? prob->cell_drift
+                                       : prob->cell_drift + cb)
+                               : nullptr;
         {
           ProfScope ps(ctx, "kgen");
           hipLaunchKernelGGL(k_kgen, dim3((unsigned)ceil_div(cn, 256), Sb),
                              dim3(256), 0, sm, sx, sy, q.idx, N, q.zact, ldz,
-                             q.sc, gb, cx + cb, cy + cb, cn, C, cd, q.V, 2 * ld,
-                             ld, method, m, mvg, want_var ? 1 : 0,
+                             q.sc, gb, (perm ? pcx : cx) + cb,
+                             (perm ? pcy : cy) + cb, cn, C, cd,
+                             perm ? perm + cb : (const int32_t *)nullptr, q.V,
+                             2 * ld, ld, method, m, mvg, want_var ? 1 : 0,
                              prec == SPK_P_MIXED ? 0 : 1, K32, K64, ldk, q.Gm,
-                             abuf, eb + cb, C);
+                             abuf, perm ? eb : eb + cb, C);
         }
         SPK_TRY(check_launch("k_kgen"));
         if (!want_var) continue;
@@ -1071,19 +1374,19 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         }
         {
           ProfScope ps(ctx, "gemm_tri_f64");
-          hipLaunchKernelGGL((k_gemm<double, false, 16>), grid, dim3(256), 0,
-                             sm, q.Li, ld, strideM, (const double *)K64, ldk,
-                             ld * ldk, q.sc, p, method, 1, q.zact, ldz,
-                             (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
-                             cn, eb + cb, vbo + cb, C,
-                             prec == SPK_P_MIXED ? 2 : 0, qpart, nrb);
+          const dim3 g64((unsigned)ceil_div(N, 128),
+                         (unsigned)ceil_div(cn, 128), Sb);
+          hipLaunchKernelGGL(k_tri64, g64, dim3(256), 0, sm, q.Li, ld, strideM,
+                             (const double *)K64, ldk, ld * ldk, q.sc, cn,
+                             qpart, ldk, nrb, prec == SPK_P_MIXED ? 2 : 0);
         }
         SPK_TRY(check_launch("k_gemm<tri>"));
         {
           ProfScope ps(ctx, "var_reduce");
           hipLaunchKernelGGL(k_var_reduce, dim3((unsigned)ceil_div(cn, 256), Sb),
                              dim3(256), 0, sm, q.sc, abuf, qpart, ldk, nrb, cn,
-                             vbo + cb, C);
+                             perm ? perm + cb : (const int32_t *)nullptr,
+                             perm ? vbo : vbo + cb, C);
         }
         SPK_TRY(check_launch("k_var_reduce"));
       }

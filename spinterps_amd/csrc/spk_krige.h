@@ -48,6 +48,11 @@ __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
 __global__ void k_recenter(const double *in, double *out, int64_t n, double o);
 __global__ void k_status_out(const StepSc *sc, int S, int32_t *info,
                              double *kappa);
+// Spatially compact order of the cells (Morton keys over the bounding box,
+// radix-sorted): perm[k] = original index of the k-th cell, pcx/pcy the
+// coordinates in that order.  Runs on ctx->stream.
+int cell_order(spk_ctx *ctx, const double *cx, const double *cy, int64_t C,
+               int32_t *perm, double *pcx, double *pcy);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
 

@@ -417,6 +417,116 @@ __global__ __launch_bounds__(256) void k_de(
   }
 }
 
+// ---- spatial order of cells -------------------------------------------------
+// Morton (Z-order) keys of the cells' coordinates quantised to 2^20 levels
+// per axis over their bounding box; sorting by key makes runs of 64 / 256
+// consecutive cells spatially compact tiles.
+__global__ void k_bbox_part(const double *__restrict__ x,
+                            const double *__restrict__ y, int64_t n,
+                            double *__restrict__ part) {
+  __shared__ double r[4][256];
+  double a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY;
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    a = fmin(a, x[i]);
+    b = fmax(b, x[i]);
+    c = fmin(c, y[i]);
+    d = fmax(d, y[i]);
+  }
+  r[0][threadIdx.x] = a;
+  r[1][threadIdx.x] = b;
+  r[2][threadIdx.x] = c;
+  r[3][threadIdx.x] = d;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r[0][threadIdx.x] = fmin(r[0][threadIdx.x], r[0][threadIdx.x + w]);
+      r[1][threadIdx.x] = fmax(r[1][threadIdx.x], r[1][threadIdx.x + w]);
+      r[2][threadIdx.x] = fmin(r[2][threadIdx.x], r[2][threadIdx.x + w]);
+      r[3][threadIdx.x] = fmax(r[3][threadIdx.x], r[3][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = r[threadIdx.x][0];
+}
+
+__device__ inline unsigned long long spread_bits(unsigned long long v) {
+  v &= 0xffffffffull;
+  v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+  v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+
+__global__ void k_morton(const double *__restrict__ x,
+                         const double *__restrict__ y, int64_t n,
+                         const double *__restrict__ part, int nparts,
+                         unsigned long long *__restrict__ key,
+                         int32_t *__restrict__ val) {
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int k = 0; k < nparts; ++k) {
+    x0 = fmin(x0, part[4 * k]);
+    x1 = fmax(x1, part[4 * k + 1]);
+    y0 = fmin(y0, part[4 * k + 2]);
+    y1 = fmax(y1, part[4 * k + 3]);
+  }
+  const double span = fmax(fmax(x1 - x0, y1 - y0), 1e-300);
+  const double sc = 1048575.0 / span;
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    double qx = (x[i] - x0) * sc, qy = (y[i] - y0) * sc;
+    qx = fmin(fmax(qx, 0.0), 1048575.0);  // NaN coordinates sort first
+    qy = fmin(fmax(qy, 0.0), 1048575.0);
+    const unsigned long long ix = (unsigned long long)qx;
+    const unsigned long long iy = (unsigned long long)qy;
+    key[i] = spread_bits(ix) | (spread_bits(iy) << 1);
+    val[i] = (int32_t)i;
+  }
+}
+
+__global__ void k_gather_xy(const double *__restrict__ x,
+                            const double *__restrict__ y,
+                            const int32_t *__restrict__ perm, int64_t n,
+                            double *__restrict__ px, double *__restrict__ py) {
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    px[i] = x[perm[i]];
+    py[i] = y[perm[i]];
+  }
+}
+
+int cell_order(spk_ctx *ctx, const double *cx, const double *cy, int64_t C,
+               int32_t *perm, double *pcx, double *pcy) {
+  if (C > INT32_MAX) return set_error(SPK_ESHAPE, "too many cells");
+  constexpr int kParts = 256;
+  hipStream_t st = ctx->stream;
+  double *part;
+  unsigned long long *k0, *k1;
+  int32_t *v0;
+  SPK_TRY(ws_get(ctx, "co_part", kParts * 4 * 8, (void **)&part));
+  SPK_TRY(ws_get(ctx, "co_k0", C * 8, (void **)&k0));
+  SPK_TRY(ws_get(ctx, "co_k1", C * 8, (void **)&k1));
+  SPK_TRY(ws_get(ctx, "co_v0", C * 4, (void **)&v0));
+  const unsigned g = (unsigned)std::min<int64_t>(ceil_div(C, 256), 4096);
+  hipLaunchKernelGGL(k_bbox_part, dim3(kParts), dim3(256), 0, st, cx, cy, C,
+                     part);
+  hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, st, cx, cy, C, part,
+                     kParts, k0, v0);
+  SPK_TRY(check_launch("k_morton"));
+  size_t tmp_bytes = 0;
+  SPK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, v0,
+                                             perm, (int)C, 0, 40, st));
+  void *tmp;
+  SPK_TRY(ws_get(ctx, "co_tmp", tmp_bytes, &tmp));
+  SPK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, perm,
+                                             (int)C, 0, 40, st));
+  hipLaunchKernelGGL(k_gather_xy, dim3(g), dim3(256), 0, st, cx, cy, perm, C,
+                     pcx, pcy);
+  return check_launch("k_gather_xy");
+}
+
 }  // namespace spk
 
 using namespace spk;

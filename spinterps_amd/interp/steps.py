@@ -115,7 +115,7 @@ def _ctx_for(device):
 def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
                  precision='fp64', want_var=True, idw_exp=2.0,
                  min_vg_val=0.0, min_var_thr=-np.inf, est=None, var=None,
-                 vg_dev=None, kappa=None):
+                 vg_dev=None, kappa=None, fp64var_kappa=0.0):
     """Interpolate T steps.  vals [T x N] (NaN = missing station).
 
     Returns (est [T x C] float64, var [T x C] float64 or None,
@@ -124,7 +124,9 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     (nearest-station fill), 8 single station, 16 ill-conditioned
     (condition estimate > 1e12), 32 variance promoted to FP64 (mixed).
     ``kappa``: optional float64 device tensor [T] that receives each
-    system's condition estimate.
+    system's condition estimate.  ``fp64var_kappa`` (mixed precision):
+    condition estimate above which a step's variance contraction runs in
+    FP64 (0 = the library's policy).
     """
     dev = prob.device
     v = _t64(vals, dev)
@@ -156,6 +158,7 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     opts.idw_exp = float(idw_exp)
     opts.min_vg_val = float(min_vg_val)
     opts.min_var_thr = float(min_var_thr)
+    opts.fp64var_kappa = float(fp64var_kappa)
     ctx = _ctx_for(dev)
     pst = prob.c_struct()
     check(lib().spk_interp_steps_dev(
