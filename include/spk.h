@@ -103,6 +103,23 @@ int spk_sel_equidist_refs(spk_ctx *ctx, double dst_x, double dst_y,
                           int64_t *ref_sel_pie_idxs, uint64_t *ref_pie_idxs,
                           uint64_t *ref_pie_cts);
 
+/* ---- class API (pyx:251-719) ----------------------------------------- */
+/* kind 0: OrdinaryKriging (pyx:251-362), 1: SimpleKriging (pyx:365-471),
+ * 2: ExternalDriftKriging_MD (pyx:588-719).  Host pointers; n input points
+ * (xi, yi, zi), C output points (xk, yk); EDK_MD drifts si [m x n] and
+ * sk [m x C].  P = n + 1 (OK), n (SK), n + 1 + m (EDK_MD).  Outputs (any
+ * may be NULL except zk): zk [C]; est_vars [C] (OK est_vars, SK
+ * est_covars); lambdas [C x n]; mus [C] (OK) or [(m+1) x C] (EDK_MD
+ * mus_arr); rhss [C x P]; in_vars [P x P] (SK: in_covars); in_vars_inv
+ * [P x P]; in_dists [n x n].  SPK_ENOTSPD when the system is singular. */
+int spk_krige_class(spk_ctx *ctx, int kind, const char *model,
+                    const double *xi, const double *yi, const double *zi,
+                    int64_t n, const double *si, int m, const double *xk,
+                    const double *yk, const double *sk, int64_t C,
+                    double *zk, double *est_vars, double *lambdas,
+                    double *mus, double *rhss, double *in_vars,
+                    double *in_vars_inv, double *in_dists);
+
 /* ---- neighbour selection for all cells (interp/grps.py:141-247) ----- */
 /* nrst: out[n_dst x min(n_nebs, n_refs)] sorted indices of the nearest. */
 int spk_neb_idxs_nrst(spk_ctx *ctx, const double *dst_xs, const double *dst_ys,

@@ -128,6 +128,11 @@ SIGNATURES = {
     'spk_vg_features_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_int64, c_int64, c_void_p, c_int64,
                                     c_double, c_int, c_void_p, c_void_p]),
+    'spk_krige_class': (c_int, [c_void_p, c_int, c_char_p, c_void_p,
+                                c_void_p, c_void_p, c_int64, c_void_p, c_int,
+                                c_void_p, c_void_p, c_void_p, c_int64,
+                                c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
     'spk_mlp_n_params': (c_int64, []),
     'spk_mlp_init_dev': (c_int, [c_void_p, c_uint64, c_void_p]),
     'spk_mlp_forward_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_int,

@@ -15,7 +15,8 @@ from ._lib import check, context, dptr, i64ptr, lib, u64ptr
 
 __all__ = ['fill_dists_one_pt', 'fill_wts_and_sum', 'get_mults_sum',
            'sel_equidist_refs', 'fill_dists_2d_mat', 'fill_vg_var_arr',
-           'fill_theo_vg_vals', 'copy_2d_arr_at_idxs', 'get_nd_dists']
+           'fill_theo_vg_vals', 'copy_2d_arr_at_idxs', 'get_nd_dists',
+           'OrdinaryKriging', 'SimpleKriging', 'ExternalDriftKriging_MD']
 
 
 def _ctx():
@@ -128,3 +129,130 @@ def sel_equidist_refs(dst_x, dst_y, ref_xs, ref_ys, n_pies, min_dist_thresh,
         dptr(dists), dptr(tem_ref_sel_dists), i64ptr(ref_sel_pie_idxs),
         u64ptr(ref_pie_idxs), u64ptr(ref_pie_cts)))
 
+
+
+# ---- class API (pyx:251-719) ------------------------------------------------
+class _KrigeClass:
+    """Common part of the kriging classes: inputs as the reference stores
+    them, outputs filled by krige() on the GPU."""
+
+    _kind = 0
+
+    def _run(self, si=None, sk=None):
+        xi = np.ascontiguousarray(self.xi, np.float64)
+        yi = np.ascontiguousarray(self.yi, np.float64)
+        zi = np.ascontiguousarray(self.zi, np.float64)
+        xk = np.ascontiguousarray(self.xk, np.float64)
+        yk = np.ascontiguousarray(self.yk, np.float64)
+        n, C = xi.size, xk.size
+        m = 0 if si is None else si.shape[0]
+        P = n + (0 if self._kind == 1 else 1 + m)
+        zk = np.zeros(C)
+        ev = np.zeros(C)
+        lam = np.zeros((C, n))
+        mus = np.zeros((1 + m, C)) if self._kind == 2 else np.zeros(C)
+        rhss = np.zeros((C, P))
+        kin = np.zeros((P, P))
+        kinv = np.zeros((P, P))
+        dists = np.zeros((n, n))
+        if si is not None:
+            si = np.ascontiguousarray(si, np.float64)
+            sk = np.ascontiguousarray(sk, np.float64)
+        check(lib().spk_krige_class(
+            _ctx(), self._kind, self.model, dptr(xi), dptr(yi), dptr(zi), n,
+            dptr(si) if si is not None else None, m, dptr(xk), dptr(yk),
+            dptr(sk) if sk is not None else None, C, dptr(zk),
+            dptr(ev) if self._kind != 2 else None, dptr(lam),
+            dptr(mus) if self._kind != 1 else None, dptr(rhss), dptr(kin),
+            dptr(kinv), dptr(dists)))
+        return zk, ev, lam, mus, rhss, kin, kinv, dists
+
+
+class OrdinaryKriging(_KrigeClass):
+    """pyx:251-362: zero-diagonal semivariogram matrix, right-hand sides
+    that skip stations at distance 0, pinv-equivalent inverse, variance
+    max(0, lambda.gamma + mu)."""
+
+    _kind = 0
+
+    def __init__(self, xi, yi, zi, xk, yk, model='1.0 Sph(2)'):
+        self.xi, self.yi, self.zi, self.xk, self.yk = xi, yi, zi, xk, yk
+        self.model = bytes(model, 'utf-8')
+        self.in_count = self.xi.shape[0]
+        self.out_count = self.xk.shape[0]
+        n, C = self.in_count, self.out_count
+        self.zk = np.zeros(C)
+        self.in_dists = np.zeros((n, n))
+        self.mus = np.zeros(C)
+        self.est_vars = np.zeros(C)
+        self.lambdas = np.zeros((C, n))
+        self.in_vars = np.zeros((n + 1, n + 1))
+        self.in_vars_inv = self.in_vars.copy()
+        self.rhss = np.zeros((C, n + 1))
+
+    def krige(self):
+        (self.zk, self.est_vars, self.lambdas, self.mus, self.rhss,
+         self.in_vars, self.in_vars_inv, self.in_dists) = self._run()
+
+
+class SimpleKriging(_KrigeClass):
+    """pyx:365-471, including its order of initialisation: the covariance
+    matrix is filled before the sills are summed, so it is -Gamma with a
+    zero diagonal; est_covars = max(0, covar - lambda.rhs)."""
+
+    _kind = 1
+
+    def __init__(self, xi, yi, zi, xk, yk, model='1.0 Sph(2)'):
+        self.xi, self.yi, self.zi, self.xk, self.yk = xi, yi, zi, xk, yk
+        self.model = bytes(model, 'utf-8')
+        self.in_count = self.xi.shape[0]
+        self.out_count = self.xk.shape[0]
+        n, C = self.in_count, self.out_count
+        self.zk = np.zeros(C)
+        self.in_dists = np.zeros((n, n))
+        self.est_covars = np.zeros(C)
+        self.lambdas = np.zeros((C, n))
+        self.in_covars = np.zeros((n, n))
+        self.in_covars_inv = self.in_covars.copy()
+        self.out_covars = np.zeros((C, n))
+        self.rhss = np.zeros((C, n))
+
+    def krige(self):
+        (self.zk, self.est_covars, self.lambdas, _, self.rhss,
+         self.in_covars, self.in_covars_inv, self.in_dists) = self._run()
+        self.out_covars = self.rhss
+
+
+class ExternalDriftKriging_MD(_KrigeClass):
+    """pyx:588-719: OK system with a [1 | drifts] border; mus_arr holds the
+    m + 1 Lagrange multipliers per output point."""
+
+    _kind = 2
+
+    def __init__(self, xi, yi, zi, si, xk, yk, sk, model='1.0 Sph(2)'):
+        assert len(xi.shape) == 1
+        assert len(si.shape) == 2
+        assert xi.shape[0] == yi.shape[0] == zi.shape[0] == si.shape[1], \
+            'Observation points and drift shapes are unequal!'
+        assert xk.shape[0] == yk.shape[0] == sk.shape[1], \
+            'Resulting points and drifts shapes are unequal!'
+        assert si.shape[0] == sk.shape[0], \
+            'Observation and reulting drifts have unequal shapes!'
+        self.xi, self.yi, self.zi, self.si = xi, yi, zi, si
+        self.xk, self.yk, self.sk = xk, yk, sk
+        self.model = bytes(model, 'utf-8')
+        self.n_drifts = sk.shape[0]
+        self.in_count = self.xi.shape[0]
+        self.out_count = self.xk.shape[0]
+        n, C, m = self.in_count, self.out_count, self.n_drifts
+        self.zk = np.zeros(C)
+        self.in_dists = np.zeros((n, n))
+        self.mus_arr = np.zeros((m + 1, C))
+        self.lambdas = np.zeros((C, n))
+        self.in_vars = np.zeros((n + m + 1, n + m + 1))
+        self.in_vars_inv = self.in_vars.copy()
+        self.rhss = np.zeros((C, n + m + 1))
+
+    def krige(self):
+        (self.zk, _, self.lambdas, self.mus_arr, self.rhss, self.in_vars,
+         self.in_vars_inv, self.in_dists) = self._run(self.si, self.sk)
