@@ -44,6 +44,13 @@ def main():
                     help='simulated fields for the variogram-fit leg '
                          '(0 skips it)')
     ap.add_argument('--cpu-vg-fields', type=int, default=4)
+    ap.add_argument('--stress', type=int, default=0,
+                    help='also run this many steps of the ill-conditioned '
+                         'stress set (daily inputs, Gaussian model, nugget '
+                         '1e-6, 10 %% of stations in 1-10 m pairs) and '
+                         'report errors vs numpy.linalg.solve')
+    ap.add_argument('--stress-cells', type=int, default=200)
+    ap.add_argument('--stress-oracle-steps', type=int, default=20)
     ap.add_argument('--calib', action='store_true',
                     help='also run k_clip_cast over 2^28 float64 values '
                          '(2 GiB read, 1 GiB written) as the reference '
@@ -261,6 +268,7 @@ def main():
         log('calibration copy done: k_clip_cast, 2 GiB read + 1 GiB write')
 
     vgfit = vgfit_leg(args, log) if args.vg_fields > 0 else None
+    stress = stress_leg(args, log) if args.stress > 0 else None
 
     out = {
         'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '
@@ -303,7 +311,82 @@ def main():
         'cpu_baseline': cpu,
         'variograms_fitted': vgfit,
     }
+    if stress is not None:
+        out['stress'] = stress
     print(json.dumps(out), flush=True)
+
+
+def stress_leg(args, log):
+    """Ill-conditioned stress set on the daily inputs: OK estimate and
+    variance on the full 500 x 500 grid by the plain FP64 and the mixed
+    paths, and errors of both on sampled cells against the oracle's
+    numpy.linalg.solve of the reference's bordered system."""
+    import numpy as np
+    import torch
+
+    from oracle import spk_oracle as O
+    from spinterps_amd import _lib, configs
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.daily()
+    sx, sy = configs.stress_stations(w)
+    vals, vgs = configs.stress_steps(w, 0, args.stress)
+    T = vals.shape[0]
+    prob = DeviceProblem(sx, sy, w.cell_x, w.cell_y)
+    res = {}
+    outs = {}
+    for prec in ('fp64', 'mixed'):
+        kap = torch.zeros(T, dtype=torch.float64, device=prob.device)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=prec,
+                                  kappa=kap)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        inf = info.cpu().numpy()
+        outs[prec] = (e, v, inf, kap.cpu().numpy())
+        res[prec] = {'seconds': dt,
+                     'cell_estimates_per_s': T * w.n_cells / dt,
+                     'steps_failed_nnb_fill': int(((inf & 4) != 0).sum()),
+                     'steps_ill_conditioned': int(((inf & 16) != 0).sum()),
+                     'steps_fp64_variance': int(((inf & 32) != 0).sum())}
+    rng = np.random.default_rng(11)
+    cidx = rng.choice(w.n_cells, args.stress_cells, replace=False)
+    cx, cy = w.cell_x[cidx], w.cell_y[cidx]
+    errs = {p: ([], []) for p in outs}
+    used = []
+    for t in range(min(T, args.stress_oracle_steps)):
+        act = ~np.isnan(vals[t])
+        xs, ys, z = sx[act], sy[act], vals[t][act]
+        rr = O.dists_2d(xs, ys, xs, ys)
+        dr = O.dists_2d(cx, cy, xs, ys)
+        K, R = O.krig_system('OK', vgs[t], rr, dr, None, None, 0.0)
+        est, var, _ = O.krige_solve(K, R, z, dr, True)
+        for p, (e, v, inf, kp) in outs.items():
+            if inf[t] & 4:      # nearest-station fill: not comparable
+                continue
+            ge = e[t].cpu().numpy()[cidx]
+            gv = v[t].cpu().numpy()[cidx]
+            fe = 1e-3 * np.abs(est).mean()
+            fv = 1e-3 * np.abs(var).mean()
+            errs[p][0].append(np.abs(ge - est) / np.maximum(np.abs(est), fe))
+            errs[p][1].append(np.abs(gv - var) / np.maximum(np.abs(var), fv))
+        used.append(t)
+    for p, (le, lv) in errs.items():
+        if le:
+            ee, ev = np.concatenate(le), np.concatenate(lv)
+            res[p].update({'est_max': float(ee.max()),
+                           'est_median': float(np.median(ee)),
+                           'var_max': float(ev.max()),
+                           'var_median': float(np.median(ev)),
+                           'steps_compared': len(le)})
+    kp = outs['fp64'][3]
+    res['kappa_median'] = float(np.median(kp))
+    res['kappa_max'] = float(kp.max())
+    res['oracle'] = (f'numpy.linalg.solve of [[G,1],[1,0]] on {len(used)} '
+                     f'steps x {args.stress_cells} cells; floors 1e-3 x mean '
+                     '|estimate| and 1e-3 x mean variance')
+    log(f'stress: {res}')
+    return res
 
 
 def vgfit_leg(args, log):

@@ -429,6 +429,23 @@ def krige_contract(Kinv, R, z, dst_ref_dists, want_var):
     return est, var, ok
 
 
+def krige_solve(K, R, z, dst_ref_dists, want_var):
+    """krige_contract with lambda from numpy.linalg.solve(K, rhs) instead of
+    the pseudo-inverse: the FP64 reference for ill-conditioned stress
+    systems, where pinv's rcond cut and a direct solve differ."""
+    n = z.size
+    L = np.linalg.solve(K, R.T).T
+    lsum = L[:, :n].sum(axis=1)
+    ok = np.isclose(lsum, 1.0)
+    nnb = np.argmin(dst_ref_dists, axis=1)
+    est = np.where(ok, (L[:, :n] * z[None, :]).sum(axis=1), z[nnb])
+    var = None
+    if want_var:
+        v = (L * R).sum(axis=1) + L[:, n]
+        var = np.where(ok, v, 0.0)
+    return est, var, ok
+
+
 def get_interp(ref_data, interp_type, ref_drfts, dst_drfts, idw_exp, models,
                interp_steps_flags, nuggetness_flags, dst_ref_dists,
                ref_ref_dists, min_vg_val, want_var, prblm=None,

@@ -16,6 +16,7 @@
 //  k_special     one-station, mean-fill and failed-system steps
 //                (interp/steps.py:280-283, 325-331, 372-384)
 #include <algorithm>
+#include <vector>
 
 #include "spk_krige.h"
 
@@ -518,8 +519,24 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
 // LDS per station chunk (a cell's stations without a slot leave its J).
 // Cells are stripe-local indices c; with perm the outputs and the cell
 // drifts are addressed at the original index perm[c].
-constexpr int kJ = 4, kSlots = 16;
-__global__ __launch_bounds__(256) void k_kgen(
+constexpr int kJ = 4, kSlots = 8;
+template <bool GEN>
+__device__ inline double kg_vg(const VgFast &f, double h, int covar_flag,
+                               double min_vg_val) {
+  return GEN ? vg_fast_entry(f, h, covar_flag, min_vg_val)
+             : vg_simple_entry(f, h, covar_flag, min_vg_val);
+}
+
+// 1 where a step's variogram has no power or hole-effect component
+__global__ void k_vg_kinds(const spk_vg *__restrict__ vgs, int64_t T,
+                           uint8_t *__restrict__ simple) {
+  for (int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x; t < T;
+       t += (int64_t)gridDim.x * 256)
+    simple[t] = vg_simple(vgs[t]) ? 1 : 0;
+}
+
+template <bool GEN>
+__global__ __launch_bounds__(256, 5) void k_kgen(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ zact, int64_t ldz,
@@ -659,7 +676,7 @@ __global__ __launch_bounds__(256) void k_kgen(
         if (v > 1e-8 * dq && dq > 0.0) {
           Lj[q][q] = sqrt(v);
           const int g = my[jq];
-          kJv[q] = c0 - vg_fast_entry(vf, dist2(x, y, sx[g], sy[g]), 0,
+          kJv[q] = c0 - kg_vg<GEN>(vf, dist2(x, y, sx[g], sy[g]), 0,
                                       min_vg_val);
           mm = q + 1;
         } else {
@@ -730,7 +747,7 @@ __global__ __launch_bounds__(256) void k_kgen(
       if (var_on) {
         for (int j = 0; j < cnt; ++j) {
           const double d = dist2(x, y, stx[j], sty[j]);
-          const double g = vg_fast_entry(vf, d, 0, min_vg_val);
+          const double g = kg_vg<GEN>(vf, d, 0, min_vg_val);
           const double k = c0 - g;
           e += k * sal[j];
           a += k * sw[j];
@@ -750,7 +767,7 @@ __global__ __launch_bounds__(256) void k_kgen(
             best = d;
             bi = base + j;
           }
-          const double g = vg_fast_entry(vf, d, cflag, min_vg_val);
+          const double g = kg_vg<GEN>(vf, d, cflag, min_vg_val);
           const double k = cflag ? g : c0 - g;
           e += k * sal[j];
           a += k * sw[j];
@@ -1235,6 +1252,21 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 50.0) : INFINITY;
   const double fp64var_thr_gau = fp64var_thr;
 
+  // which steps have only simple variogram components (the cell kernels
+  // are specialised for them)
+  std::vector<uint8_t> vg_simple_h;
+  if (batched) {
+    uint8_t *vk;
+    SPK_TRY(ws_get(ctx, "vg_kinds", T, (void **)&vk));
+    hipLaunchKernelGGL(k_vg_kinds,
+                       dim3((unsigned)std::min<int64_t>(ceil_div(T, 256), 4096)),
+                       dim3(256), 0, sm, vgs, T, vk);
+    SPK_TRY(check_launch("k_vg_kinds"));
+    vg_simple_h.resize(T);
+    SPK_TRY(d2h(ctx, vg_simple_h.data(), vk, T));
+    SPK_TRY(sync(ctx));
+  }
+
   // spatially compact cell order for the batched cell kernels
   int32_t *perm = nullptr;
   double *pcx = nullptr, *pcy = nullptr;
@@ -1311,6 +1343,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
 
   auto contract = [&](int64_t t0, int Sb, Set &q) -> int {
     const spk_vg *gb = vgs + t0;
+    bool simple = batched;
+    for (int s = 0; batched && s < Sb; ++s) simple = simple && vg_simple_h[t0 + s];
     double *eb = est + t0 * C;
     double *vbo = want_var ? var + t0 * C : nullptr;
     if (!batched) {
@@ -1352,7 +1386,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                : nullptr;
         {
           ProfScope ps(ctx, "kgen");
-          hipLaunchKernelGGL(k_kgen, dim3((unsigned)ceil_div(cn, 256), Sb),
+          hipLaunchKernelGGL(simple ? k_kgen<false> : k_kgen<true>,
+                             dim3((unsigned)ceil_div(cn, 256), Sb),
                              dim3(256), 0, sm, sx, sy, q.idx, N, q.zact, ldz,
                              q.sc, gb, (perm ? pcx : cx) + cb,
                              (perm ? pcy : cy) + cb, cn, C, cd,

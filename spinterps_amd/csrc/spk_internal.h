@@ -210,6 +210,38 @@ __device__ inline double vg_fast_entry(const VgFast &f, double h,
   return acc;
 }
 
+// vg_fast_entry for records whose components are all Rng / Nug / Sph / Exp
+// / Gau / Lin (vg_simple); the power and hole-effect branches are left
+// out, which keeps register use low in the cell kernels.
+__device__ inline double vg_simple_entry(const VgFast &f, double h,
+                                         int covar_flag, double min_vg_val) {
+  double acc = 0.0;
+  for (int k = 0; k < f.n; ++k) {
+    const double s = f.s[k];
+    double g;
+    switch (f.type[k]) {
+      case SPK_VG_RNG: g = h; break;
+      case SPK_VG_NUG: g = s; break;
+      case SPK_VG_SPH:
+        g = (h >= f.r[k]) ? s : s * (h * f.c1[k] - (h * h * h) * f.c2[k]);
+        break;
+      case SPK_VG_EXP: g = s * (1.0 - exp(h * f.c1[k])); break;
+      case SPK_VG_GAU: g = s * (1.0 - exp((h * h) * f.c1[k])); break;
+      case SPK_VG_LIN: g = (h > f.r[k]) ? s : s * (h * f.c1[k]); break;
+      default: g = NAN;
+    }
+    acc += covar_flag ? (s - g) : g;
+  }
+  if (acc <= min_vg_val) acc = 0.0;
+  return acc;
+}
+
+__host__ __device__ inline bool vg_simple(const spk_vg &vg) {
+  for (int k = 0; k < vg.n; ++k)
+    if (vg.type[k] == SPK_VG_POW || vg.type[k] == SPK_VG_HOL) return false;
+  return true;
+}
+
 // True when every component is bounded by its sill, so that
 // c0 = sum of sills turns the semivariogram matrix into a covariance.
 inline bool vg_bounded(const spk_vg &vg) {
