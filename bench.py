@@ -24,6 +24,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 
+# HBM bytes per k_tri64 dispatch measured by rocprofv3 --pmc on this
+# workload (2 x 8.78 GB fetched + 0.04 GB written, calibrated with
+# k_clip_cast); the algorithmic operands are about 3.1 GB per dispatch
+TRI64_TRAFFIC = 2 * 8776922 * 1024 + 42411 * 1024
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -294,14 +300,18 @@ def main():
                        'gemm_tflops': achp},
         'error_mixed_vs_fp64': err,
         'error_vs_oracle': err_oracle,
-        'roofline': {'bound': 'mfma', 'achieved': ach32, 'peak': 157.3,
-                     'unit': 'TFLOP/s', 'frac': ach32 / 157.3,
-                     'traffic': None,
-                     'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA)'},
+        'roofline': {'bound': 'mfma', 'achieved': ach64, 'peak': 78.6,
+                     'unit': 'TFLOP/s', 'frac': ach64 / 78.6,
+                     'traffic': TRI64_TRAFFIC,
+                     'kernel': 'k_tri64 (L^-1 delta, FP64 MFMA), the '
+                               'dominant kernel',
+                     'traffic_note': 'HBM bytes per dispatch, FETCH_SIZE x2 '
+                                     '+ WRITE_SIZE x1 (calibrated), from '
+                                     'profiles/kernels_hourly_mixed_b64.md'},
         'rooflines': {
-            'gemm_tri_f64': {'bound': 'mfma', 'achieved': ach64,
-                             'peak': 78.6, 'unit': 'TFLOP/s',
-                             'frac': ach64 / 78.6},
+            'gemm_tri_f32': {'bound': 'mfma', 'achieved': ach32,
+                             'peak': 157.3, 'unit': 'TFLOP/s',
+                             'frac': ach32 / 157.3},
             'kgen': {'bound': 'hbm', 'achieved': kg_gbs, 'peak': 8000.0,
                      'unit': 'GB/s', 'frac': kg_gbs / 8000.0},
             'gemm_plain_f64': {'bound': 'mfma', 'achieved': achp,
