@@ -56,6 +56,11 @@ def main():
                          '1e-6, 10 %% of stations in 1-10 m pairs) and '
                          'report errors vs numpy.linalg.solve')
     ap.add_argument('--stress-cells', type=int, default=200)
+    ap.add_argument('--e2e', type=int, default=0,
+                    help='also run this many hourly steps end to end: '
+                         'proposal-seeded variogram fits of the step '
+                         'values, then batched mixed OK with the fitted '
+                         'variograms')
     ap.add_argument('--stress-oracle-steps', type=int, default=20)
     ap.add_argument('--calib', action='store_true',
                     help='also run k_clip_cast over 2^28 float64 values '
@@ -275,6 +280,7 @@ def main():
 
     vgfit = vgfit_leg(args, log) if args.vg_fields > 0 else None
     stress = stress_leg(args, log) if args.stress > 0 else None
+    e2e = e2e_leg(args, log) if args.e2e > 0 else None
 
     out = {
         'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '
@@ -323,7 +329,51 @@ def main():
     }
     if stress is not None:
         out['stress'] = stress
+    if e2e is not None:
+        out['end_to_end'] = e2e
     print(json.dumps(out), flush=True)
+
+
+def e2e_leg(args, log):
+    """Fit + interpolate: each hourly step's variogram is fitted from its
+    own values by the proposal-seeded fit (fitvgs.py settings, Sph/Exp/Gau
+    with a nugget), then all steps are kriged (OK + variance) by the
+    batched mixed path with those strings."""
+    import numpy as np
+    import torch
+
+    from spinterps_amd import configs
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    from spinterps_amd.variograms.learned import (LearnedVariogramFitter,
+                                                  load_proposal_net)
+    T = args.e2e
+    w = configs.hourly(n_side=args.grid, cell=5e5 / args.grid)
+    vals, _ = configs.hourly_steps(w, 0, T)
+    prob = DeviceProblem.from_workload(w)
+    lf = LearnedVariogramFitter(w.stn_x, w.stn_y, load_proposal_net())
+    lf.fit(vals[:4], verify=False, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    strs = lf.fit(vals, verify=False, seed=2)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    ok = [s is not None for s in strs]
+    vgs = [s if s is not None else '1.0 Nug(0.0)' for s in strs]
+    est = torch.empty((T, w.n_cells), dtype=torch.float64, device=prob.device)
+    var = torch.empty_like(est)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    interp_steps(prob, vals, vgs, 'OK', precision='mixed', est=est, var=var)
+    torch.cuda.synchronize()
+    t_ok = time.perf_counter() - t1
+    res = {'steps': T, 'fitted': int(sum(ok)), 'fit_seconds': t_fit,
+           'variograms_fitted_per_s': sum(ok) / t_fit,
+           'krige_seconds': t_ok,
+           'cell_estimates_per_s_krige': T * w.n_cells / t_ok,
+           'cell_estimates_per_s_end_to_end': T * w.n_cells / (t_fit + t_ok),
+           'example_vg': strs[0]}
+    log(f'end to end: {res}')
+    return res
 
 
 def stress_leg(args, log):
