@@ -514,9 +514,11 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
 // flagged SPK_ST_FP64VAR (or the FP64 batched path), to K64.  Any beta gives
 // the same variance in exact arithmetic, so J is cut where C_JJ's Cholesky
 // pivot falls below 1e-8 of its diagonal.
-// Cells come in a spatially compact order, so the 256 cells of a block share
-// few distinct J stations: up to kSlots of their G columns are staged in
-// LDS per station chunk (a cell's stations without a slot leave its J).
+// Cells come in a spatially compact order.  The nearest stations of a
+// block's 256 cells fill a table of up to kSlots stations whose G columns
+// are staged in LDS per station chunk; each cell's J is its kJ nearest
+// stations in that table, so every cell gets a local fit even when the
+// block's cells have more distinct nearest stations than slots.
 // Cells are stripe-local indices c; with perm the outputs and the cell
 // drifts are addressed at the original index perm[c].
 constexpr int kJ = 4, kSlots = 8;
@@ -585,15 +587,9 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
   }
   double qloc = 0.0, sbeta = 0.0;
   if (var_on) {
-    // pass 1: the cell's kJ nearest active stations (ascending distance,
-    // ties to the lower index)
-    double bd[kJ];
-    int bj[kJ];
-#pragma unroll
-    for (int q = 0; q < kJ; ++q) {
-      bd[q] = INFINITY;
-      bj[q] = -1;
-    }
+    // pass 1: the cell's nearest active station (ties to the lower index)
+    double bd = INFINITY;
+    int bj = -1;
     for (int base = 0; base < n; base += 256) {
       __syncthreads();
       if (base + tid < n) {
@@ -606,40 +602,55 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
       if (valid)
         for (int j = 0; j < cnt; ++j) {
           const double dx = x - stx[j], dy = y - sty[j];
-          double d2 = dx * dx + dy * dy;
-          if (d2 < bd[kJ - 1]) {
-            int id = base + j;
-#pragma unroll
-            for (int q = 0; q < kJ; ++q) {  // insertion, strict: stable
-              if (d2 < bd[q]) {
-                const double td = bd[q];
-                const int ti = bj[q];
-                bd[q] = d2;
-                bj[q] = id;
-                d2 = td;
-                id = ti;
-              }
-            }
+          const double d2 = dx * dx + dy * dy;
+          if (d2 < bd) {
+            bd = d2;
+            bj = base + j;
           }
         }
     }
-    // slots of the block's table of staged G columns
-    if (valid) {
-#pragma unroll
-      for (int q = 0; q < kJ; ++q) {
-        if (bj[q] < 0) continue;
-        for (int k = 0; k < kSlots; ++k) {
-          const int cur = atomicCAS(&slot_j[k], -1, bj[q]);
-          if (cur == -1 || cur == bj[q]) {
-            slt[q] = k;
-            if (cur == -1) atomicAdd(&n_slots, 1);
-            break;
-          }
+    // the block's table of staged stations: the cells' nearest stations,
+    // first come first served
+    if (valid && bj >= 0) {
+      for (int k = 0; k < kSlots; ++k) {
+        const int cur = atomicCAS(&slot_j[k], -1, bj);
+        if (cur == -1 || cur == bj) {
+          if (cur == -1) atomicAdd(&n_slots, 1);
+          break;
         }
       }
     }
-    // local system on the slotted stations, in distance order: Cholesky of
-    // C_JJ = c0 - G_JJ, cut at the first small pivot
+    __syncthreads();
+    const int ns = n_slots;
+    if (tid < ns) {
+      const int g = my[slot_j[tid]];
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+    }
+    __syncthreads();
+    // J: the cell's kJ nearest stations among the table (ascending)
+    double jd[kJ];
+#pragma unroll
+    for (int q = 0; q < kJ; ++q) jd[q] = INFINITY;
+    if (valid)
+      for (int k = 0; k < ns; ++k) {
+        const double dx = x - stx[k], dy = y - sty[k];
+        double d2 = dx * dx + dy * dy;
+        int id = k;
+#pragma unroll
+        for (int q = 0; q < kJ; ++q) {  // insertion, strict: stable
+          if (d2 < jd[q]) {
+            const double td = jd[q];
+            const int ti = slt[q];
+            jd[q] = d2;
+            slt[q] = id;
+            d2 = td;
+            id = ti;
+          }
+        }
+      }
+    // local system on J in distance order: Cholesky of C_JJ = c0 - G_JJ,
+    // cut at the first small pivot
     double Lj[kJ][kJ], kJv[kJ];
     int mm = 0;
 #pragma unroll
@@ -657,27 +668,26 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
           slt[q] = -1;
           continue;
         }
-        const int jq = bj[q];
+        const int jq = slot_j[slt[q]];
         const double dq = c0 - Gs[(int64_t)jq * ld + jq];
 #pragma unroll
         for (int r = 0; r < kJ; ++r) {
           if (r < q) {
-            double v = c0 - Gs[(int64_t)bj[r] * ld + jq];
+            double v = c0 - Gs[(int64_t)slot_j[slt[r]] * ld + jq];
 #pragma unroll
-            for (int t = 0; t < kJ; ++t)
-              if (t < r) v -= Lj[q][t] * Lj[r][t];
+            for (int t2 = 0; t2 < kJ; ++t2)
+              if (t2 < r) v -= Lj[q][t2] * Lj[r][t2];
             Lj[q][r] = v / Lj[r][r];
           }
         }
         double v = dq;
 #pragma unroll
-        for (int t = 0; t < kJ; ++t)
-          if (t < q) v -= Lj[q][t] * Lj[q][t];
+        for (int t2 = 0; t2 < kJ; ++t2)
+          if (t2 < q) v -= Lj[q][t2] * Lj[q][t2];
         if (v > 1e-8 * dq && dq > 0.0) {
           Lj[q][q] = sqrt(v);
-          const int g = my[jq];
-          kJv[q] = c0 - kg_vg<GEN>(vf, dist2(x, y, sx[g], sy[g]), 0,
-                                      min_vg_val);
+          kJv[q] = c0 - kg_vg<GEN>(vf, dist2(x, y, stx[slt[q]], sty[slt[q]]),
+                                   0, min_vg_val);
           mm = q + 1;
         } else {
           ok = false;
