@@ -41,7 +41,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--batch', type=int, default=64,
                     help='time steps per bench step')
-    ap.add_argument('--plain-steps', type=int, default=6)
+    ap.add_argument('--plain-steps', type=int, default=16)
     ap.add_argument('--grid', type=int, default=1000)
     ap.add_argument('--cpu-cells', type=int, default=200)
     ap.add_argument('--cpu-steps', type=int, default=6)
@@ -95,30 +95,30 @@ def main():
     vals_d = torch.as_tensor(vals, device=dev)
     vg_d = vg_device(vgs, dev)
     n_act = (~np.isnan(vals)).sum(1)
-    est = torch.empty((B, C), dtype=torch.float64, device=dev)
-    var = torch.empty((B, C), dtype=torch.float64, device=dev)
+    # the timed steps run as one call, as a user interpolates a series:
+    # batch b+1's systems are factorised while batch b's cells contract
+    rows = max(K, W) * B
+    est = torch.empty((rows, C), dtype=torch.float64, device=dev)
+    var = torch.empty((rows, C), dtype=torch.float64, device=dev)
+    vrec = _lib.VG_DTYPE.itemsize
 
-    def batch(i):
-        s0 = i * B
-        vrec = _lib.VG_DTYPE.itemsize
-        return (vals_d[s0:s0 + B], vg_d[s0 * vrec:(s0 + B) * vrec])
+    def steps(s0, s1):
+        return (vals_d[s0:s1], vg_d[s0 * vrec:s1 * vrec])
 
     # ---- warmup + timed region (mixed) --------------------------------
-    infos = []
-    for i in range(W):
-        v, g = batch(i)
-        interp_steps(prob, v, None, 'OK', precision='mixed', est=est, var=var,
-                     vg_dev=g)
+    if W:
+        v, g = steps(0, W * B)
+        interp_steps(prob, v, None, 'OK', precision='mixed',
+                     est=est[:W * B], var=var[:W * B], vg_dev=g)
         torch.cuda.synchronize()
-        log(f'warmup {i + 1}/{W} done')
+        log(f'warmup: {W} x {B} steps done')
     prof_enable(0, True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(W, W + K):
-        v, g = batch(i)
-        _, _, info = interp_steps(prob, v, None, 'OK', precision='mixed',
-                                  est=est, var=var, vg_dev=g)
-        infos.append(info)
+    v, g = steps(W * B, (W + K) * B)
+    _, _, info = interp_steps(prob, v, None, 'OK', precision='mixed',
+                              est=est[:K * B], var=var[:K * B], vg_dev=g)
+    infos = [info]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / K
@@ -126,8 +126,9 @@ def main():
     log(f'mixed: {ms_per_step:.1f} ms per batch of {B} steps, '
         f'{value:.3e} cell-estimates/s')
     kern = {}
-    for nm in ('kgen', 'gemm_tri_f32', 'gemm_tri_f64', 'potrf', 'trtri',
-               'dual', 'assemble', 'compact', 'special', 'cast_tri'):
+    for nm in ('nearest', 'kgen', 'gemm_tri_f32', 'gemm_tri_f64',
+               'var_reduce', 'potrf', 'trtri', 'dual', 'assemble', 'special',
+               'cast_tri'):
         ms, cnt = prof_get(nm)
         kern[nm] = (ms, cnt)
     prof_enable(0, False)
@@ -182,12 +183,16 @@ def main():
     pi = pinfo.cpu().numpy()
     mi = minfo.cpu().numpy()
     kp = kap.cpu().numpy()
-    # errors are stated on well-posed steps (condition estimate <= 1e8),
-    # where the FP64 paths agree with the FP64 oracle to ~1e-9; beyond
-    # that FP64 itself (like the reference's pinv) is rounding-dominated
-    comparable = ((pi & (_lib.ST_ILLCOND | _lib.ST_FAILED)) == 0) & (kp <= 1e8)
+    # errors are stated on steps with condition estimate <= 1e6; above it
+    # two FP64 formulations (the plain path's explicit inverse, like the
+    # reference's pinv, and the batched path's dual form) differ from each
+    # other by more than 1e-6, so those steps are reported apart
+    solved = (pi & (_lib.ST_ILLCOND | _lib.ST_FAILED)) == 0
+    comparable = solved & (kp <= 1e6)
+    high = solved & (kp > 1e6)
     per_step = []
     ee_all, ev_all = [], []
+    eh_all, vh_all = [], []
     for s in range(P):
         a, b = em[s], e64[s]
         floor = 1e-3 * b.abs().mean()
@@ -202,10 +207,16 @@ def main():
         if comparable[s]:
             ee_all.append(re)
             ev_all.append(rv)
+        elif high[s]:
+            eh_all.append(re.max())
+            vh_all.append(rv.max())
     err = {'est_floor': '1e-3 x mean |estimate| of the step',
            'var_floor': '1e-3 x mean variance of the step',
            'steps_compared': int(comparable.sum()),
            'steps_excluded': int((~comparable).sum()),
+           'high_kappa': {'steps': int(high.sum()),
+                          'est_max': float(max(eh_all, default=0.0)),
+                          'var_max': float(max(vh_all, default=0.0))},
            'per_step': per_step}
     if ee_all:
         ee = np.concatenate(ee_all)

@@ -150,10 +150,12 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 #define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
 #define SPK_ST_ILLCOND  16   /* condition estimate > 1e12: results are set
                                 by rounding (as the reference's pinv is) */
-#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: condition estimate above the
-                                FP32 limit (50 by default, see
-                                spk_opts.fp64var_kappa), so the variance
-                                contraction ran in FP64                   */
+#define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: the variance contraction ran
+                                in FP64: condition estimate above the FP32
+                                limit (1e5, Gaussian models 1e3, or
+                                spk_opts.fp64var_kappa), or a block of
+                                cells whose nearest stations exceed the
+                                local fit's staging                       */
 
 typedef struct spk_problem {
   int64_t n_stns;               /* N                                      */
@@ -177,7 +179,8 @@ typedef struct spk_opts {
   double min_var_thr;           /* interp_steps_flags (steps.py:760-765)  */
   double fp64var_kappa;         /* SPK_P_MIXED: condition estimate above
                                    which a step's variance contraction runs
-                                   in FP64; 0 = the library's policy      */
+                                   in FP64, for every model; 0 = the
+                                   library's policy (1e5, Gaussian 1e3)   */
 } spk_opts;
 
 /* Interpolate n_steps steps.  vals: device [n_steps x N], NaN = station

@@ -521,7 +521,8 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
 // block's cells have more distinct nearest stations than slots.
 // Cells are stripe-local indices c; with perm the outputs and the cell
 // drifts are addressed at the original index perm[c].
-constexpr int kJ = 4, kSlots = 8;
+// kChunk: stations staged per pass of k_kgen's main loop
+constexpr int kJ = 4, kSlots = 16, kChunk = 128;
 template <bool GEN>
 __device__ inline double kg_vg(const VgFast &f, double h, int covar_flag,
                                double min_vg_val) {
@@ -537,6 +538,68 @@ __global__ void k_vg_kinds(const spk_vg *__restrict__ vgs, int64_t T,
     simple[t] = vg_simple(vgs[t]) ? 1 : 0;
 }
 
+// Nearest active station of every cell of a stripe (ties to the lower
+// index), for the local fit of k_kgen.  With promote set (mixed path) a
+// step is moved to the FP64 contraction (SPK_ST_FP64VAR) when any block of
+// 256 cells has more distinct nearest stations than k_kgen stages
+// (kSlots): cells whose nearest station is not staged get a poorer local
+// fit, a larger residual delta and so a larger FP32 rounding term.  The
+// block partition is the one k_kgen uses.
+__global__ __launch_bounds__(256) void k_nearest(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ idx, int64_t N, StepSc *__restrict__ sc,
+    const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
+    int32_t *__restrict__ nrst, int64_t ldn, int promote) {
+  const int sl = blockIdx.y;
+  StepSc *stp = sc + sl;
+  if (stp->status) return;
+  const int n = stp->n;
+  const int32_t *my = idx + (int64_t)sl * N;
+  __shared__ double stx[256], sty[256];
+  __shared__ int slot_j[kSlots];
+  __shared__ int ovf;
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x * (int64_t)256 + tid;
+  const bool valid = c < Cs;
+  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  if (tid < kSlots) slot_j[tid] = -1;
+  if (tid == 0) ovf = 0;
+  double bd = INFINITY;
+  int bj = -1;
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + tid < n) {
+      const int g = my[base + tid];
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    if (valid)
+      for (int j = 0; j < cnt; ++j) {
+        const double dx = x - stx[j], dy = y - sty[j];
+        const double d2 = dx * dx + dy * dy;
+        if (d2 < bd) {
+          bd = d2;
+          bj = base + j;
+        }
+      }
+  }
+  if (valid) nrst[(int64_t)sl * ldn + c] = bj;
+  if (!promote) return;
+  __syncthreads();
+  if (valid && bj >= 0) {
+    bool placed = false;
+    for (int k = 0; k < kSlots && !placed; ++k) {
+      const int cur = atomicCAS(&slot_j[k], -1, bj);
+      placed = cur == -1 || cur == bj;
+    }
+    if (!placed) ovf = 1;
+  }
+  __syncthreads();
+  if (tid == 0 && ovf) atomicOr(&stp->flags, SPK_ST_FP64VAR);
+}
+
 template <bool GEN>
 __global__ __launch_bounds__(256, 5) void k_kgen(
     const double *__restrict__ sx, const double *__restrict__ sy,
@@ -545,7 +608,8 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     const StepSc *__restrict__ sc, const spk_vg *__restrict__ vgs,
     const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
     int64_t C, const double *__restrict__ cell_drift,
-    const int32_t *__restrict__ perm,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ nrst,
+    int64_t ldn,
     const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
     double min_vg_val, int want_var, int all64, float *__restrict__ K32,
     double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
@@ -557,8 +621,8 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
   const int32_t *my = idx + (int64_t)sl * N;
   const double *alpha = V + (int64_t)sl * ldv;
   const double *w = alpha + ld;
-  __shared__ double stx[256], sty[256], sal[256], sw[256];
-  __shared__ double sG[kSlots][256];
+  __shared__ double stx[kChunk], sty[kChunk], sal[kChunk], sw[kChunk];
+  __shared__ double sG[kSlots][kChunk];
   __shared__ int slot_j[kSlots];
   __shared__ int n_slots;
   __shared__ VgFast vf;
@@ -587,28 +651,9 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
   }
   double qloc = 0.0, sbeta = 0.0;
   if (var_on) {
-    // pass 1: the cell's nearest active station (ties to the lower index)
-    double bd = INFINITY;
-    int bj = -1;
-    for (int base = 0; base < n; base += 256) {
-      __syncthreads();
-      if (base + tid < n) {
-        const int g = my[base + tid];
-        stx[tid] = sx[g];
-        sty[tid] = sy[g];
-      }
-      __syncthreads();
-      const int cnt = min(256, n - base);
-      if (valid)
-        for (int j = 0; j < cnt; ++j) {
-          const double dx = x - stx[j], dy = y - sty[j];
-          const double d2 = dx * dx + dy * dy;
-          if (d2 < bd) {
-            bd = d2;
-            bj = base + j;
-          }
-        }
-    }
+    // the cell's nearest active station, from k_nearest
+    const int bj = valid ? nrst[(int64_t)sl * ldn + c] : -1;
+    __syncthreads();  // slot_j and n_slots initialised
     // the block's table of staged stations: the cells' nearest stations,
     // first come first served
     if (valid && bj >= 0) {
@@ -735,20 +780,20 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
   // delta_i = k_i - sum_q (c0 - G_{i,J_q}) beta_q
   //         = c0 (1 - sum beta) - g_i + sum_q G_{i,J_q} beta_q
   const double dconst = c0 * (1.0 - sbeta);
-  for (int base = 0; base < n; base += 256) {
+  for (int base = 0; base < n; base += kChunk) {
     __syncthreads();
-    if (base + tid < n) {
+    if (tid < kChunk && base + tid < n) {
       const int g = my[base + tid];
       stx[tid] = sx[g];
       sty[tid] = sy[g];
       sal[tid] = alpha[base + tid];
       sw[tid] = w[base + tid];
     }
-    const int cnt = min(256, n - base);
+    const int cnt = min(kChunk, n - base);
     if (var_on) {
       const int ns = n_slots;
-      for (int e2 = tid; e2 < ns * 256; e2 += 256) {
-        const int k = e2 >> 8, u = e2 & 255;
+      for (int e2 = tid; e2 < ns * kChunk; e2 += 256) {
+        const int k = e2 / kChunk, u = e2 % kChunk;
         if (u < cnt) sG[k][u] = Gs[(int64_t)slot_j[k] * ld + base + u];
       }
     }
@@ -1191,7 +1236,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                           : (want_var ? strideM * 8 : 0) +
                                 (prec == SPK_P_MIXED && want_var ? strideM * 4
                                                                  : 0));
-  const int64_t budget = (int64_t)6 << 30;
+  const int64_t budget = (int64_t)20 << 30;
   int64_t S = std::max<int64_t>(1, std::min<int64_t>(budget / per_step, 64));
   S = std::min<int64_t>(S, T);
   const int nsets = T > S ? 2 : 1;
@@ -1225,7 +1270,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   }
   // cell stripes for the materialised right-hand sides / deflated covariances
   int64_t Cs = C;
-  const size_t rhs_budget = (size_t)4 << 30;
+  const size_t rhs_budget = (size_t)8 << 30;
   if (!batched) {
     Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
   } else if (want_var) {
@@ -1238,7 +1283,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int nrb = (int)ceil_div(N, 64);
   double *R = nullptr, *abuf = nullptr, *K64 = nullptr, *qpart = nullptr;
   float *K32 = nullptr;
-  int32_t *nnb = nullptr;
+  int32_t *nnb = nullptr, *nrst = nullptr;
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -1250,17 +1295,26 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
     SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
     SPK_TRY(ws_get(ctx, "qpart", S * nrb * ldk * 8, (void **)&qpart));
+    if (method == SPK_M_OK)
+      SPK_TRY(ws_get(ctx, "nrst", S * ldk * 4, (void **)&nrst));
   }
   // SPK_P_MIXED: condition estimate above which a step's variance runs in
-  // FP64.  The FP32 contraction loses about u * kappa^(1/2) to cancellation
-  // in L^-1 delta: measured on the hourly workload it stays within 1e-6
-  // (relative, floor 1e-3 x the step's mean variance) up to kappa ~ 90 and
-  // exceeds it from kappa ~ 130; the batched FP64 path runs every step in
-  // FP64.
+  // FP64.  The FP32 contraction's error is rounding in L^-1 delta amplified
+  // by cancellation; it grows with the condition estimate and with the
+  // size of delta against the variance.  Measured on 1 024 hourly steps
+  // (2 000 stations, 500 m cells; error relative to the variance with a
+  // floor of 1e-3 x the step's mean variance): spherical and exponential
+  // models stay below 2.1e-7 up to the largest condition estimate seen
+  // (7e4); Gaussian models stay below 5.5e-7 up to 2e3 and exceed 1e-6
+  // from 2.4e3.  A step whose cells have more distinct nearest stations
+  // per block than the local fit stages is promoted by k_nearest.  The
+  // batched FP64 path runs every step in FP64; a caller's limit applies
+  // to all models.
   const double kap = opts->fp64var_kappa;
   const double fp64var_thr =
-      prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 50.0) : INFINITY;
-  const double fp64var_thr_gau = fp64var_thr;
+      prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e5) : INFINITY;
+  const double fp64var_thr_gau =
+      prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e3) : INFINITY;
 
   // which steps have only simple variogram components (the cell kernels
   // are specialised for them)
@@ -1394,6 +1448,14 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                ? (perm ? prob->cell_drift
                                        : prob->cell_drift + cb)
                                : nullptr;
+        if (nrst) {
+          ProfScope ps(ctx, "nearest");
+          hipLaunchKernelGGL(k_nearest, dim3((unsigned)ceil_div(cn, 256), Sb),
+                             dim3(256), 0, sm, sx, sy, q.idx, N, q.sc,
+                             (perm ? pcx : cx) + cb, (perm ? pcy : cy) + cb,
+                             cn, nrst, ldk, prec == SPK_P_MIXED ? 1 : 0);
+          SPK_TRY(check_launch("k_nearest"));
+        }
         {
           ProfScope ps(ctx, "kgen");
           hipLaunchKernelGGL(simple ? k_kgen<false> : k_kgen<true>,
@@ -1401,7 +1463,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              dim3(256), 0, sm, sx, sy, q.idx, N, q.zact, ldz,
                              q.sc, gb, (perm ? pcx : cx) + cb,
                              (perm ? pcy : cy) + cb, cn, C, cd,
-                             perm ? perm + cb : (const int32_t *)nullptr, q.V,
+                             perm ? perm + cb : (const int32_t *)nullptr,
+                             (const int32_t *)nrst, ldk, q.V,
                              2 * ld, ld, method, m, mvg, want_var ? 1 : 0,
                              prec == SPK_P_MIXED ? 0 : 1, K32, K64, ldk, q.Gm,
                              abuf, perm ? eb : eb + cb, C);

@@ -126,7 +126,8 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     ``kappa``: optional float64 device tensor [T] that receives each
     system's condition estimate.  ``fp64var_kappa`` (mixed precision):
     condition estimate above which a step's variance contraction runs in
-    FP64 (0 = the library's policy).
+    FP64, for every variogram model (0 = the library's policy: 1e5, and
+    1e3 for steps with a Gaussian component).
     """
     dev = prob.device
     v = _t64(vals, dev)
