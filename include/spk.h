@@ -173,7 +173,12 @@ typedef struct spk_opts {
   int32_t method;               /* SPK_M_*                                */
   int32_t precision;            /* SPK_P_*                                */
   int32_t want_var;             /* OK variance                            */
-  int32_t pad_;
+  int32_t sk_mean;              /* SPK_M_SK: 0 = the reference's SK
+                                   (lambda^T z, nearest station where
+                                   sum(lambda) is not ~1,
+                                   interp/steps.py:415-429); 1 = simple
+                                   kriging with the step mean as known
+                                   mean: m + lambda^T (z - m)            */
   double idw_exp;
   double min_vg_val;            /* interp/data.py default 0.0             */
   double min_var_thr;           /* interp_steps_flags (steps.py:760-765)  */

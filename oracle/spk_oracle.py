@@ -370,6 +370,7 @@ def krig_system(method, vg_str, ref_ref_dists, dst_ref_dists, ref_drfts,
 
     OK : K = [[G, 1], [1^T, 0]], R = [g | 1]          (steps.py:212-216)
     SK : K = Cov, R = cov                            (steps.py:218-219)
+    SKM: the SK system (simple kriging with the step mean as known mean)
     EDK: K = [[G, 1, S], [1^T, 0, 0], [S^T, 0, 0]], R = [g | 1 | s]
                                                      (steps.py:221-234)
     G and g are semivariances (covar_flag 0), Cov and cov covariances
@@ -379,7 +380,7 @@ def krig_system(method, vg_str, ref_ref_dists, dst_ref_dists, ref_drfts,
     ncell = dst_ref_dists.shape[0]
     if method == 'OK':
         add, cflag = 1, 0
-    elif method == 'SK':
+    elif method in ('SK', 'SKM'):
         add, cflag = 0, 1
     elif method == 'EDK':
         add, cflag = 1 + ref_drfts.shape[1], 0
@@ -410,14 +411,20 @@ def krig_system(method, vg_str, ref_ref_dists, dst_ref_dists, ref_drfts,
     return K, R
 
 
-def krige_contract(Kinv, R, z, dst_ref_dists, want_var):
+def krige_contract(Kinv, R, z, dst_ref_dists, want_var, known_mean=None):
     """interp/steps.py:403-435 for one step: per cell lmds = Kinv @ rhs; if
     not isclose(sum(lmds[:n]), 1) the cell takes the nearest station's
     value (first index on ties) and variance 0; else estimate = lmds[:n].z
     and variance = sum(lmds * rhs) + lmds[n].  Vectorised over cells with
-    one GEMM; the per-cell arithmetic is the same."""
+    one GEMM; the per-cell arithmetic is the same.
+
+    known_mean m (simple kriging with a known mean, no counterpart in the
+    reference): estimate = m + lmds.(z - m) for every cell, no sum test."""
     n = z.size
     L = R @ Kinv.T                                   # [C x (n+add)]
+    if known_mean is not None:
+        est = known_mean + L[:, :n] @ (z - known_mean)
+        return est, None, np.ones(est.shape, dtype=bool)
     lsum = L[:, :n].sum(axis=1)
     ok = np.isclose(lsum, 1.0)
     nnb = np.argmin(dst_ref_dists, axis=1)
@@ -485,7 +492,7 @@ def get_interp(ref_data, interp_type, ref_drfts, dst_drfts, idw_exp, models,
             vals = num / wsum[None, :]
         for j in range(n_time):
             dst_data[j, :] = vals[j] if interp_steps_flags[j] else ref_means[j]
-    elif interp_type in ('OK', 'SK', 'EDK'):
+    elif interp_type in ('OK', 'SK', 'SKM', 'EDK'):
         order = np.argsort(np.asarray(models, dtype=object).astype(str),
                            kind='stable')
         old_model = ''
@@ -519,9 +526,10 @@ def get_interp(ref_data, interp_type, ref_drfts, dst_drfts, idw_exp, models,
                 if est_vars is not None:
                     est_vars[j, :] = 0.0
             else:
-                est, var, _ = krige_contract(Kinv, R, ref_data[j],
-                                             dst_ref_dists,
-                                             est_vars is not None)
+                est, var, _ = krige_contract(
+                    Kinv, R, ref_data[j], dst_ref_dists,
+                    est_vars is not None,
+                    known_mean=ref_means[j] if interp_type == 'SKM' else None)
                 dst_data[j, :] = est
                 if est_vars is not None:
                     est_vars[j, :] = var

@@ -47,7 +47,7 @@ class SpkProblem(ctypes.Structure):
 
 class SpkOpts(ctypes.Structure):
     _fields_ = [('method', c_int32), ('precision', c_int32),
-                ('want_var', c_int32), ('pad_', c_int32),
+                ('want_var', c_int32), ('sk_mean', c_int32),
                 ('idw_exp', c_double), ('min_vg_val', c_double),
                 ('min_var_thr', c_double), ('fp64var_kappa', c_double)]
 

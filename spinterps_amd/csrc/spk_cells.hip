@@ -133,7 +133,7 @@ template <typename T, bool PLAIN, int BK>
 __global__ __launch_bounds__(256) void k_gemm(
     const T *__restrict__ A, int64_t lda, int64_t strideA,
     const T *__restrict__ B, int64_t ldb, int64_t strideB,
-    const StepSc *__restrict__ sc, int p, int method, int want_var,
+    const StepSc *__restrict__ sc, int p, int method, int want_var, int skm,
     const double *__restrict__ zact, int64_t ldz,
     const int32_t *__restrict__ nnb, int64_t ldn,
     const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
@@ -332,7 +332,10 @@ __global__ __launch_bounds__(256) void k_gemm(
     const int64_t col = c0 + wave * 32 + tj * 16 + li;
     if (col >= Cs) continue;
     if (PLAIN) {
-      if (isclose1(acc_l[tj])) {
+      if (skm) {
+        // SK with the step mean m as known mean: m + lambda^T (z - m)
+        eo[col] = acc_e[tj] + st.mean * (1.0 - acc_l[tj]);
+      } else if (isclose1(acc_l[tj])) {
         eo[col] = acc_e[tj];
         if (want_var) vo[col] = acc_v[tj] + acc_m[tj];
       } else {
@@ -611,7 +614,8 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     const int32_t *__restrict__ perm, const int32_t *__restrict__ nrst,
     int64_t ldn,
     const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
-    double min_vg_val, int want_var, int all64, float *__restrict__ K32,
+    double min_vg_val, int want_var, int all64, int skm,
+    float *__restrict__ K32,
     double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
     double *__restrict__ abuf, double *__restrict__ est, int64_t ldo) {
   const int sl = blockIdx.y;
@@ -844,7 +848,11 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     for (int k = 0; k < m; ++k)
       v += stp->h[1 + k] * cell_drift[(int64_t)k * C + oc];
     eo[oc] = v;
-  } else {  // SK
+  } else if (skm) {
+    // SK with the step mean m as known mean: m + k^T C^-1 (z - m 1)
+    // = k.alpha + m (1 - k.w)
+    eo[oc] = e + stp->mean * (1.0 - a);
+  } else {  // the reference's SK
     eo[oc] = isclose1(a) ? e : z[bi];
   }
 }
@@ -1155,6 +1163,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   if (prec < SPK_P_FP64 || prec > SPK_P_FP64B)
     return set_error(SPK_EARG, "unknown precision %d", prec);
   const bool want_var = opts->want_var && method == SPK_M_OK && var;
+  const int skm = method == SPK_M_SK && opts->sk_mean ? 1 : 0;
   SPK_HIP(hipSetDevice(ctx->device));
   hipStream_t sm = ctx->stream;
 
@@ -1430,7 +1439,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                dim3((unsigned)ceil_div(cn, 128), 1), dim3(256),
                                0, sm, q.M + s * strideM, ld, (int64_t)0, R, ldk,
                                (int64_t)0, q.sc + s, p, method,
-                               want_var ? 1 : 0, q.zact + s * ldz, ldz, nnb,
+                               want_var ? 1 : 0, skm, q.zact + s * ldz, ldz,
+                               nnb,
                                ldk, (const double *)nullptr, (int64_t)0, cn,
                                eb + s * C + cb,
                                vbo ? vbo + s * C + cb : nullptr, C, 0,
@@ -1466,7 +1476,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              perm ? perm + cb : (const int32_t *)nullptr,
                              (const int32_t *)nrst, ldk, q.V,
                              2 * ld, ld, method, m, mvg, want_var ? 1 : 0,
-                             prec == SPK_P_MIXED ? 0 : 1, K32, K64, ldk, q.Gm,
+                             prec == SPK_P_MIXED ? 0 : 1, skm, K32, K64, ldk,
+                             q.Gm,
                              abuf, perm ? eb : eb + cb, C);
         }
         SPK_TRY(check_launch("k_kgen"));
@@ -1476,7 +1487,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           ProfScope ps(ctx, "gemm_tri_f32");
           hipLaunchKernelGGL((k_gemm<float, false, 32>), grid, dim3(256), 0, sm,
                              q.L32, ld, strideM, (const float *)K32, ldk,
-                             ld * ldk, q.sc, p, method, 1, q.zact, ldz,
+                             ld * ldk, q.sc, p, method, 1, 0, q.zact, ldz,
                              (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
                              cn, eb + cb, vbo + cb, C, 1, qpart, nrb);
         }

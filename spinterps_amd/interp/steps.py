@@ -118,6 +118,10 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
                  vg_dev=None, kappa=None, fp64var_kappa=0.0):
     """Interpolate T steps.  vals [T x N] (NaN = missing station).
 
+    method: 'OK', 'SK' (the reference's SK: lambda^T z, nearest station
+    where sum(lambda) is not ~1), 'SKM' (simple kriging with the step mean
+    as known mean, m + lambda^T (z - m)), 'EDK', 'IDW' or 'NNB'.
+
     Returns (est [T x C] float64, var [T x C] float64 or None,
     info [T] int32) as device tensors.  info holds SPK_ST_* flags per step:
     1 no station (outputs left NaN), 2 mean fill, 4 failed system
@@ -133,6 +137,11 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     v = _t64(vals, dev)
     assert v.ndim == 2 and v.shape[1] == prob.n_stns
     T = v.shape[0]
+    # 'SKM': simple kriging with the step mean as known mean,
+    # m + lambda^T (z - m), reported apart from the reference's SK
+    sk_mean = method == 'SKM'
+    if sk_mean:
+        method = 'SK'
     m = METHODS[method]
     krig = method in ('OK', 'SK', 'EDK')
     if krig:
@@ -160,6 +169,7 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     opts.min_vg_val = float(min_vg_val)
     opts.min_var_thr = float(min_var_thr)
     opts.fp64var_kappa = float(fp64var_kappa)
+    opts.sk_mean = 1 if sk_mean else 0
     ctx = _ctx_for(dev)
     pst = prob.c_struct()
     check(lib().spk_interp_steps_dev(
@@ -215,7 +225,7 @@ def interpolate(prob: DeviceProblem, vals, vgs, methods, *,
     prblm = set()
     v = _t64(vals, prob.device)
     vg_dev = None
-    if any(m[0] in ('OK', 'SK', 'EDK') for m in methods):
+    if any(m[0] in ('OK', 'SK', 'SKM', 'EDK') for m in methods):
         if any(str(s) == 'nan' for s in vgs):
             raise AssertionError('NaN VGs not allowed! Use Nugget or any '
                                  'other appropriate one!')

@@ -175,9 +175,11 @@ def test_ok_small_all_precisions(gpu, precision):
     assert err <= (1e-9 if precision != 'mixed' else 1e-4)
 
 
-@pytest.mark.parametrize('method', ['SK', 'EDK'])
+@pytest.mark.parametrize('method', ['SK', 'SKM', 'EDK'])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_sk_edk_daily_subset(gpu, method, precision):
+    """SK (the reference's), SKM (simple kriging with the step mean as
+    known mean) and EDK against the oracle, 1e-9 relative."""
     from spinterps_amd.interp.steps import DeviceProblem, interp_steps
     w = configs.daily(n_side=40, cell=12500.0, n_stns=300)
     vals, vgs = configs.daily_steps(w, 0, 4)
@@ -312,3 +314,75 @@ def test_multi_batch_overlap_hourly_small(gpu):
                                    rtol=1e-9, atol=1e-12 * np.abs(ref['OK']).max())
         np.testing.assert_allclose(out['fp64b'][1][t], ref['EST_VARS_OK'][0],
                                    rtol=1e-9)
+
+
+_SOLVED_MASK = 1 | 2 | 4 | 8 | 16   # no stations, mean fill, failed, one station, ill-conditioned
+
+
+def test_mixed_precision_policy_500m_grid(gpu):
+    """The mixed path's precision policy on the hourly workload's geometry
+    (2 000 stations, 500 m cells): the variance contraction runs in FP32
+    for spherical and exponential steps with condition estimate up to 1e5
+    and Gaussian steps up to 1e3, in FP64 above.  Against the plain FP64
+    path every step with condition estimate <= 1e6 stays within 1e-6
+    (relative, floor 1e-3 x the step's mean)."""
+    import torch
+    from spinterps_amd import _lib
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.hourly(n_side=64, cell=500.0)
+    T = 96
+    vals, vgs = configs.hourly_steps(w, 0, T)
+    prob = DeviceProblem.from_workload(w)
+    kap = torch.zeros(T, dtype=torch.float64, device='cuda')
+    e, v, info = interp_steps(prob, vals, vgs, 'OK', precision='mixed',
+                              kappa=kap)
+    e64, v64, i64 = interp_steps(prob, vals, vgs, 'OK', precision='fp64')
+    e, v, inf = e.cpu().numpy(), v.cpu().numpy(), info.cpu().numpy()
+    e64, v64, i64 = e64.cpu().numpy(), v64.cpu().numpy(), i64.cpu().numpy()
+    k = kap.cpu().numpy()
+    gau = np.array(['Gau' in g for g in vgs])
+    promoted = (inf & _lib.ST_FP64VAR) != 0
+    solved = (i64 & _SOLVED_MASK) == 0
+    assert solved.sum() > 80
+    assert promoted[solved & gau & (k > 1e3)].all()
+    assert promoted[solved & ~gau & (k > 1e5)].all()
+    fp32 = solved & ~promoted
+    # the FP32 limit above 1e3 is exercised
+    assert (fp32 & ~gau & (k > 1e3)).sum() >= 5, k[fp32]
+    cmp = solved & (k <= 1e6)
+    floor_e = 1e-3 * np.abs(e64).mean(axis=1, keepdims=True)
+    floor_v = 1e-3 * np.abs(v64).mean(axis=1, keepdims=True)
+    with np.errstate(invalid='ignore'):   # steps without variance are NaN
+        re = (np.abs(e - e64) / np.maximum(np.abs(e64), floor_e)).max(axis=1)
+        rv = (np.abs(v - v64) / np.maximum(np.abs(v64), floor_v)).max(axis=1)
+    print(f'fp32 steps {fp32.sum()}, promoted {promoted[solved].sum()}; '
+          f'est max {re[cmp].max():.2e}, var max fp32 {rv[cmp & fp32].max():.2e}'
+          f' fp64 {rv[cmp & ~fp32].max(initial=0):.2e}')
+    worst = np.where(cmp)[0][np.argmax(rv[cmp])]
+    assert re[cmp].max() < 1e-6
+    assert rv[cmp].max() < 1e-6, (worst, k[worst], vgs[worst])
+
+
+def test_mixed_slot_overflow_promotes(gpu):
+    """On a coarse grid a 256-cell block has far more than 16 distinct
+    nearest stations, so not every cell gets a staged local fit: the
+    step's variance then runs in FP64 (SPK_ST_FP64VAR) at any condition
+    estimate and matches the plain FP64 path to 1e-9."""
+    import torch
+    from spinterps_amd import _lib
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.hourly(n_side=48, cell=5e5 / 48)   # 2 304 cells of 10.4 km
+    T = 6
+    vals, _ = configs.hourly_steps(w, 0, T, min_active=1500)
+    vgs = ['0.2 Nug(0.0) + 1.0 Exp(20000.0)'] * T
+    prob = DeviceProblem.from_workload(w)
+    kap = torch.zeros(T, dtype=torch.float64, device='cuda')
+    _, v, info = interp_steps(prob, vals, vgs, 'OK', precision='mixed',
+                              kappa=kap)
+    _, v64, _ = interp_steps(prob, vals, vgs, 'OK', precision='fp64')
+    k, inf = kap.cpu().numpy(), info.cpu().numpy()
+    assert (k < 1e5).all(), k        # FP32 by condition estimate alone
+    assert ((inf & _lib.ST_FP64VAR) != 0).all(), inf
+    v, v64 = v.cpu().numpy(), v64.cpu().numpy()
+    floor = 1e-3 * np.abs(v64).mean(axis=1, keepdims=True)
+    assert np.max(np.abs(v - v64) / np.maximum(np.abs(v64), floor)) < 1e-9

@@ -180,6 +180,36 @@ def test_edk_reproduces_drift_and_sk_matches_solve():
     np.testing.assert_allclose(Rs @ np.linalg.pinv(Ks).T, lam, atol=1e-10)
 
 
+def test_sk_known_mean_invariants():
+    """Simple kriging with the step mean as known mean (labelled SKM): a
+    cell beyond the range of every station has zero covariances and takes
+    the mean; a cell on a station (no nugget in the SK covariance) takes
+    that station's value; shifting all values by t shifts every estimate
+    by t (the mean shifts with them)."""
+    sx, sy, cx, cy, rr, dr, z, vg = _system_case(n=7, vg='1.0 Sph(3000.0)')
+    m = z.mean()
+    far = np.array([[1e6, 1e6]])
+    dr_far = O.dists_2d(far[:, 0], far[:, 1], sx, sy)
+    K, R = O.krig_system('SKM', vg, rr, dr_far, None, None, 0.0)
+    est, var, ok = O.krige_contract(np.linalg.pinv(K), R, z, dr_far, False,
+                                    known_mean=m)
+    assert var is None and ok.all()
+    np.testing.assert_allclose(est, m, rtol=1e-15)
+    dr_st = O.dists_2d(sx, sy, sx, sy)
+    K, R = O.krig_system('SKM', vg, rr, dr_st, None, None, 0.0)
+    est, _, _ = O.krige_contract(np.linalg.pinv(K), R, z, dr_st, False,
+                                 known_mean=m)
+    np.testing.assert_allclose(est, z, rtol=1e-10)
+    K, R = O.krig_system('SKM', vg, rr, dr, None, None, 0.0)
+    Ki = np.linalg.pinv(K)
+    e1, _, _ = O.krige_contract(Ki, R, z, dr, False, known_mean=m)
+    e2, _, _ = O.krige_contract(Ki, R, z + 7.5, dr, False, known_mean=m + 7.5)
+    np.testing.assert_allclose(e2 - e1, 7.5, rtol=1e-12)
+    # the reference's SK on the same system differs (no mean term)
+    e_sk, _, _ = O.krige_contract(Ki, R, z, dr, False)
+    assert np.abs(e_sk - e1).max() > 1e-3
+
+
 def test_get_interp_idw_and_special_steps():
     # two stations at distances 1 and 2: normalised 0.5, 1 -> w = 4, 1
     dr = np.array([[1.0, 2.0]])
