@@ -24,11 +24,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 
-# HBM bytes per k_tri64 dispatch measured by rocprofv3 --pmc on this
-# workload (2 x 8.78 GB fetched + 0.04 GB written, calibrated with
-# k_clip_cast); the algorithmic operands are about 3.1 GB per dispatch
-TRI64_TRAFFIC = 2 * 8776922 * 1024 + 42411 * 1024
-TRI32_TRAFFIC = 0
+# HBM bytes per dispatch of the two variance contractions, measured by
+# rocprofv3 --pmc on this workload (batches of 64 steps, stripes of 10 880
+# cells; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with k_clip_cast)
+TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
+TRI32_TRAFFIC = 2 * 5439736 * 1024 + 68983 * 1024
 
 
 def log(*a):
