@@ -69,9 +69,9 @@ def main():
                          'kernel for HBM counter scaling')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path '
-              '(time steps shard across GPUs with no exchange; see '
-              'DESIGN.md).  Run with --gpus 1.', file=sys.stderr)
+        print('bench.py measures one GPU: the library has no multi-GPU '
+              'path (time steps would shard across GPUs with no exchange '
+              'during compute).  Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

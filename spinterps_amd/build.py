@@ -21,7 +21,8 @@ SOURCES = ['spk_core.hip', 'spk_factor.hip', 'spk_cells.hip', 'spk_neb.hip',
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
-    # packed FP32 instructions stay off (numerics)
+    # packed FP32 instructions stay off: some of their forms have given
+    # wrong results without an error on this hardware
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
     '-Wno-unused-result', '-Wno-unused-command-line-argument',
 ]
