@@ -128,8 +128,8 @@ def main():
         f'{value:.3e} cell-estimates/s')
     kern = {}
     for nm in ('nearest', 'kgen', 'gemm_tri_f32', 'gemm_tri_f64',
-               'var_reduce', 'potrf', 'trtri', 'dual', 'assemble', 'special',
-               'cast_tri'):
+               'slice_cols', 'gemm_tri_i8', 'var_reduce', 'potrf', 'trtri',
+               'dual', 'slice_rows', 'assemble', 'special', 'cast_tri'):
         ms, cnt = prof_get(nm)
         kern[nm] = (ms, cnt)
     prof_enable(0, False)

@@ -151,11 +151,16 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 #define SPK_ST_ILLCOND  16   /* condition estimate > 1e12: results are set
                                 by rounding (as the reference's pinv is) */
 #define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: the variance contraction ran
-                                in FP64: condition estimate above the FP32
-                                limit (1e5, Gaussian models 1e3, or
-                                spk_opts.fp64var_kappa), or a block of
-                                cells whose nearest stations exceed the
-                                local fit's staging                       */
+                                in FP64: condition estimate above a
+                                caller's int8 limit (spk_opts.int8var_kappa),
+                                or a
+                                block of cells whose nearest stations
+                                exceed the local fit's staging            */
+#define SPK_ST_INT8VAR  64   /* SPK_P_MIXED: condition estimate above the
+                                FP32 limit (1e5, Gaussian models 1e3, or
+                                spk_opts.fp64var_kappa): the variance
+                                contraction used exact int8 plane
+                                products (unless FP64VAR is also set)    */
 
 typedef struct spk_problem {
   int64_t n_stns;               /* N                                      */
@@ -183,9 +188,13 @@ typedef struct spk_opts {
   double min_vg_val;            /* interp/data.py default 0.0             */
   double min_var_thr;           /* interp_steps_flags (steps.py:760-765)  */
   double fp64var_kappa;         /* SPK_P_MIXED: condition estimate above
-                                   which a step's variance contraction runs
-                                   in FP64, for every model; 0 = the
+                                   which a step's variance contraction
+                                   leaves FP32, for every model; 0 = the
                                    library's policy (1e5, Gaussian 1e3)   */
+  double int8var_kappa;         /* SPK_P_MIXED: condition estimate up to
+                                   which such a step uses exact int8 plane
+                                   products, above it FP64; 0 = no limit,
+                                   negative = never                       */
 } spk_opts;
 
 /* Interpolate n_steps steps.  vals: device [n_steps x N], NaN = station

@@ -27,7 +27,7 @@ P_FP64, P_MIXED, P_FP64B = range(3)
 PRECISIONS = {'fp64': P_FP64, 'mixed': P_MIXED, 'fp64b': P_FP64B}
 
 ST_NOSTNS, ST_MEANFILL, ST_FAILED, ST_ONESTN = 1, 2, 4, 8
-ST_ILLCOND, ST_FP64VAR = 16, 32
+ST_ILLCOND, ST_FP64VAR, ST_INT8VAR = 16, 32, 64
 
 
 class SpkVg(ctypes.Structure):
@@ -49,7 +49,8 @@ class SpkOpts(ctypes.Structure):
     _fields_ = [('method', c_int32), ('precision', c_int32),
                 ('want_var', c_int32), ('sk_mean', c_int32),
                 ('idw_exp', c_double), ('min_vg_val', c_double),
-                ('min_var_thr', c_double), ('fp64var_kappa', c_double)]
+                ('min_var_thr', c_double), ('fp64var_kappa', c_double),
+                ('int8var_kappa', c_double)]
 
 
 VG_DTYPE = np.dtype([('n', '<i4'), ('type', '<i4', (VG_MAX,)),

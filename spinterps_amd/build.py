@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 SRC = HERE / 'csrc'
 OUT = HERE / 'libspk.so'
 SOURCES = ['spk_core.hip', 'spk_factor.hip', 'spk_cells.hip', 'spk_neb.hip',
-           'spk_vgfit.hip', 'spk_mlp.hip', 'spk_class.hip']
+           'spk_vgfit.hip', 'spk_mlp.hip', 'spk_class.hip', 'spk_int8.hip']
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',

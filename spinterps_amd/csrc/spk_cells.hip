@@ -163,8 +163,9 @@ __global__ __launch_bounds__(256) void k_gemm(
   const int sl = PLAIN ? blockIdx.y : blockIdx.z;
   const StepSc st = sc[sl];
   if (st.status) return;
-  // sel 1: only steps without SPK_ST_FP64VAR, sel 2: only steps with it
-  if (sel == 1 && (st.flags & SPK_ST_FP64VAR)) return;
+  // sel 1: only the FP32 steps (neither SPK_ST_FP64VAR nor SPK_ST_INT8VAR),
+  // sel 2: only steps with SPK_ST_FP64VAR
+  if (sel == 1 && (st.flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR))) return;
   if (sel == 2 && !(st.flags & SPK_ST_FP64VAR)) return;
   const int n = st.n;
   const int P = PLAIN ? n + p : n;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
   const int sl = blockIdx.z;
   const StepSc st = sc[sl];
   if (st.status) return;
-  if (sel == 1 && (st.flags & SPK_ST_FP64VAR)) return;
+  if (sel == 1 && (st.flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR))) return;
   if (sel == 2 && !(st.flags & SPK_ST_FP64VAR)) return;
   const int n = st.n;
   const int r0 = rti * BM;
@@ -617,7 +618,8 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     double min_vg_val, int want_var, int all64, int skm,
     float *__restrict__ K32,
     double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
-    double *__restrict__ abuf, double *__restrict__ est, int64_t ldo) {
+    double *__restrict__ abuf, double *__restrict__ est, int64_t ldo,
+    double *__restrict__ dmax) {
   const int sl = blockIdx.y;
   const StepSc *stp = sc + sl;
   if (stp->status) return;
@@ -642,7 +644,9 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
   const int cflag = method == SPK_M_SK ? 1 : 0;
   const double c0 = stp->c0;
   const bool var_on = want_var && method == SPK_M_OK;
-  const bool use64 = all64 || (stp->flags & SPK_ST_FP64VAR);
+  // FP64 delta for the FP64 and the int8 contractions
+  const bool use64 =
+      all64 || (stp->flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR));
   float *K32s = (var_on && !use64) ? K32 + (int64_t)sl * ld * ldk : nullptr;
   double *K64s = (var_on && use64) ? K64 + (int64_t)sl * ld * ldk : nullptr;
   const double *Gs = var_on ? Gfull + (int64_t)sl * ld * ld : nullptr;
@@ -778,6 +782,7 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
       qloc = 2.0 * bk - tt;
     }
   }
+  double dm = 0.0;  // max |delta| of the cell (int8 plane scaling)
   double best = INFINITY;
   int bi = 0;
   double e = 0.0, a = 0.0;
@@ -814,10 +819,12 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
 #pragma unroll
           for (int q = 0; q < kJ; ++q)
             if (slt[q] >= 0) delta += sG[slt[q]][j] * beta[q];
-          if (K32s)
+          if (K32s) {
             K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
-          else
+          } else {
             K64s[(int64_t)(base + j) * ldk + c] = delta;
+            dm = fmax(dm, fabs(delta));
+          }
         }
       } else {
         for (int j = 0; j < cnt; ++j) {
@@ -843,6 +850,7 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     if (var_on)
       abuf[(int64_t)sl * ldk + c] =
           c0 - qloc + (a - 1.0) * (a - 2.0) / stp->b;
+    if (var_on && dmax) dmax[(int64_t)sl * ldk + c] = dm;
   } else if (method == SPK_M_EDK) {
     double v = e + stp->h[0];
     for (int k = 0; k < m; ++k)
@@ -1253,8 +1261,13 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     int32_t *idx;
     double *zact, *A, *Li, *M, *V, *T0, *Gm;
     float *L32;
+    int8_t *L8;  // int8 planes of L^-1 rows (mixed path)
+    int *ea;     // their row exponents
     StepSc *sc;
   } set[2] = {};
+  // int8 plane layout: rows padded to R (a multiple of the 128-row tile),
+  // k to KS (a multiple of the 64-deep MFMA)
+  const int64_t R8 = round_up(N, 128), KS8 = round_up(N, 64);
   for (int k = 0; k < nsets; ++k) {
     char nm[32];
     Set &q = set[k];
@@ -1273,17 +1286,22 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     } else {
       SPK_TRY(get("V", S * 2 * ld * 8, (void **)&q.V));
       if (want_var) SPK_TRY(get("Gm", S * strideM * 8, (void **)&q.Gm));
-      if (prec == SPK_P_MIXED && want_var)
+      if (prec == SPK_P_MIXED && want_var) {
         SPK_TRY(get("L32", S * strideM * 4, (void **)&q.L32));
+        SPK_TRY(get("L8", S * kInt8Planes * R8 * KS8, (void **)&q.L8));
+        SPK_TRY(get("ea", S * R8 * sizeof(int), (void **)&q.ea));
+      }
     }
   }
   // cell stripes for the materialised right-hand sides / deflated covariances
   int64_t Cs = C;
-  const size_t rhs_budget = (size_t)8 << 30;
+  const size_t rhs_budget = (size_t)12 << 30;
   if (!batched) {
     Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
   } else if (want_var) {
-    const int64_t eb = prec == SPK_P_MIXED ? 12 : 8;
+    // bytes per (step, station, cell): FP64 delta, plus FP32 delta and
+    // int8 planes in the mixed path
+    const int64_t eb = prec == SPK_P_MIXED ? 12 + kInt8Planes : 8;
     Cs = std::min<int64_t>(C, (int64_t)(2 * rhs_budget / (S * ld * eb)));
   }
   Cs = std::max<int64_t>(128, Cs / 128 * 128);
@@ -1293,6 +1311,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   double *R = nullptr, *abuf = nullptr, *K64 = nullptr, *qpart = nullptr;
   float *K32 = nullptr;
   int32_t *nnb = nullptr, *nrst = nullptr;
+  int8_t *B8 = nullptr;
+  int *eb8 = nullptr;
+  double *dmax = nullptr;
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -1300,15 +1321,19 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     // FP64 buffer for the batched FP64 path and for the steps the mixed
     // path promotes to FP64 (SPK_ST_FP64VAR)
     SPK_TRY(ws_get(ctx, "K64", S * ld * ldk * 8, (void **)&K64));
-    if (prec == SPK_P_MIXED)
+    if (prec == SPK_P_MIXED) {
       SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
+      SPK_TRY(ws_get(ctx, "B8", S * kInt8Planes * ldk * KS8, (void **)&B8));
+      SPK_TRY(ws_get(ctx, "eb8", S * ldk * sizeof(int), (void **)&eb8));
+      SPK_TRY(ws_get(ctx, "dmax", S * ldk * 8, (void **)&dmax));
+    }
     SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
     SPK_TRY(ws_get(ctx, "qpart", S * nrb * ldk * 8, (void **)&qpart));
     if (method == SPK_M_OK)
       SPK_TRY(ws_get(ctx, "nrst", S * ldk * 4, (void **)&nrst));
   }
-  // SPK_P_MIXED: condition estimate above which a step's variance runs in
-  // FP64.  The FP32 contraction's error is rounding in L^-1 delta amplified
+  // SPK_P_MIXED: condition estimate above which a step's variance leaves
+  // FP32.  The FP32 contraction's error is rounding in L^-1 delta amplified
   // by cancellation; it grows with the condition estimate and with the
   // size of delta against the variance.  Measured on 1 024 hourly steps
   // (2 000 stations, 500 m cells; error relative to the variance with a
@@ -1316,14 +1341,29 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // models stay below 2.1e-7 up to the largest condition estimate seen
   // (7e4); Gaussian models stay below 5.5e-7 up to 2e3 and exceed 1e-6
   // from 2.4e3.  A step whose cells have more distinct nearest stations
-  // per block than the local fit stages is promoted by k_nearest.  The
-  // batched FP64 path runs every step in FP64; a caller's limit applies
-  // to all models.
+  // per block than the local fit stages is promoted to FP64 by k_nearest.
+  // The batched FP64 path runs every step in FP64; a caller's limit
+  // applies to all models.
   const double kap = opts->fp64var_kappa;
   const double fp64var_thr =
       prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e5) : INFINITY;
   const double fp64var_thr_gau =
       prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e3) : INFINITY;
+  // Above the FP32 limit and up to this one the variance uses exact int8
+  // plane products (five 7-bit planes per operand, plane products up to
+  // the fourth diagonal).  Measured against the batched FP64 contraction on
+  // 768 hourly steps (500 m cells; relative, floor 1e-3 x the step's mean
+  // variance): 3e-9 below condition estimate 1e3, 1.3e-8 to 1e5, 7.8e-8 to
+  // 1e6, 3e-7 to 1e7, 6e-6 to 1e8 and 4e-5 to 1e12, where two FP64
+  // formulations already differ by far more; above 1e12 the step is
+  // flagged ill-conditioned and every formulation's variance is set by
+  // rounding.  By default every step above the FP32 limit uses the planes;
+  // FP64 remains for steps whose local fit is short of stations
+  // (k_nearest) and for a caller's limit.
+  const double k8 = opts->int8var_kappa;
+  const double int8var_thr =
+      prec != SPK_P_MIXED ? -INFINITY
+                          : (k8 > 0.0 ? k8 : (k8 < 0.0 ? -INFINITY : INFINITY));
 
   // which steps have only simple variogram components (the cell kernels
   // are specialised for them)
@@ -1402,7 +1442,13 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
                            strideM, q.sc, q.zact, ldz, q.idx, N,
                            prob->stn_drift, m, method, q.V, 2 * ld, q.T0, gb,
-                           mvg, fp64var_thr, fp64var_thr_gau);
+                           mvg, fp64var_thr, fp64var_thr_gau, int8var_thr);
+      }
+      if (q.L8) {
+        ProfScope ps(ctx, "slice_rows", sf);
+        hipLaunchKernelGGL(k_slice_rows, dim3((unsigned)(R8 / 64), Sb),
+                           dim3(256), 0, sf, q.Li, ld, strideM, q.sc, q.L8,
+                           KS8, R8, q.ea);
       }
       if (q.L32) {
         ProfScope ps(ctx, "cast_tri", sf);
@@ -1478,10 +1524,30 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              2 * ld, ld, method, m, mvg, want_var ? 1 : 0,
                              prec == SPK_P_MIXED ? 0 : 1, skm, K32, K64, ldk,
                              q.Gm,
-                             abuf, perm ? eb : eb + cb, C);
+                             abuf, perm ? eb : eb + cb, C, dmax);
         }
         SPK_TRY(check_launch("k_kgen"));
         if (!want_var) continue;
+        if (B8) {
+          {
+            ProfScope ps(ctx, "slice_cols");
+            hipLaunchKernelGGL(k_slice_cols,
+                               dim3((unsigned)(ldk / 64), (unsigned)(KS8 / 64),
+                                    Sb),
+                               dim3(256), 0, sm, (const double *)K64, ld, ldk,
+                               cn, q.sc, (const double *)dmax, B8, KS8, eb8);
+          }
+          {
+            ProfScope ps(ctx, "gemm_tri_i8");
+            hipLaunchKernelGGL(k_tri8,
+                               dim3((unsigned)ceil_div(N, 128),
+                                    (unsigned)(ldk / 64), Sb),
+                               dim3(256), 0, sm, (const int8_t *)q.L8, R8,
+                               (const int *)q.ea, (const int8_t *)B8, ldk,
+                               (const int *)eb8, KS8, q.sc, cn, qpart, nrb);
+          }
+          SPK_TRY(check_launch("k_tri8"));
+        }
         const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
         if (prec == SPK_P_MIXED) {
           ProfScope ps(ctx, "gemm_tri_f32");

@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_dual(
     const double *__restrict__ stn_drift, int m, int method,
     double *__restrict__ V, int64_t ldv, double *__restrict__ T0,
     const spk_vg *__restrict__ vgs, double min_vg_val, double fp64var_thr,
-    double fp64var_thr_gau) {
+    double fp64var_thr_gau, double int8var_thr) {
   const int64_t ldt = ld;
   const int s = blockIdx.x;
   if (sc[s].status) return;
@@ -590,8 +590,10 @@ __global__ __launch_bounds__(256) void k_dual(
       if (!(kap <= 1e12)) fl |= SPK_ST_ILLCOND;
       bool gau = false;
       for (int k = 0; k < vgs[s].n; ++k) gau |= vgs[s].type[k] == SPK_VG_GAU;
+      // above the FP32 limit: exact int8 plane products up to the int8
+      // limit, FP64 beyond it
       if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr)))
-        fl |= SPK_ST_FP64VAR;
+        fl |= kap <= int8var_thr ? SPK_ST_INT8VAR : SPK_ST_FP64VAR;
       sc[s].flags = fl;
     }
     __syncthreads();

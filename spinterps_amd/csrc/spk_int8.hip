@@ -1,0 +1,306 @@
+// Variance contraction by exact int8 plane products (mixed path).
+//
+// For steps whose condition estimate is above the FP32 limit but at most
+// the int8 limit, |L^-1 delta|^2 is formed from integer products:
+//   * each row i of L^-1 is scaled by 2^-e_i (max_k |L^-1_ik| < 2^e_i) and
+//     each cell column of delta by 2^-f_c, so every scaled entry u lies in
+//     (-1, 1);
+//   * u is cut into kPlanes signed 7-bit planes by repeated truncation,
+//     u = sum_s a_s 2^-7(s+1) + r with |a_s| <= 127 and |r| < 2^-7 kPlanes;
+//   * Z_d = sum_{s+t=d} A_s B_t is summed exactly in int32 by
+//     mfma_i32_16x16x64_i8 (|Z_d| <= 5 * 2048 * 127^2 < 2^31), for the
+//     diagonals d <= kDiag;
+//   * y_ic = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d,ic in FP64 and the squared
+//     norms go to the same 64-row partials as the FP64 contraction.
+// The only rounding is the truncation of the planes and of the dropped
+// diagonals (relative 2^-35 of the row and column maxima), and the final
+// FP64 combination; nothing is lost to cancellation in the sums.
+#include "spk_krige.h"
+
+namespace spk {
+
+typedef int i4_t __attribute__((ext_vector_type(4)));
+
+// kPlanes int8 planes per operand; plane products with s + t <= kDiag
+constexpr int kPlanes = kInt8Planes, kDiag = 4;
+
+// Cut x / 2^e (|x| < 2^e) into kPlanes 7-bit planes.
+__device__ inline void planes_of(double x, int e, int8_t *out) {
+  double r = ldexp(x, -e);
+#pragma unroll
+  for (int s = 0; s < kPlanes; ++s) {
+    r *= 128.0;
+    const double a = trunc(r);
+    out[s] = (int8_t)(int)a;
+    r -= a;
+  }
+}
+
+// exponent e with |x| < 2^e (0 for x == 0)
+__device__ inline int exp_above(double x) {
+  int e = 0;
+  if (x != 0.0) frexp(x, &e);
+  return e;
+}
+
+__device__ inline bool int8_step(const StepSc &st) {
+  return !st.status && (st.flags & SPK_ST_INT8VAR) &&
+         !(st.flags & SPK_ST_FP64VAR);
+}
+
+// L^-1 of each int8 step (FP64, column-major lower triangle, ld) -> planes
+// A8[step][s][R][KS] (row i, k contiguous; zero above the diagonal and
+// beyond n) and row exponents ea[step][R].  Grid (R / 64, steps), 256
+// threads: 64 rows per block, transposed through LDS in 64 x 64 tiles.
+__global__ __launch_bounds__(256) void k_slice_rows(
+    const double *__restrict__ Li, int64_t ld, int64_t stride,
+    const StepSc *__restrict__ sc, int8_t *__restrict__ A8, int64_t KS,
+    int64_t R, int *__restrict__ ea) {
+  const int sl = blockIdx.y;
+  const StepSc st = sc[sl];
+  if (!int8_step(st)) return;
+  const int n = st.n;
+  const int r0 = blockIdx.x * 64;
+  const double *L = Li + (int64_t)sl * stride;
+  int8_t *Ap = A8 + (int64_t)sl * kPlanes * R * KS;
+  __shared__ double tile[64][65];
+  __shared__ double red[4][64];
+  __shared__ int rexp[64];
+  const int tid = threadIdx.x, rr = tid & 63, kq = tid >> 6;
+  const int row = r0 + rr;
+  double mx = 0.0;
+  if (row < n)
+    for (int k = kq; k <= row; k += 4)
+      mx = fmax(mx, fabs(L[row + (int64_t)k * ld]));
+  red[kq][rr] = mx;
+  __syncthreads();
+  if (tid < 64) {
+    const double m = fmax(fmax(red[0][tid], red[1][tid]),
+                          fmax(red[2][tid], red[3][tid]));
+    const int e = exp_above(m);
+    rexp[tid] = e;
+    ea[(int64_t)sl * R + r0 + tid] = e;
+  }
+  __syncthreads();
+  // writer lane: row wrow, 16 k's starting at 16 wseg
+  const int wrow = tid >> 2, wseg = tid & 3;
+  for (int k0 = 0; k0 < KS; k0 += 64) {
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int kk = e >> 6, r2 = e & 63;
+      const int k = k0 + kk, i = r0 + r2;
+      tile[kk][r2] = (i < n && k <= i) ? L[i + (int64_t)k * ld] : 0.0;
+    }
+    __syncthreads();
+    int8_t p[kPlanes][16];
+    const int e = rexp[wrow];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      int8_t q[kPlanes];
+      planes_of(tile[wseg * 16 + j][wrow], e, q);
+#pragma unroll
+      for (int s = 0; s < kPlanes; ++s) p[s][j] = q[s];
+    }
+#pragma unroll
+    for (int s = 0; s < kPlanes; ++s)
+      *reinterpret_cast<i4_t *>(Ap + (int64_t)s * R * KS +
+                                (int64_t)(r0 + wrow) * KS + k0 + wseg * 16) =
+          *reinterpret_cast<const i4_t *>(p[s]);
+    __syncthreads();
+  }
+}
+
+// delta of each int8 step (FP64, [step][k][c] with ldk, rows < n valid,
+// cells < Cs valid) -> planes B8[step][t][ldk][KS] (cell c, k contiguous;
+// zero beyond n and beyond Cs) and cell exponents eb[step][ldk] from the
+// per-cell maxima dmax[step][ldk] that k_kgen leaves.  Grid (ldk / 64,
+// KS / 64, steps), 256 threads.
+__global__ __launch_bounds__(256) void k_slice_cols(
+    const double *__restrict__ K64, int64_t ld, int64_t ldk, int64_t Cs,
+    const StepSc *__restrict__ sc, const double *__restrict__ dmax,
+    int8_t *__restrict__ B8, int64_t KS, int *__restrict__ eb) {
+  const int sl = blockIdx.z;
+  const StepSc st = sc[sl];
+  if (!int8_step(st)) return;
+  const int n = st.n;
+  const int64_t c0 = blockIdx.x * (int64_t)64;
+  const int k0 = blockIdx.y * 64;
+  const double *D = K64 + (int64_t)sl * ld * ldk;
+  int8_t *Bp = B8 + (int64_t)sl * kPlanes * ldk * KS;
+  __shared__ double tile[64][65];
+  __shared__ int cexp[64];
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const int64_t c = c0 + tid;
+    const int e = c < Cs ? exp_above(dmax[(int64_t)sl * ldk + c]) : 0;
+    cexp[tid] = e;
+    if (blockIdx.y == 0) eb[(int64_t)sl * ldk + c] = e;
+  }
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int kk = e >> 6, cc = e & 63;
+    const int k = k0 + kk;
+    const int64_t c = c0 + cc;
+    tile[kk][cc] = (k < n && c < Cs) ? D[(int64_t)k * ldk + c] : 0.0;
+  }
+  __syncthreads();
+  const int wc = tid >> 2, wseg = tid & 3;
+  int8_t p[kPlanes][16];
+  const int e = cexp[wc];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    int8_t q[kPlanes];
+    planes_of(tile[wseg * 16 + j][wc], e, q);
+#pragma unroll
+    for (int s = 0; s < kPlanes; ++s) p[s][j] = q[s];
+  }
+#pragma unroll
+  for (int s = 0; s < kPlanes; ++s)
+    *reinterpret_cast<i4_t *>(Bp + (int64_t)s * ldk * KS +
+                              (c0 + wc) * KS + k0 + wseg * 16) =
+        *reinterpret_cast<const i4_t *>(p[s]);
+}
+
+// q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the planes.
+// Block tile 128 rows x 64 cells, k in chunks of 64 (one MFMA depth) up to
+// the tile's last row; 4 waves in 2 x 2, each 64 rows x 32 cells = 4 x 2
+// MFMA tiles per diagonal, the kDiag + 1 diagonal sums held in int32 for
+// the whole k range.  Each chunk's planes are loaded into registers while
+// the previous chunk is multiplied out of LDS (rows padded to 80 bytes so
+// the 16-byte reads of 16 rows hit distinct banks).  XCD remap and the
+// 64-row partial layout are those of k_tri64.
+__global__ __launch_bounds__(256, 1) void k_tri8(
+    const int8_t *__restrict__ A8, int64_t R, const int *__restrict__ ea,
+    const int8_t *__restrict__ B8, int64_t ldk, const int *__restrict__ eb,
+    int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
+    double *__restrict__ qpart, int nrb) {
+  constexpr int BM = 128, BN = 64, BK = 64, RS = 80, TJ = 2;
+  const int G = gridDim.x * gridDim.y;
+  int lin = blockIdx.x + blockIdx.y * gridDim.x;
+  if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
+  const int rti = lin % gridDim.x, cti = lin / gridDim.x;
+  const int sl = blockIdx.z;
+  const StepSc st = sc[sl];
+  if (!int8_step(st)) return;
+  const int n = st.n;
+  const int r0 = rti * BM;
+  if (r0 >= n) return;
+  const int64_t c0 = cti * (int64_t)BN;
+  if (c0 >= Cs) return;
+  const int kend = min(n, r0 + BM);
+  const int8_t *A = A8 + (int64_t)sl * kPlanes * R * KS;
+  const int8_t *B = B8 + (int64_t)sl * kPlanes * ldk * KS;
+  __shared__ __align__(16) int8_t As[kPlanes][BM * RS];
+  __shared__ __align__(16) int8_t Bs[kPlanes][BN * RS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  // staging: A chunk = 128 rows x 64 B per plane (2 x 16 B per thread),
+  // B chunk = 64 cells x 64 B per plane (16 B per thread)
+  i4_t ra[kPlanes][2], rb[kPlanes];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < kPlanes; ++s) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
+        ra[s][q] = *reinterpret_cast<const i4_t *>(
+            A + (int64_t)s * R * KS + (int64_t)(r0 + row) * KS + k0 +
+            seg * 16);
+      }
+      const int row = tid >> 2, seg = tid & 3;
+      rb[s] = *reinterpret_cast<const i4_t *>(
+          B + (int64_t)s * ldk * KS + (c0 + row) * KS + k0 + seg * 16);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int s = 0; s < kPlanes; ++s) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
+        *reinterpret_cast<i4_t *>(&As[s][row * RS + seg * 16]) = ra[s][q];
+      }
+      const int row = tid >> 2, seg = tid & 3;
+      *reinterpret_cast<i4_t *>(&Bs[s][row * RS + seg * 16]) = rb[s];
+    }
+  };
+  i4_t acc[kDiag + 1][4][TJ];
+#pragma unroll
+  for (int d = 0; d <= kDiag; ++d)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < TJ; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
+  const int nch = (kend + BK - 1) / BK;
+  load(0);
+  store();
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    if (ch + 1 < nch) load((ch + 1) * BK);
+    i4_t af[kPlanes][4];
+#pragma unroll
+    for (int s = 0; s < kPlanes; ++s)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+        af[s][ti] = *reinterpret_cast<const i4_t *>(
+            &As[s][(wr * 64 + ti * 16 + li) * RS + lg * 16]);
+#pragma unroll
+    for (int t = 0; t < kPlanes; ++t) {
+      i4_t bf[TJ];
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj)
+        bf[tj] = *reinterpret_cast<const i4_t *>(
+            &Bs[t][(wc * 32 + tj * 16 + li) * RS + lg * 16]);
+#pragma unroll
+      for (int s = 0; s < kPlanes; ++s) {
+        if (s + t > kDiag) continue;
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < TJ; ++tj)
+            acc[s + t][ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                af[s][ti], bf[tj], acc[s + t][ti][tj], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    if (ch + 1 < nch) {
+      store();
+      __syncthreads();
+    }
+  }
+  // y = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d; result slot r of lane group lg
+  // is row 4 lg + r of the 16 x 16 tile
+  const int *er = ea + (int64_t)sl * R;
+  double part[TJ] = {0.0, 0.0};
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = r0 + wr * 64 + ti * 16 + 4 * lg + r;
+      if (row >= n) continue;
+      const int e2 = 2 * er[row];
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj) {
+        double y = 0.0;
+#pragma unroll
+        for (int d = kDiag; d >= 0; --d)
+          y += ldexp((double)acc[d][ti][tj][r], -7 * (d + 2));
+        part[tj] += ldexp(y * y, e2);
+      }
+    }
+#pragma unroll
+  for (int tj = 0; tj < TJ; ++tj) {
+    part[tj] += __shfl_xor(part[tj], 16, 64);
+    part[tj] += __shfl_xor(part[tj], 32, 64);
+  }
+  const int prow = 2 * rti + wr;
+  if (lg != 0 || prow >= nrb) return;
+#pragma unroll
+  for (int tj = 0; tj < TJ; ++tj) {
+    const int64_t col = c0 + wc * 32 + tj * 16 + li;
+    if (col < Cs)
+      qpart[((int64_t)sl * nrb + prow) * ldk + col] =
+          ldexp(part[tj], 2 * eb[(int64_t)sl * ldk + col]);
+  }
+}
+
+}  // namespace spk

@@ -40,7 +40,8 @@ __global__ void k_dual(const double *Lc, const double *Li, int64_t ld,
                        const int32_t *idx, int64_t N, const double *stn_drift,
                        int m, int method, double *V, int64_t ldv, double *T0,
                        const spk_vg *vgs, double min_vg_val,
-                       double fp64var_thr, double fp64var_thr_gau);
+                       double fp64var_thr, double fp64var_thr_gau,
+                       double int8var_thr);
 __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const int32_t *idx, int64_t N,
                        const double *stn_drift, int m, int method, double *M,
@@ -55,5 +56,17 @@ int cell_order(spk_ctx *ctx, const double *cx, const double *cy, int64_t C,
                int32_t *perm, double *pcx, double *pcy);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
+// exact int8 plane products (spk_int8.hip): 5 planes per operand
+constexpr int kInt8Planes = 5;
+__global__ void k_slice_rows(const double *Li, int64_t ld, int64_t stride,
+                             const StepSc *sc, int8_t *A8, int64_t KS,
+                             int64_t R, int *ea);
+__global__ void k_slice_cols(const double *K64, int64_t ld, int64_t ldk,
+                             int64_t Cs, const StepSc *sc, const double *dmax,
+                             int8_t *B8, int64_t KS, int *eb);
+__global__ void k_tri8(const int8_t *A8, int64_t R, const int *ea,
+                       const int8_t *B8, int64_t ldk, const int *eb,
+                       int64_t KS, const StepSc *sc, int64_t Cs,
+                       double *qpart, int nrb);
 
 }  // namespace spk

@@ -115,7 +115,8 @@ def _ctx_for(device):
 def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
                  precision='fp64', want_var=True, idw_exp=2.0,
                  min_vg_val=0.0, min_var_thr=-np.inf, est=None, var=None,
-                 vg_dev=None, kappa=None, fp64var_kappa=0.0):
+                 vg_dev=None, kappa=None, fp64var_kappa=0.0,
+                 int8var_kappa=0.0):
     """Interpolate T steps.  vals [T x N] (NaN = missing station).
 
     method: 'OK', 'SK' (the reference's SK: lambda^T z, nearest station
@@ -131,7 +132,9 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     system's condition estimate.  ``fp64var_kappa`` (mixed precision):
     condition estimate above which a step's variance contraction runs in
     FP64, for every variogram model (0 = the library's policy: 1e5, and
-    1e3 for steps with a Gaussian component).
+    1e3 for steps with a Gaussian component).  ``int8var_kappa``: up to
+    this condition estimate such steps use exact int8 plane products
+    (flag 64), above it FP64 (0 = no limit, negative = never).
     """
     dev = prob.device
     v = _t64(vals, dev)
@@ -169,6 +172,7 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     opts.min_vg_val = float(min_vg_val)
     opts.min_var_thr = float(min_var_thr)
     opts.fp64var_kappa = float(fp64var_kappa)
+    opts.int8var_kappa = float(int8var_kappa)
     opts.sk_mean = 1 if sk_mean else 0
     ctx = _ctx_for(dev)
     pst = prob.c_struct()
