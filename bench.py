@@ -38,7 +38,7 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--batch', type=int, default=64,
                     help='time steps per bench step')
@@ -309,7 +309,7 @@ def main():
                     'unit': 'TFLOP/s', 'frac': ach32 / 157.3,
                     'traffic': TRI32_TRAFFIC,
                     'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA, '
-                              'FP64 sums every 32 rows)',
+                              'FP64 sums every 16 rows)',
                     'traffic_note': note}
     out = {
         'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '

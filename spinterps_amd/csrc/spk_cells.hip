@@ -1551,7 +1551,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
         if (prec == SPK_P_MIXED) {
           ProfScope ps(ctx, "gemm_tri_f32");
-          hipLaunchKernelGGL((k_gemm<float, false, 32>), grid, dim3(256), 0, sm,
+          // k chunks of 16: FP32 sums of 16 products, 3 waves per SIMD
+          hipLaunchKernelGGL((k_gemm<float, false, 16>), grid, dim3(256), 0, sm,
                              q.L32, ld, strideM, (const float *)K32, ldk,
                              ld * ldk, q.sc, p, method, 1, 0, q.zact, ldz,
                              (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
