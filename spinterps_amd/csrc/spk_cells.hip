@@ -819,12 +819,11 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
 #pragma unroll
           for (int q = 0; q < kJ; ++q)
             if (slt[q] >= 0) delta += sG[slt[q]][j] * beta[q];
-          if (K32s) {
+          if (K32s)
             K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
-          } else {
+          else
             K64s[(int64_t)(base + j) * ldk + c] = delta;
-            dm = fmax(dm, fabs(delta));
-          }
+          dm = fmax(dm, fabs(delta));
         }
       } else {
         for (int j = 0; j < cnt; ++j) {
@@ -1531,7 +1530,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         if (B8) {
           {
             ProfScope ps(ctx, "slice_cols");
-            hipLaunchKernelGGL(k_slice_cols,
+            hipLaunchKernelGGL((k_slice_cols<double, kInt8Planes>),
                                dim3((unsigned)(ldk / 64), (unsigned)(KS8 / 64),
                                     Sb),
                                dim3(256), 0, sm, (const double *)K64, ld, ldk,
@@ -1539,7 +1538,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           }
           {
             ProfScope ps(ctx, "gemm_tri_i8");
-            hipLaunchKernelGGL(k_tri8,
+            hipLaunchKernelGGL((k_tri8<kInt8Planes, 4, 2>),
                                dim3((unsigned)ceil_div(N, 128),
                                     (unsigned)(ldk / 64), Sb),
                                dim3(256), 0, sm, (const int8_t *)q.L8, R8,

@@ -87,6 +87,7 @@ __global__ __launch_bounds__(256) void k_compact(
     }
     o.status = st;
     o.flags = 0;
+    o.planes = 0;
     o.kappa = 0.0;
     o.c0 = 0.0;
     o.b = 0.0;
@@ -592,9 +593,16 @@ __global__ __launch_bounds__(256) void k_dual(
       for (int k = 0; k < vgs[s].n; ++k) gau |= vgs[s].type[k] == SPK_VG_GAU;
       // above the FP32 limit: exact int8 plane products up to the int8
       // limit, FP64 beyond it
-      if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr)))
-        fl |= kap <= int8var_thr ? SPK_ST_INT8VAR : SPK_ST_FP64VAR;
+      int planes = 0;
+      if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr))) {
+        if (kap <= int8var_thr)
+          planes = kInt8Planes;
+        else
+          fl |= SPK_ST_FP64VAR;
+      }
+      if (planes) fl |= SPK_ST_INT8VAR;
       sc[s].flags = fl;
+      sc[s].planes = planes;
     }
     __syncthreads();
   }
@@ -810,7 +818,9 @@ __global__ void k_cast_tri(const double *__restrict__ Li, float *__restrict__ L3
                            int64_t ld, int64_t stride,
                            const StepSc *__restrict__ sc) {
   const int s = blockIdx.y;
-  if (sc[s].status) return;
+  // only the steps of the FP32 contraction read L32
+  if (sc[s].status || (sc[s].flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR)))
+    return;
   const int n = sc[s].n;
   const int64_t tot = (int64_t)n * n;
   const double *src = Li + (int64_t)s * stride;

@@ -1,19 +1,22 @@
 // Variance contraction by exact int8 plane products (mixed path).
 //
-// For steps whose condition estimate is above the FP32 limit but at most
-// the int8 limit, |L^-1 delta|^2 is formed from integer products:
+// For steps above the FP32 limit, |L^-1 delta|^2 is formed from integer
+// products (the kernels are written for S planes and diagonals up to P;
+// the library uses S = 5, P = 4):
 //   * each row i of L^-1 is scaled by 2^-e_i (max_k |L^-1_ik| < 2^e_i) and
 //     each cell column of delta by 2^-f_c, so every scaled entry u lies in
 //     (-1, 1);
-//   * u is cut into kPlanes signed 7-bit planes by repeated truncation,
-//     u = sum_s a_s 2^-7(s+1) + r with |a_s| <= 127 and |r| < 2^-7 kPlanes;
+//   * u is cut into S signed 7-bit planes by repeated truncation,
+//     u = sum_s a_s 2^-7(s+1) + r with |a_s| <= 127 and |r| < 2^-7 S (the
+//     planes are the leading digits, so the first 4 of 5 planes are the
+//     4-plane cut);
 //   * Z_d = sum_{s+t=d} A_s B_t is summed exactly in int32 by
 //     mfma_i32_16x16x64_i8 (|Z_d| <= 5 * 2048 * 127^2 < 2^31), for the
-//     diagonals d <= kDiag;
+//     diagonals d <= P;
 //   * y_ic = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d,ic in FP64 and the squared
 //     norms go to the same 64-row partials as the FP64 contraction.
 // The only rounding is the truncation of the planes and of the dropped
-// diagonals (relative 2^-35 of the row and column maxima), and the final
+// diagonals (relative 2^-7S of the row and column maxima) and the final
 // FP64 combination; nothing is lost to cancellation in the sums.
 #include "spk_krige.h"
 
@@ -21,19 +24,23 @@ namespace spk {
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
 
-// kPlanes int8 planes per operand; plane products with s + t <= kDiag
-constexpr int kPlanes = kInt8Planes, kDiag = 4;
+// planes stored per operand (the 5-plane cut)
+constexpr int kPlanes = kInt8Planes;
 
-// Cut x / 2^e (|x| < 2^e) into kPlanes 7-bit planes.
+// Cut x / 2^e (|x| < 2^e) into kPlanes 7-bit planes: the base-128 digits
+// of trunc(|x| 2^(7 kPlanes - e)), most significant first, with the sign of
+// x (the same digits as truncating 128 x / 2^e plane by plane).  The
+// scaling is exact; the digits above 2^28 and below are split in FP64 so
+// that each part converts exactly to int32.
 __device__ inline void planes_of(double x, int e, int8_t *out) {
-  double r = ldexp(x, -e);
+  const double w = ldexp(fabs(x), 7 * kPlanes - e);  // < 2^35
+  const double hi = trunc(ldexp(w, -28));            // < 2^7
+  const int lo = (int)(w - ldexp(hi, 28));           // < 2^28
+  const long long t = ((long long)(int)hi << 28) | (long long)lo;
+  const int sg = x < 0.0 ? -1 : 1;
 #pragma unroll
-  for (int s = 0; s < kPlanes; ++s) {
-    r *= 128.0;
-    const double a = trunc(r);
-    out[s] = (int8_t)(int)a;
-    r -= a;
-  }
+  for (int s = 0; s < kPlanes; ++s)
+    out[s] = (int8_t)(sg * (int)((t >> (7 * (kPlanes - 1 - s))) & 127));
 }
 
 // exponent e with |x| < 2^e (0 for x == 0)
@@ -109,29 +116,33 @@ __global__ __launch_bounds__(256) void k_slice_rows(
   }
 }
 
-// delta of each int8 step (FP64, [step][k][c] with ldk, rows < n valid,
-// cells < Cs valid) -> planes B8[step][t][ldk][KS] (cell c, k contiguous;
-// zero beyond n and beyond Cs) and cell exponents eb[step][ldk] from the
-// per-cell maxima dmax[step][ldk] that k_kgen leaves.  Grid (ldk / 64,
-// KS / 64, steps), 256 threads.
+// delta of each int8 step (T = double, [step][k][c] with ldk, rows < n
+// valid, cells < Cs valid) -> the first S planes B8[step][t][ldk][KS]
+// (cell c, k contiguous; zero beyond n and beyond Cs) and cell exponents
+// eb[step][ldk] from the per-cell maxima dmax[step][ldk] that k_kgen
+// leaves (for FP32 input raised by 2^-22 so that the rounded values stay
+// below 2^e).  Grid (ldk / 64, KS / 64, steps), 256 threads: 64 cells x 64
+// k per block, transposed through LDS.
+template <typename T, int S>
 __global__ __launch_bounds__(256) void k_slice_cols(
-    const double *__restrict__ K64, int64_t ld, int64_t ldk, int64_t Cs,
+    const T *__restrict__ K, int64_t ld, int64_t ldk, int64_t Cs,
     const StepSc *__restrict__ sc, const double *__restrict__ dmax,
     int8_t *__restrict__ B8, int64_t KS, int *__restrict__ eb) {
   const int sl = blockIdx.z;
   const StepSc st = sc[sl];
-  if (!int8_step(st)) return;
+  if (!int8_step(st) || st.planes != S) return;
   const int n = st.n;
   const int64_t c0 = blockIdx.x * (int64_t)64;
   const int k0 = blockIdx.y * 64;
-  const double *D = K64 + (int64_t)sl * ld * ldk;
+  const T *D = K + (int64_t)sl * ld * ldk;
   int8_t *Bp = B8 + (int64_t)sl * kPlanes * ldk * KS;
   __shared__ double tile[64][65];
   __shared__ int cexp[64];
   const int tid = threadIdx.x;
   if (tid < 64) {
     const int64_t c = c0 + tid;
-    const int e = c < Cs ? exp_above(dmax[(int64_t)sl * ldk + c]) : 0;
+    const double m = c < Cs ? dmax[(int64_t)sl * ldk + c] : 0.0;
+    const int e = exp_above(sizeof(T) == 4 ? m * (1.0 + 0x1p-22) : m);
     cexp[tid] = e;
     if (blockIdx.y == 0) eb[(int64_t)sl * ldk + c] = e;
   }
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(256) void k_slice_cols(
     const int kk = e >> 6, cc = e & 63;
     const int k = k0 + kk;
     const int64_t c = c0 + cc;
-    tile[kk][cc] = (k < n && c < Cs) ? D[(int64_t)k * ldk + c] : 0.0;
+    tile[kk][cc] = (k < n && c < Cs) ? (double)D[(int64_t)k * ldk + c] : 0.0;
   }
   __syncthreads();
   const int wc = tid >> 2, wseg = tid & 3;
@@ -153,33 +164,40 @@ __global__ __launch_bounds__(256) void k_slice_cols(
     for (int s = 0; s < kPlanes; ++s) p[s][j] = q[s];
   }
 #pragma unroll
-  for (int s = 0; s < kPlanes; ++s)
+  for (int s = 0; s < S; ++s)
     *reinterpret_cast<i4_t *>(Bp + (int64_t)s * ldk * KS +
                               (c0 + wc) * KS + k0 + wseg * 16) =
         *reinterpret_cast<const i4_t *>(p[s]);
 }
+template __global__ void k_slice_cols<double, 5>(
+    const double *, int64_t, int64_t, int64_t, const StepSc *,
+    const double *, int8_t *, int64_t, int *);
 
-// q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the planes.
-// Block tile 128 rows x 64 cells, k in chunks of 64 (one MFMA depth) up to
-// the tile's last row; 4 waves in 2 x 2, each 64 rows x 32 cells = 4 x 2
-// MFMA tiles per diagonal, the kDiag + 1 diagonal sums held in int32 for
-// the whole k range.  Each chunk's planes are loaded into registers while
-// the previous chunk is multiplied out of LDS (rows padded to 80 bytes so
-// the 16-byte reads of 16 rows hit distinct banks).  XCD remap and the
-// 64-row partial layout are those of k_tri64.
+// q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the first S
+// planes, diagonals s + t <= P.  Block tile 128 rows x 32 TJ cells, k in
+// chunks of 64 (one MFMA depth) up to the tile's last row; 4 waves in
+// 2 x 2, each 64 rows x 16 TJ cells = 4 x TJ MFMA tiles per diagonal, the
+// P + 1 diagonal sums held in int32 for the whole k range.  Each chunk's
+// planes are loaded into registers while the previous chunk is multiplied
+// out of LDS (rows padded to 80 bytes so the 16-byte reads of 16 rows hit
+// distinct banks).  XCD remap and the 64-row partial layout are those of
+// k_tri64.
+template <int S, int P, int TJ>
 __global__ __launch_bounds__(256, 1) void k_tri8(
     const int8_t *__restrict__ A8, int64_t R, const int *__restrict__ ea,
     const int8_t *__restrict__ B8, int64_t ldk, const int *__restrict__ eb,
     int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
     double *__restrict__ qpart, int nrb) {
-  constexpr int BM = 128, BN = 64, BK = 64, RS = 80, TJ = 2;
+  constexpr int BM = 128, BN = 32 * TJ, BK = 64, RS = 80;
+  constexpr int NB = BN / 64;  // 16-byte B loads per plane and thread
+  static_assert(S <= kPlanes && P <= 2 * (S - 1) && NB >= 1, "shape");
   const int G = gridDim.x * gridDim.y;
   int lin = blockIdx.x + blockIdx.y * gridDim.x;
   if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
   const int rti = lin % gridDim.x, cti = lin / gridDim.x;
   const int sl = blockIdx.z;
   const StepSc st = sc[sl];
-  if (!int8_step(st)) return;
+  if (!int8_step(st) || st.planes != S) return;
   const int n = st.n;
   const int r0 = rti * BM;
   if (r0 >= n) return;
@@ -188,17 +206,17 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   const int kend = min(n, r0 + BM);
   const int8_t *A = A8 + (int64_t)sl * kPlanes * R * KS;
   const int8_t *B = B8 + (int64_t)sl * kPlanes * ldk * KS;
-  __shared__ __align__(16) int8_t As[kPlanes][BM * RS];
-  __shared__ __align__(16) int8_t Bs[kPlanes][BN * RS];
+  __shared__ __align__(16) int8_t As[S][BM * RS];
+  __shared__ __align__(16) int8_t Bs[S][BN * RS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
   // staging: A chunk = 128 rows x 64 B per plane (2 x 16 B per thread),
-  // B chunk = 64 cells x 64 B per plane (16 B per thread)
-  i4_t ra[kPlanes][2], rb[kPlanes];
+  // B chunk = BN cells x 64 B per plane (NB x 16 B per thread)
+  i4_t ra[S][2], rb[S][NB];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int s = 0; s < kPlanes; ++s) {
+    for (int s = 0; s < S; ++s) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int e = tid + q * 256, row = e >> 2, seg = e & 3;
@@ -206,26 +224,32 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
             A + (int64_t)s * R * KS + (int64_t)(r0 + row) * KS + k0 +
             seg * 16);
       }
-      const int row = tid >> 2, seg = tid & 3;
-      rb[s] = *reinterpret_cast<const i4_t *>(
-          B + (int64_t)s * ldk * KS + (c0 + row) * KS + k0 + seg * 16);
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
+        rb[s][q] = *reinterpret_cast<const i4_t *>(
+            B + (int64_t)s * ldk * KS + (c0 + row) * KS + k0 + seg * 16);
+      }
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int s = 0; s < kPlanes; ++s) {
+    for (int s = 0; s < S; ++s) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int e = tid + q * 256, row = e >> 2, seg = e & 3;
         *reinterpret_cast<i4_t *>(&As[s][row * RS + seg * 16]) = ra[s][q];
       }
-      const int row = tid >> 2, seg = tid & 3;
-      *reinterpret_cast<i4_t *>(&Bs[s][row * RS + seg * 16]) = rb[s];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
+        *reinterpret_cast<i4_t *>(&Bs[s][row * RS + seg * 16]) = rb[s][q];
+      }
     }
   };
-  i4_t acc[kDiag + 1][4][TJ];
+  i4_t acc[P + 1][4][TJ];
 #pragma unroll
-  for (int d = 0; d <= kDiag; ++d)
+  for (int d = 0; d <= P; ++d)
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -236,23 +260,23 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
     if (ch + 1 < nch) load((ch + 1) * BK);
-    i4_t af[kPlanes][4];
+    i4_t af[S][4];
 #pragma unroll
-    for (int s = 0; s < kPlanes; ++s)
+    for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti)
         af[s][ti] = *reinterpret_cast<const i4_t *>(
             &As[s][(wr * 64 + ti * 16 + li) * RS + lg * 16]);
 #pragma unroll
-    for (int t = 0; t < kPlanes; ++t) {
+    for (int t = 0; t < S; ++t) {
       i4_t bf[TJ];
 #pragma unroll
       for (int tj = 0; tj < TJ; ++tj)
         bf[tj] = *reinterpret_cast<const i4_t *>(
-            &Bs[t][(wc * 32 + tj * 16 + li) * RS + lg * 16]);
+            &Bs[t][(wc * 16 * TJ + tj * 16 + li) * RS + lg * 16]);
 #pragma unroll
-      for (int s = 0; s < kPlanes; ++s) {
-        if (s + t > kDiag) continue;
+      for (int s = 0; s < S; ++s) {
+        if (s + t > P) continue;
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
@@ -270,7 +294,9 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   // y = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d; result slot r of lane group lg
   // is row 4 lg + r of the 16 x 16 tile
   const int *er = ea + (int64_t)sl * R;
-  double part[TJ] = {0.0, 0.0};
+  double part[TJ];
+#pragma unroll
+  for (int tj = 0; tj < TJ; ++tj) part[tj] = 0.0;
 #pragma unroll
   for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
@@ -282,7 +308,7 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
       for (int tj = 0; tj < TJ; ++tj) {
         double y = 0.0;
 #pragma unroll
-        for (int d = kDiag; d >= 0; --d)
+        for (int d = P; d >= 0; --d)
           y += ldexp((double)acc[d][ti][tj][r], -7 * (d + 2));
         part[tj] += ldexp(y * y, e2);
       }
@@ -296,11 +322,15 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   if (lg != 0 || prow >= nrb) return;
 #pragma unroll
   for (int tj = 0; tj < TJ; ++tj) {
-    const int64_t col = c0 + wc * 32 + tj * 16 + li;
+    const int64_t col = c0 + wc * 16 * TJ + tj * 16 + li;
     if (col < Cs)
       qpart[((int64_t)sl * nrb + prow) * ldk + col] =
           ldexp(part[tj], 2 * eb[(int64_t)sl * ldk + col]);
   }
 }
+template __global__ void k_tri8<5, 4, 2>(const int8_t *, int64_t, const int *,
+                                         const int8_t *, int64_t, const int *,
+                                         int64_t, const StepSc *, int64_t,
+                                         double *, int);
 
 }  // namespace spk

@@ -12,7 +12,7 @@ struct StepSc {
   int n;          // active stations
   int status;     // SPK_ST_* that skip the step's regular kernels (0 = run)
   int flags;      // informational SPK_ST_* bits (ill-conditioned, FP64 var)
-  int pad_;
+  int planes;     // int8 planes of the variance contraction (0, 4 or 5)
   double kappa;   // max diag(C^-1) * max diag(C), a condition estimate
   double mean;    // mean of the active values (interp/steps.py:276)
   double c0;      // covariance shift of the OK/EDK system
@@ -56,14 +56,17 @@ int cell_order(spk_ctx *ctx, const double *cx, const double *cy, int64_t C,
                int32_t *perm, double *pcx, double *pcy);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
-// exact int8 plane products (spk_int8.hip): 5 planes per operand
+// exact int8 plane products (spk_int8.hip): 5 planes per operand, plane
+// products up to the fourth diagonal
 constexpr int kInt8Planes = 5;
 __global__ void k_slice_rows(const double *Li, int64_t ld, int64_t stride,
                              const StepSc *sc, int8_t *A8, int64_t KS,
                              int64_t R, int *ea);
-__global__ void k_slice_cols(const double *K64, int64_t ld, int64_t ldk,
-                             int64_t Cs, const StepSc *sc, const double *dmax,
-                             int8_t *B8, int64_t KS, int *eb);
+template <typename T, int S>
+__global__ void k_slice_cols(const T *K, int64_t ld, int64_t ldk, int64_t Cs,
+                             const StepSc *sc, const double *dmax, int8_t *B8,
+                             int64_t KS, int *eb);
+template <int S, int P, int TJ>
 __global__ void k_tri8(const int8_t *A8, int64_t R, const int *ea,
                        const int8_t *B8, int64_t ldk, const int *eb,
                        int64_t KS, const StepSc *sc, int64_t Cs,

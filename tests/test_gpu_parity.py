@@ -319,14 +319,16 @@ def test_multi_batch_overlap_hourly_small(gpu):
 _SOLVED_MASK = 1 | 2 | 4 | 8 | 16   # no stations, mean fill, failed, one station, ill-conditioned
 
 
-def test_mixed_precision_policy_500m_grid(gpu):
+@pytest.mark.parametrize('mode', ['int8', 'fp32', 'int8_limit'])
+def test_mixed_precision_policy_500m_grid(gpu, mode):
     """The mixed path's precision policy on the hourly workload's geometry
-    (2 000 stations, 500 m cells): the variance contraction runs in FP32
-    for spherical and exponential steps with condition estimate up to 1e5
-    and Gaussian steps up to 1e3, by exact int8 plane products above that
-    (FP64 only for a caller's int8 limit, checked here at 1e10).  Against the plain FP64 path every step with
-    condition estimate <= 1e6 stays within 1e-6 (relative, floor 1e-3 x
-    the step's mean)."""
+    (2 000 stations, 500 m cells).  Default ('int8'): the variance
+    contraction runs in FP32 below the FP32 limit (condition estimate 1e5,
+    1e3 for Gaussian models) and by exact int8 plane products above it.
+    'fp32' (int8 switched off): FP64 above the limit.  'int8_limit'
+    (caller's int8 limit 1e10): FP64 above that.  Against the plain FP64 path every step with condition
+    estimate <= 1e6 stays within 1e-6 (relative, floor 1e-3 x the step's
+    mean)."""
     import torch
     from spinterps_amd import _lib
     from spinterps_amd.interp.steps import DeviceProblem, interp_steps
@@ -335,8 +337,9 @@ def test_mixed_precision_policy_500m_grid(gpu):
     vals, vgs = configs.hourly_steps(w, 0, T)
     prob = DeviceProblem.from_workload(w)
     kap = torch.zeros(T, dtype=torch.float64, device='cuda')
+    k8 = {'int8': 0.0, 'fp32': -1.0, 'int8_limit': 1e10}[mode]
     e, v, info = interp_steps(prob, vals, vgs, 'OK', precision='mixed',
-                              kappa=kap, int8var_kappa=1e10)
+                              kappa=kap, int8var_kappa=k8)
     e64, v64, i64 = interp_steps(prob, vals, vgs, 'OK', precision='fp64')
     e, v, inf = e.cpu().numpy(), v.cpu().numpy(), info.cpu().numpy()
     e64, v64, i64 = e64.cpu().numpy(), v64.cpu().numpy(), i64.cpu().numpy()
@@ -344,28 +347,33 @@ def test_mixed_precision_policy_500m_grid(gpu):
     gau = np.array(['Gau' in g for g in vgs])
     fp64 = (inf & _lib.ST_FP64VAR) != 0
     int8 = ((inf & _lib.ST_INT8VAR) != 0) & ~fp64
-    promoted = fp64 | int8
     solved = (i64 & _SOLVED_MASK) == 0
-    assert solved.sum() > 80
-    assert promoted[solved & gau & (k > 1e3)].all()
-    assert promoted[solved & ~gau & (k > 1e5)].all()
     runs = (inf & (1 | 2 | 4 | 8)) == 0      # steps whose systems were solved
-    assert fp64[runs & (k > 1e10)].all()
-    assert (fp64 | int8)[runs & (k > 1e10)].sum() >= 1
-    assert int8[solved & gau & (k > 1e3) & (k <= 1e10)].sum() >= 5
-    fp32 = solved & ~promoted
-    # the FP32 limit above 1e3 is exercised
-    assert (fp32 & ~gau & (k > 1e3)).sum() >= 5, k[fp32]
+    assert solved.sum() > 80
+    above = runs & np.where(gau, k > 1e3, k > 1e5)
+    if mode == 'int8':
+        assert int8[above].all() and not int8[runs & ~above].any()
+        assert not fp64[runs].any()
+        # the FP32 limit above 1e3 is exercised
+        assert (runs & ~above & ~gau & (k > 1e3)).sum() >= 5
+    elif mode == 'fp32':
+        assert not int8[runs].any()
+        assert fp64[above].all() and not fp64[runs & ~above].any()
+        # the FP32 limit above 1e3 is exercised
+        assert (runs & ~fp64 & ~gau & (k > 1e3)).sum() >= 5
+    else:
+        assert fp64[runs & (k > 1e10)].all() and (runs & (k > 1e10)).any()
+        assert int8[above & (k <= 1e10)].all()
     cmp = solved & (k <= 1e6)
     floor_e = 1e-3 * np.abs(e64).mean(axis=1, keepdims=True)
     floor_v = 1e-3 * np.abs(v64).mean(axis=1, keepdims=True)
     with np.errstate(invalid='ignore'):   # steps without variance are NaN
         re = (np.abs(e - e64) / np.maximum(np.abs(e64), floor_e)).max(axis=1)
         rv = (np.abs(v - v64) / np.maximum(np.abs(v64), floor_v)).max(axis=1)
-    print(f'fp32 steps {fp32.sum()}, int8 {int8[solved].sum()}, fp64 '
-          f'{fp64[solved].sum()}; est max {re[cmp].max():.2e}, var max fp32 '
-          f'{rv[cmp & fp32].max():.2e} int8 {rv[cmp & int8].max(initial=0):.2e}'
-          f' fp64 {rv[cmp & fp64].max(initial=0):.2e}')
+    print(f'{mode}: int8 {int8[runs].sum()}, fp64 {fp64[runs].sum()} of '
+          f'{runs.sum()} steps; est max {re[cmp].max():.2e}, var max below '
+          f'the FP32 limit {rv[cmp & ~above].max():.2e}, above '
+          f'{rv[cmp & above].max(initial=0):.2e}')
     worst = np.where(cmp)[0][np.argmax(rv[cmp])]
     assert re[cmp].max() < 1e-6
     assert rv[cmp].max() < 1e-6, (worst, k[worst], vgs[worst])
@@ -397,13 +405,13 @@ def test_mixed_slot_overflow_promotes(gpu):
 
 
 def test_int8_plane_contraction_matches_fp64(gpu):
-    """Every step forced off FP32 (limit 1e-300) and onto the int8 plane
-    products: the variances match the batched FP64 path, whose contraction
-    is the same |L^-1 delta|^2 in FP64, to 1e-9 relative (floor 1e-3 x
-    the step's mean variance) up to condition estimate 1e5, 1e-7 up to
-    1e6, 1e-5 up to 1e8 and 1e-4 up to 1e12, where cancellation between
-    the FP64 terms amplifies the planes' truncation (2^-35 of the row and
-    column maxima)."""
+    """Exact int8 plane products (every step forced above the FP32 limit)
+    against the batched FP64 path, whose contraction is the same
+    |L^-1 delta|^2 in FP64 (relative, floor 1e-3 x the step's mean
+    variance): 1e-9 up to condition estimate 1e5, 1e-7 up to 1e6, 1e-5 up
+    to 1e8 and 1e-4 up to 1e12, where cancellation between the FP64 terms
+    amplifies the planes' truncation (2^-35 of the row and column
+    maxima)."""
     import torch
     from spinterps_amd import _lib
     from spinterps_amd.interp.steps import DeviceProblem, interp_steps
