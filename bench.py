@@ -24,11 +24,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 
-# HBM bytes per dispatch of the two variance contractions, measured by
-# rocprofv3 --pmc on this workload (batches of 64 steps, stripes of 10 880
-# cells; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with k_clip_cast)
+# HBM bytes per dispatch of the variance contractions, measured by
+# rocprofv3 --pmc on this workload (batches of 64 steps, stripes of 10 752
+# cells; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with k_clip_cast).  The
+# FP64 figure is from a run in which k_tri64 contracted every step above
+# the FP32 limit (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
-TRI32_TRAFFIC = 2 * 5439736 * 1024 + 68983 * 1024
+TRI32_TRAFFIC = 2 * 5998922 * 1024 + 72858 * 1024
+TRI8_TRAFFIC = 2 * 5467884 * 1024 + 24228 * 1024
 
 
 def log(*a):
@@ -135,18 +138,29 @@ def main():
     prof_enable(0, False)
     info_t = torch.cat(infos).cpu().numpy()
     timed_n = n_act[W * B:(W + K) * B]
-    f32_steps = ((info_t & 63) & ~_lib.ST_ILLCOND) == 0
-    f64_steps = (info_t & 32) != 0
+    solved_t = (info_t & (_lib.ST_NOSTNS | _lib.ST_MEANFILL | _lib.ST_FAILED |
+                          _lib.ST_ONESTN)) == 0
+    f64_steps = solved_t & ((info_t & _lib.ST_FP64VAR) != 0)
+    i8_steps = solved_t & ~f64_steps & ((info_t & _lib.ST_INT8VAR) != 0)
+    f32_steps = solved_t & ~f64_steps & ~i8_steps
     # algorithmic flops of W = L^-1 delta: n^2 per cell (n(n+1)/2 MACs)
-    fl32 = float(np.sum(timed_n[f32_steps].astype(np.float64) ** 2) * C)
-    fl64 = float(np.sum(timed_n[f64_steps].astype(np.float64) ** 2) * C)
+    nsq = timed_n.astype(np.float64) ** 2
+    fl32 = float(np.sum(nsq[f32_steps]) * C)
+    fl64 = float(np.sum(nsq[f64_steps]) * C)
+    fl8 = float(np.sum(nsq[i8_steps]) * C)
     ms32, c32 = kern['gemm_tri_f32']
     ms64, c64 = kern['gemm_tri_f64']
+    ms8, c8 = kern['gemm_tri_i8']
     ach32 = fl32 / (ms32 * 1e-3) / 1e12 if ms32 > 0 else 0.0
     ach64 = fl64 / (ms64 * 1e-3) / 1e12 if ms64 > 0 else 0.0
+    # int8: 15 plane products of the triangle per step (int8 ops) and the
+    # FP64-equivalent rate of the same L^-1 delta
+    ach8 = 15 * fl8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
+    eq8 = fl8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
     # kgen: algorithmic HBM bytes = delta written (4 or 8 B) + cell coords
     kb = float(np.sum(timed_n[f32_steps]) * C * 4 +
-               np.sum(timed_n[f64_steps]) * C * 8 + 16 * C * K * B)
+               np.sum(timed_n[f64_steps | i8_steps]) * C * 8 +
+               16 * C * K * B)
     mskg, ckg = kern['kgen']
     kg_gbs = kb / (mskg * 1e-3) / 1e9 if mskg > 0 else 0.0
     log('kernel totals (ms, launches): ' +
@@ -298,19 +312,22 @@ def main():
     # share of the timed region
     note = ('HBM bytes per dispatch, FETCH_SIZE x2 + WRITE_SIZE x1 '
             '(calibrated), from profiles/kernels_hourly_mixed_b64.md')
-    if ms64 >= ms32:
-        dominant = {'bound': 'mfma', 'achieved': ach64, 'peak': 78.6,
-                    'unit': 'TFLOP/s', 'frac': ach64 / 78.6,
-                    'traffic': TRI64_TRAFFIC,
-                    'kernel': 'k_tri64 (L^-1 delta, FP64 MFMA)',
-                    'traffic_note': note}
-    else:
-        dominant = {'bound': 'mfma', 'achieved': ach32, 'peak': 157.3,
-                    'unit': 'TFLOP/s', 'frac': ach32 / 157.3,
-                    'traffic': TRI32_TRAFFIC,
-                    'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA, '
-                              'FP64 sums every 16 rows)',
-                    'traffic_note': note}
+    r64 = {'bound': 'mfma', 'achieved': ach64, 'peak': 78.6,
+           'unit': 'TFLOP/s', 'frac': ach64 / 78.6,
+           'traffic': TRI64_TRAFFIC,
+           'kernel': 'k_tri64 (L^-1 delta, FP64 MFMA)', 'traffic_note': note}
+    r32 = {'bound': 'mfma', 'achieved': ach32, 'peak': 157.3,
+           'unit': 'TFLOP/s', 'frac': ach32 / 157.3,
+           'traffic': TRI32_TRAFFIC,
+           'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA, FP64 sums '
+                     'every 16 rows)', 'traffic_note': note}
+    r8 = {'bound': 'mfma', 'achieved': ach8, 'peak': 5000.0,
+          'unit': 'TOPS', 'frac': ach8 / 5000.0, 'traffic': TRI8_TRAFFIC,
+          'fp64_equivalent_tflops': eq8,
+          'kernel': 'k_tri8 (L^-1 delta, exact int8 plane products, int8 '
+                    'MFMA)', 'traffic_note': note}
+    dominant = max(((ms32, r32), (ms64, r64), (ms8, r8)),
+                   key=lambda x: x[0])[1]
     out = {
         'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '
                   'plain FP64 @<=1e-6 rel',
@@ -343,12 +360,18 @@ def main():
             'gemm_tri_f64': {'bound': 'mfma', 'achieved': ach64,
                              'peak': 78.6, 'unit': 'TFLOP/s',
                              'frac': ach64 / 78.6},
+            'gemm_tri_i8': {'bound': 'mfma', 'achieved': ach8,
+                            'peak': 5000.0, 'unit': 'TOPS',
+                            'frac': ach8 / 5000.0,
+                            'fp64_equivalent_tflops': eq8},
             'kgen': {'bound': 'hbm', 'achieved': kg_gbs, 'peak': 8000.0,
                      'unit': 'GB/s', 'frac': kg_gbs / 8000.0},
             'gemm_plain_f64': {'bound': 'mfma', 'achieved': achp,
                                'peak': 78.6, 'unit': 'TFLOP/s',
                                'frac': achp / 78.6}},
         'steps_fp64_variance': int(f64_steps.sum()),
+        'steps_int8_variance': int(i8_steps.sum()),
+        'steps_fp32_variance': int(f32_steps.sum()),
         'cpu_baseline': cpu,
         'variograms_fitted': vgfit,
     }
