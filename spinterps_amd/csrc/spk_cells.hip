@@ -1349,20 +1349,23 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const double fp64var_thr_gau =
       prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e3) : INFINITY;
   // Above the FP32 limit and up to this one the variance uses exact int8
-  // plane products (five 7-bit planes per operand, plane products up to
-  // the fourth diagonal).  Measured against the batched FP64 contraction on
-  // 768 hourly steps (500 m cells; relative, floor 1e-3 x the step's mean
-  // variance): 3e-9 below condition estimate 1e3, 1.3e-8 to 1e5, 7.8e-8 to
-  // 1e6, 3e-7 to 1e7, 6e-6 to 1e8 and 4e-5 to 1e12, where two FP64
-  // formulations already differ by far more; above 1e12 the step is
-  // flagged ill-conditioned and every formulation's variance is set by
-  // rounding.  By default every step above the FP32 limit uses the planes;
-  // FP64 remains for steps whose local fit is short of stations
+  // plane products (five 7-bit planes per operand and the plane products
+  // up to the fourth diagonal up to condition estimate 1e7, six planes and
+  // the fifth diagonal above).  Measured against the batched FP64
+  // contraction on 768 hourly steps (500 m cells; relative, floor 1e-3 x
+  // the step's mean variance): 4e-9 below condition estimate 1e3, 1.2e-8 to
+  // 1e5, 7.8e-8 to 1e6, 3e-7 to 1e7 (five planes), 1e-7 to 1e10, 4.5e-7 to
+  // 1e12 and 2.2e-6 above (six planes; above 1e12 the step is flagged
+  // ill-conditioned).  By default every step above the FP32 limit uses the
+  // planes; FP64 remains for steps whose local fit is short of stations
   // (k_nearest) and for a caller's limit.
+  // The int32 diagonal sums stay exact while 6 * n * 127^2 < 2^31, i.e.
+  // for up to 22 000 active stations; larger systems use FP64.
   const double k8 = opts->int8var_kappa;
   const double int8var_thr =
-      prec != SPK_P_MIXED ? -INFINITY
-                          : (k8 > 0.0 ? k8 : (k8 < 0.0 ? -INFINITY : INFINITY));
+      prec != SPK_P_MIXED || N > 20000
+          ? -INFINITY
+          : (k8 > 0.0 ? k8 : (k8 < 0.0 ? -INFINITY : INFINITY));
 
   // which steps have only simple variogram components (the cell kernels
   // are specialised for them)
@@ -1530,20 +1533,26 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         if (B8) {
           {
             ProfScope ps(ctx, "slice_cols");
-            hipLaunchKernelGGL((k_slice_cols<double, kInt8Planes>),
-                               dim3((unsigned)(ldk / 64), (unsigned)(KS8 / 64),
-                                    Sb),
-                               dim3(256), 0, sm, (const double *)K64, ld, ldk,
-                               cn, q.sc, (const double *)dmax, B8, KS8, eb8);
+            const dim3 gs((unsigned)(ldk / 64), (unsigned)(KS8 / 64), Sb);
+            hipLaunchKernelGGL((k_slice_cols<double, 5>), gs, dim3(256), 0, sm,
+                               (const double *)K64, ld, ldk, cn, q.sc,
+                               (const double *)dmax, B8, KS8, eb8);
+            hipLaunchKernelGGL((k_slice_cols<double, 6>), gs, dim3(256), 0, sm,
+                               (const double *)K64, ld, ldk, cn, q.sc,
+                               (const double *)dmax, B8, KS8, eb8);
           }
           {
             ProfScope ps(ctx, "gemm_tri_i8");
-            hipLaunchKernelGGL((k_tri8<kInt8Planes, 4, 2>),
-                               dim3((unsigned)ceil_div(N, 128),
-                                    (unsigned)(ldk / 64), Sb),
-                               dim3(256), 0, sm, (const int8_t *)q.L8, R8,
-                               (const int *)q.ea, (const int8_t *)B8, ldk,
-                               (const int *)eb8, KS8, q.sc, cn, qpart, nrb);
+            const dim3 g8((unsigned)ceil_div(N, 128), (unsigned)(ldk / 64),
+                          Sb);
+            hipLaunchKernelGGL((k_tri8<5, 4, 2>), g8, dim3(256), 0, sm,
+                               (const int8_t *)q.L8, R8, (const int *)q.ea,
+                               (const int8_t *)B8, ldk, (const int *)eb8, KS8,
+                               q.sc, cn, qpart, nrb);
+            hipLaunchKernelGGL((k_tri8<6, 5, 2>), g8, dim3(256), 0, sm,
+                               (const int8_t *)q.L8, R8, (const int *)q.ea,
+                               (const int8_t *)B8, ldk, (const int *)eb8, KS8,
+                               q.sc, cn, qpart, nrb);
           }
           SPK_TRY(check_launch("k_tri8"));
         }

@@ -596,7 +596,7 @@ __global__ __launch_bounds__(256) void k_dual(
       int planes = 0;
       if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr))) {
         if (kap <= int8var_thr)
-          planes = kInt8Planes;
+          planes = kap <= kInt8Deep ? 5 : 6;
         else
           fl |= SPK_ST_FP64VAR;
       }

@@ -2,7 +2,8 @@
 //
 // For steps above the FP32 limit, |L^-1 delta|^2 is formed from integer
 // products (the kernels are written for S planes and diagonals up to P;
-// the library uses S = 5, P = 4):
+// the library uses S = 5, P = 4 up to condition estimate 1e7 and S = 6,
+// P = 5 above):
 //   * each row i of L^-1 is scaled by 2^-e_i (max_k |L^-1_ik| < 2^e_i) and
 //     each cell column of delta by 2^-f_c, so every scaled entry u lies in
 //     (-1, 1);
@@ -11,7 +12,7 @@
 //     planes are the leading digits, so the first 4 of 5 planes are the
 //     4-plane cut);
 //   * Z_d = sum_{s+t=d} A_s B_t is summed exactly in int32 by
-//     mfma_i32_16x16x64_i8 (|Z_d| <= 5 * 2048 * 127^2 < 2^31), for the
+//     mfma_i32_16x16x64_i8 (|Z_d| <= 6 * 2048 * 127^2 < 2^31), for the
 //     diagonals d <= P;
 //   * y_ic = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d,ic in FP64 and the squared
 //     norms go to the same 64-row partials as the FP64 contraction.
@@ -33,8 +34,9 @@ constexpr int kPlanes = kInt8Planes;
 // scaling is exact; the digits above 2^28 and below are split in FP64 so
 // that each part converts exactly to int32.
 __device__ inline void planes_of(double x, int e, int8_t *out) {
-  const double w = ldexp(fabs(x), 7 * kPlanes - e);  // < 2^35
-  const double hi = trunc(ldexp(w, -28));            // < 2^7
+  static_assert(7 * kPlanes - 28 <= 30, "high digits fit in int32");
+  const double w = ldexp(fabs(x), 7 * kPlanes - e);  // < 2^(7 kPlanes)
+  const double hi = trunc(ldexp(w, -28));
   const int lo = (int)(w - ldexp(hi, 28));           // < 2^28
   const long long t = ((long long)(int)hi << 28) | (long long)lo;
   const int sg = x < 0.0 ? -1 : 1;
@@ -170,6 +172,9 @@ __global__ __launch_bounds__(256) void k_slice_cols(
         *reinterpret_cast<const i4_t *>(p[s]);
 }
 template __global__ void k_slice_cols<double, 5>(
+    const double *, int64_t, int64_t, int64_t, const StepSc *,
+    const double *, int8_t *, int64_t, int *);
+template __global__ void k_slice_cols<double, 6>(
     const double *, int64_t, int64_t, int64_t, const StepSc *,
     const double *, int8_t *, int64_t, int *);
 
@@ -329,6 +334,10 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   }
 }
 template __global__ void k_tri8<5, 4, 2>(const int8_t *, int64_t, const int *,
+                                         const int8_t *, int64_t, const int *,
+                                         int64_t, const StepSc *, int64_t,
+                                         double *, int);
+template __global__ void k_tri8<6, 5, 2>(const int8_t *, int64_t, const int *,
                                          const int8_t *, int64_t, const int *,
                                          int64_t, const StepSc *, int64_t,
                                          double *, int);

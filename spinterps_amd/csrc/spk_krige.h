@@ -56,9 +56,12 @@ int cell_order(spk_ctx *ctx, const double *cx, const double *cy, int64_t C,
                int32_t *perm, double *pcx, double *pcy);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
-// exact int8 plane products (spk_int8.hip): 5 planes per operand, plane
-// products up to the fourth diagonal
-constexpr int kInt8Planes = 5;
+// exact int8 plane products (spk_int8.hip): up to 6 planes per operand;
+// steps up to condition estimate kInt8Deep use 5 planes and the plane
+// products up to the fourth diagonal, steps above it 6 planes and the
+// fifth diagonal
+constexpr int kInt8Planes = 6;
+constexpr double kInt8Deep = 1e7;
 __global__ void k_slice_rows(const double *Li, int64_t ld, int64_t stride,
                              const StepSc *sc, int8_t *A8, int64_t KS,
                              int64_t R, int *ea);
