@@ -9,8 +9,13 @@
 //                       estimate / sum-of-weights / variance epilogue of
 //                       interp/steps.py:418-434 fused in
 //                TRI  : W = L^-1 K, q = |W|^2 per cell (variance term)
+//  k_tri64       the TRI contraction in FP64, 128 x 128 tiles
+//  k_nearest     each cell's nearest active station (batched variance)
 //  k_kgen<..>    fused distance -> covariance for the batched path, with
-//                the dual-form estimate k.alpha accumulated in FP64
+//                the dual-form estimate k.alpha accumulated in FP64 and the
+//                locally deflated covariances delta for the variance
+//  k_var_reduce  variance from the FP64 terms and the partial norms
+//  (the int8 plane-product contraction is in spk_int8.hip)
 //  k_idw_nnb     interp/steps.py:285-313 (IDW with per-cell max
 //                normalisation, nearest neighbour)
 //  k_special     one-station, mean-fill and failed-system steps
@@ -514,10 +519,11 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
 // the residual of local kriging, small near stations (where the variance is
 // small) and wherever J screens the cell; that keeps the FP32 contraction
 // term small next to the variance.  All other terms are FP64; abuf receives
-// c0 - (beta terms) + (a-1)(a-2)/b and delta goes to K32 or, for steps
-// flagged SPK_ST_FP64VAR (or the FP64 batched path), to K64.  Any beta gives
-// the same variance in exact arithmetic, so J is cut where C_JJ's Cholesky
-// pivot falls below 1e-8 of its diagonal.
+// c0 - (beta terms) + (a-1)(a-2)/b and delta goes to K32 (the FP32
+// contraction) or K64 (steps flagged SPK_ST_FP64VAR or SPK_ST_INT8VAR, and
+// the FP64 batched path), with each cell's max |delta| in dmax for the int8
+// planes.  Any beta gives the same variance in exact arithmetic, so J is
+// cut where C_JJ's Cholesky pivot falls below 1e-8 of its diagonal.
 // Cells come in a spatially compact order.  The nearest stations of a
 // block's 256 cells fill a table of up to kSlots stations whose G columns
 // are staged in LDS per station chunk; each cell's J is its kJ nearest
