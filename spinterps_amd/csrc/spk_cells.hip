@@ -21,6 +21,7 @@
 //  k_special     one-station, mean-fill and failed-system steps
 //                (interp/steps.py:280-283, 325-331, 372-384)
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "spk_krige.h"
@@ -533,6 +534,50 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
 // drifts are addressed at the original index perm[c].
 // kChunk: stations staged per pass of k_kgen's main loop
 constexpr int kJ = 4, kSlots = 16, kChunk = 128;
+
+// One bounded model (spherical, exponential or Gaussian) with or without a
+// nugget, the common form of fitted variograms, so that k_kgen's station
+// loop can be specialised on the model with no component loop inside.  The
+// operations are those of vg_simple_entry for these components (its sum of
+// two terms commutes exactly).
+struct VgOne {
+  double nug, s, r, c1, c2;
+};
+__device__ inline int vg_one_of(const VgFast &f, VgOne &o) {
+  int model = -1, nugs = 0;
+  o.nug = 0.0;
+  for (int k = 0; k < f.n; ++k) {
+    const int t = f.type[k];
+    if (t == SPK_VG_NUG) {
+      ++nugs;
+      o.nug = f.s[k];
+    } else if (model < 0 &&
+               (t == SPK_VG_SPH || t == SPK_VG_EXP || t == SPK_VG_GAU)) {
+      model = k;
+    } else {
+      return -1;
+    }
+  }
+  if (model < 0 || nugs > 1) return -1;
+  o.s = f.s[model];
+  o.r = f.r[model];
+  o.c1 = f.c1[model];
+  o.c2 = f.c2[model];
+  return f.type[model];
+}
+template <int MT>
+__device__ inline double vg_one(const VgOne &o, double h, double min_vg_val) {
+  double g;
+  if constexpr (MT == SPK_VG_SPH)
+    g = (h >= o.r) ? o.s : o.s * (h * o.c1 - (h * h * h) * o.c2);
+  else if constexpr (MT == SPK_VG_EXP)
+    g = o.s * (1.0 - exp(h * o.c1));
+  else
+    g = o.s * (1.0 - exp((h * h) * o.c1));
+  double acc = o.nug + g;
+  if (acc <= min_vg_val) acc = 0.0;
+  return acc;
+}
 template <bool GEN>
 __device__ inline double kg_vg(const VgFast &f, double h, int covar_flag,
                                double min_vg_val) {
@@ -789,6 +834,8 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     }
   }
   double dm = 0.0;  // max |delta| of the cell (int8 plane scaling)
+  VgOne vo;
+  const int one = vg_one_of(vf, vo);
   double best = INFINITY;
   int bi = 0;
   double e = 0.0, a = 0.0;
@@ -815,21 +862,42 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     __syncthreads();
     if (valid) {
       if (var_on) {
-        for (int j = 0; j < cnt; ++j) {
-          const double d = dist2(x, y, stx[j], sty[j]);
-          const double g = kg_vg<GEN>(vf, d, 0, min_vg_val);
-          const double k = c0 - g;
-          e += k * sal[j];
-          a += k * sw[j];
-          double delta = dconst - g;
+        // the station loop, specialised on a single bounded model
+        auto var_loop = [&](auto tag) {
+          constexpr int MT = decltype(tag)::value;
+          for (int j = 0; j < cnt; ++j) {
+            const double d = dist2(x, y, stx[j], sty[j]);
+            double g;
+            if constexpr (MT < 0)
+              g = kg_vg<GEN>(vf, d, 0, min_vg_val);
+            else
+              g = vg_one<MT>(vo, d, min_vg_val);
+            const double k = c0 - g;
+            e += k * sal[j];
+            a += k * sw[j];
+            double delta = dconst - g;
 #pragma unroll
-          for (int q = 0; q < kJ; ++q)
-            if (slt[q] >= 0) delta += sG[slt[q]][j] * beta[q];
-          if (K32s)
-            K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
-          else
-            K64s[(int64_t)(base + j) * ldk + c] = delta;
-          dm = fmax(dm, fabs(delta));
+            for (int q = 0; q < kJ; ++q)
+              if (slt[q] >= 0) delta += sG[slt[q]][j] * beta[q];
+            if (K32s)
+              K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
+            else
+              K64s[(int64_t)(base + j) * ldk + c] = delta;
+            dm = fmax(dm, fabs(delta));
+          }
+        };
+        switch (one) {
+          case SPK_VG_SPH:
+            var_loop(std::integral_constant<int, SPK_VG_SPH>{});
+            break;
+          case SPK_VG_EXP:
+            var_loop(std::integral_constant<int, SPK_VG_EXP>{});
+            break;
+          case SPK_VG_GAU:
+            var_loop(std::integral_constant<int, SPK_VG_GAU>{});
+            break;
+          default:
+            var_loop(std::integral_constant<int, -1>{});
         }
       } else {
         for (int j = 0; j < cnt; ++j) {
