@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void k_gemm(
     const int32_t *__restrict__ nnb, int64_t ldn,
     const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
     double *__restrict__ est, double *__restrict__ var, int64_t ldo,
-    int sel, double *__restrict__ qpart, int nrb) {
+    int sel, double *__restrict__ qpart, int nrb,
+    const uint8_t *__restrict__ nzb, int64_t nzk) {
   using MF = Mfma<T>;
   using V = typename MF::V;
   constexpr int BM = 64, BN = 128;
@@ -185,6 +186,8 @@ __global__ __launch_bounds__(256) void k_gemm(
   const double *z = zact + (int64_t)sl * ldz;
   __shared__ __align__(16) T As[2][BK][BM + PAD];
   __shared__ __align__(16) T Bs[2][BK][BN + PAD];
+  __shared__ int s_list[256];  // nonzero chunks (TRI with a chunk map)
+  __shared__ int s_cnt[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
 
@@ -255,14 +258,32 @@ __global__ __launch_bounds__(256) void k_gemm(
           for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
     }
     const int kend = PLAIN ? P : min(P, r0 + BM);
-    const int nch = (kend + BK - 1) / BK;
-    load_tiles(r0, 0, kend);
+    int nch = (kend + BK - 1) / BK;
+    // TRI with a map of the 16-row chunks of delta that hold a nonzero
+    // value in this cell tile: only those chunks are multiplied (a zero
+    // chunk adds exactly nothing)
+    const bool skip = !PLAIN && BK == 16 && nzb != nullptr && nch <= 256;
+    if (skip) {
+      const uint8_t *nz = nzb + ((int64_t)sl * gridDim.y + cti) * nzk;
+      const bool f = tid < nch && nz[tid] != 0;
+      const unsigned long long bm = __ballot(f);
+      if (lane == 0) s_cnt[wave] = __popcll(bm);
+      __syncthreads();
+      int off = 0;
+      for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
+      if (f) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] = tid;
+      const int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+      __syncthreads();
+      nch = tot;
+    }
+    auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
+    load_tiles(r0, nch > 0 ? kchunk(0) : 0, kend);
     __syncthreads();  // previous row block's readers are done with LDS
     store_tiles(0);
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
       const int buf = ch & 1;
-      if (ch + 1 < nch) load_tiles(r0, (ch + 1) * BK, kend);
+      if (ch + 1 < nch) load_tiles(r0, kchunk(ch + 1), kend);
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
         const int ka = kk * 4 + lg;
@@ -670,7 +691,8 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
     float *__restrict__ K32,
     double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
     double *__restrict__ abuf, double *__restrict__ est, int64_t ldo,
-    double *__restrict__ dmax) {
+    double *__restrict__ dmax, uint8_t *__restrict__ nzb, int64_t nzt,
+    int64_t nzk) {
   const int sl = blockIdx.y;
   const StepSc *stp = sc + sl;
   if (stp->status) return;
@@ -826,10 +848,7 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
       }
       double bk = 0.0;
 #pragma unroll
-      for (int q = 0; q < kJ; ++q) {
-        bk += beta[q] * kJv[q];
-        sbeta += beta[q];
-      }
+      for (int q = 0; q < kJ; ++q) bk += beta[q] * kJv[q];
       qloc = 2.0 * bk - tt;
     }
   }
@@ -840,8 +859,6 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
   int bi = 0;
   double e = 0.0, a = 0.0;
   // delta_i = k_i - sum_q (c0 - G_{i,J_q}) beta_q
-  //         = c0 (1 - sum beta) - g_i + sum_q G_{i,J_q} beta_q
-  const double dconst = c0 * (1.0 - sbeta);
   for (int base = 0; base < n; base += kChunk) {
     __syncthreads();
     if (tid < kChunk && base + tid < n) {
@@ -856,10 +873,13 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
       const int ns = n_slots;
       for (int e2 = tid; e2 < ns * kChunk; e2 += 256) {
         const int k = e2 / kChunk, u = e2 % kChunk;
-        if (u < cnt) sG[k][u] = Gs[(int64_t)slot_j[k] * ld + base + u];
+        // staged as covariances c0 - G: a station out of range of the cell
+        // and of every J station then gives delta = 0 exactly
+        if (u < cnt) sG[k][u] = c0 - Gs[(int64_t)slot_j[k] * ld + base + u];
       }
     }
     __syncthreads();
+    unsigned nzf = 0;  // 16-station chunks of this pass with delta != 0
     if (valid) {
       if (var_on) {
         // the station loop, specialised on a single bounded model
@@ -875,15 +895,16 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
             const double k = c0 - g;
             e += k * sal[j];
             a += k * sw[j];
-            double delta = dconst - g;
+            double delta = k;
 #pragma unroll
             for (int q = 0; q < kJ; ++q)
-              if (slt[q] >= 0) delta += sG[slt[q]][j] * beta[q];
+              if (slt[q] >= 0) delta -= sG[slt[q]][j] * beta[q];
             if (K32s)
               K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
             else
               K64s[(int64_t)(base + j) * ldk + c] = delta;
             dm = fmax(dm, fabs(delta));
+            if (delta != 0.0) nzf |= 1u << (j >> 4);
           }
         };
         switch (one) {
@@ -902,7 +923,7 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
       } else {
         for (int j = 0; j < cnt; ++j) {
           const double d = dist2(x, y, stx[j], sty[j]);
-          if (d < best) {
+          if (d < best || (d == best && my[base + j] < my[bi])) {
             best = d;
             bi = base + j;
           }
@@ -911,6 +932,20 @@ __global__ __launch_bounds__(256, 5) void k_kgen(
           e += k * sal[j];
           a += k * sw[j];
         }
+      }
+    }
+    // the chunk map of this wave's 64 cells (one 128-cell tile of the
+    // contraction): any lane's nonzero delta marks the chunk
+    if (var_on && nzb) {
+      unsigned m = nzf;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) m |= __shfl_xor(m, off, 64);
+      if ((tid & 63) == 0 && m) {
+        const int64_t cw = blockIdx.x * (int64_t)256 + (tid & ~63);
+        uint8_t *nz = nzb + ((int64_t)sl * nzt + (cw >> 7)) * nzk +
+                      (base >> 4);
+        for (int b2 = 0; b2 < kChunk / 16; ++b2)
+          if ((m >> b2) & 1u) nz[b2] = 1;
       }
     }
   }
@@ -1057,10 +1092,12 @@ __global__ __launch_bounds__(256) void k_special(
     }
     __syncthreads();
     const int cnt = min(256, n - base);
+    // ties go to the lower station index, whatever order the step's
+    // stations are compacted in
     if (valid)
       for (int j = 0; j < cnt; ++j) {
         const double d = dist2(x, y, stx[j], sty[j]);
-        if (d < best) {
+        if (d < best || (d == best && my[base + j] < my[bi])) {
           best = d;
           bi = base + j;
         }
@@ -1291,7 +1328,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       double *eb = est + t0 * C;
       hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm, vals + t0 * N,
                          N, idx, zact, ldz, sc, (const spk_vg *)nullptr,
-                         method, thr, mvg);
+                         method, thr, mvg, (const int32_t *)nullptr);
       {
         ProfScope ps(ctx, "idw_nnb");
         hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
@@ -1387,6 +1424,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   int8_t *B8 = nullptr;
   int *eb8 = nullptr;
   double *dmax = nullptr;
+  // map of the 16-row chunks of delta with a nonzero value, per step and
+  // 128-cell tile (mixed path): the FP32 contraction skips the others
+  uint8_t *nzb = nullptr;
+  const int64_t nzk = ld / 16;
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -1399,6 +1440,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       SPK_TRY(ws_get(ctx, "B8", S * kInt8Planes * ldk * KS8, (void **)&B8));
       SPK_TRY(ws_get(ctx, "eb8", S * ldk * sizeof(int), (void **)&eb8));
       SPK_TRY(ws_get(ctx, "dmax", S * ldk * 8, (void **)&dmax));
+      SPK_TRY(ws_get(ctx, "nzb", S * (ldk / 128) * nzk, (void **)&nzb));
     }
     SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
     SPK_TRY(ws_get(ctx, "qpart", S * nrb * ldk * 8, (void **)&qpart));
@@ -1465,6 +1507,16 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     pcy = pcx + C;
     SPK_TRY(cell_order(ctx, cx, cy, C, perm, pcx, pcy));
   }
+  // the batched path compacts each step's stations in a spatially compact
+  // order too, so that the stations near a block of cells are few runs of
+  // the triangle's rows
+  int32_t *sorder = nullptr;
+  if (batched && N >= 2) {
+    double *psx;
+    SPK_TRY(ws_get(ctx, "stn_order", N * sizeof(int32_t), (void **)&sorder));
+    SPK_TRY(ws_get(ctx, "stn_pxy", 2 * N * sizeof(double), (void **)&psx));
+    SPK_TRY(cell_order(ctx, sx, sy, N, sorder, psx, psx + N));
+  }
 
   // second stream + events for the overlap
   if (!ctx->aux_stream)
@@ -1484,7 +1536,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   auto factorise = [&](int64_t t0, int Sb, Set &q) -> int {
     const spk_vg *gb = vgs + t0;
     hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sf, vals + t0 * N,
-                       N, q.idx, q.zact, ldz, q.sc, gb, method, thr, mvg);
+                       N, q.idx, q.zact, ldz, q.sc, gb, method, thr, mvg,
+                       (const int32_t *)sorder);
     SPK_TRY(check_launch("k_compact"));
     const int tiles = (int)ceil_div(N, 64);
     {
@@ -1566,7 +1619,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                ldk, (const double *)nullptr, (int64_t)0, cn,
                                eb + s * C + cb,
                                vbo ? vbo + s * C + cb : nullptr, C, 0,
-                               (double *)nullptr, 0);
+                               (double *)nullptr, 0, (const uint8_t *)nullptr,
+                               (int64_t)0);
           }
           SPK_TRY(check_launch("k_gemm<double,plain>"));
         }
@@ -1588,6 +1642,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              cn, nrst, ldk, prec == SPK_P_MIXED ? 1 : 0);
           SPK_TRY(check_launch("k_nearest"));
         }
+        const int64_t nzt = ceil_div(cn, 128);
+        if (nzb && want_var)
+          SPK_HIP(hipMemsetAsync(nzb, 0, (size_t)Sb * nzt * nzk, sm));
         {
           ProfScope ps(ctx, "kgen");
           hipLaunchKernelGGL(simple ? k_kgen<false> : k_kgen<true>,
@@ -1600,7 +1657,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              2 * ld, ld, method, m, mvg, want_var ? 1 : 0,
                              prec == SPK_P_MIXED ? 0 : 1, skm, K32, K64, ldk,
                              q.Gm,
-                             abuf, perm ? eb : eb + cb, C, dmax);
+                             abuf, perm ? eb : eb + cb, C, dmax, nzb,
+                             nzt, nzk);
         }
         SPK_TRY(check_launch("k_kgen"));
         if (!want_var) continue;
@@ -1638,7 +1696,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              q.L32, ld, strideM, (const float *)K32, ldk,
                              ld * ldk, q.sc, p, method, 1, 0, q.zact, ldz,
                              (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
-                             cn, eb + cb, vbo + cb, C, 1, qpart, nrb);
+                             cn, eb + cb, vbo + cb, C, 1, qpart, nrb,
+                             (const uint8_t *)nzb, nzk);
         }
         {
           ProfScope ps(ctx, "gemm_tri_f64");

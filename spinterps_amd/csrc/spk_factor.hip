@@ -14,7 +14,8 @@
 namespace spk {
 
 // ---- step compaction -------------------------------------------------------
-// Active stations (non-NaN values) of each step, in station order, as
+// Active stations (non-NaN values) of each step, in station order (or in
+// the order sorder gives, a spatially compact one for the batched path), as
 // interp/grps.py:57-101 forms a step's station set; the step mean over them
 // (interp/steps.py:276) and the "any value >= min_var_thr" flag
 // (interp/steps.py:760-765); then the step status.
@@ -22,7 +23,7 @@ __global__ __launch_bounds__(256) void k_compact(
     const double *__restrict__ vals, int64_t N, int32_t *__restrict__ idx,
     double *__restrict__ zact, int64_t ldz, StepSc *__restrict__ sc,
     const spk_vg *__restrict__ vgs, int method, double thr,
-    double min_vg_val) {
+    double min_vg_val, const int32_t *__restrict__ sorder) {
   const int s = blockIdx.x;
   const double *row = vals + (int64_t)s * N;
   int32_t *my_idx = idx + (int64_t)s * N;
@@ -35,8 +36,9 @@ __global__ __launch_bounds__(256) void k_compact(
   double sum = 0.0;
   int any = 0;
   for (int64_t b = 0; b < N; b += 256) {
-    const int64_t i = b + tid;
-    const double v = (i < N) ? row[i] : NAN;
+    const int64_t i = (b + tid < N && sorder) ? (int64_t)sorder[b + tid]
+                                               : b + tid;
+    const double v = (b + tid < N) ? row[i] : NAN;
     const bool act = !isnan(v);
     const unsigned long long m = __ballot(act);
     const int pre = __popcll(m & ((1ull << lane) - 1ull));

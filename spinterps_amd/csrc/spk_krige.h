@@ -24,7 +24,7 @@ struct StepSc {
 __global__ void k_compact(const double *vals, int64_t N, int32_t *idx,
                           double *zact, int64_t ldz, StepSc *sc,
                           const spk_vg *vgs, int method, double thr,
-                          double min_vg_val);
+                          double min_vg_val, const int32_t *sorder);
 __global__ void k_assemble(const double *sx, const double *sy,
                            const int32_t *idx, int64_t N, const StepSc *sc,
                            const spk_vg *vgs, int covar_flag, double min_vg_val,

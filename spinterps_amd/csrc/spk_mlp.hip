@@ -442,7 +442,8 @@ int spk_simulate_fields_dev(spk_ctx *ctx, const spk_problem *prob, int64_t T,
     const int Sb = (int)std::min<int64_t>(S, T - t0);
     hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm,
                        mask_vals + t0 * N, N, idx, zact, ldz, sc,
-                       (const spk_vg *)nullptr, SPK_M_NNB, -INFINITY, 0.0);
+                       (const spk_vg *)nullptr, SPK_M_NNB, -INFINITY, 0.0,
+                       (const int32_t *)nullptr);
     hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles, Sb), dim3(256), 0, sm,
                        sx, sy, idx, N, sc, vgs + t0, 1, -INFINITY, A, ld,
                        strideM, (double *)nullptr);

