@@ -677,7 +677,7 @@ __global__ __launch_bounds__(256) void k_nearest(
 }
 
 template <bool GEN>
-__global__ __launch_bounds__(256, 5) void k_kgen(
+__global__ __launch_bounds__(256, 4) void k_kgen(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ zact, int64_t ldz,
