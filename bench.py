@@ -26,12 +26,13 @@ sys.path.insert(0, str(ROOT))
 
 # HBM bytes per dispatch of the variance contractions, measured by
 # rocprofv3 --pmc on this workload (batches of 64 steps, stripes of 10 752
-# cells; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with k_clip_cast).  The
-# FP64 figure is from a run in which k_tri64 contracted every step above
-# the FP32 limit (stripes of 10 880 cells).
+# cells; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with k_clip_cast); the
+# int8 figure adds its two launches (5 and 6 planes).  The FP64 figure is
+# from a run in which k_tri64 contracted every step above the FP32 limit
+# (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
-TRI32_TRAFFIC = 2 * 5998922 * 1024 + 72858 * 1024
-TRI8_TRAFFIC = 2 * 5467884 * 1024 + 24228 * 1024
+TRI32_TRAFFIC = 2 * 1804649 * 1024 + 68983 * 1024
+TRI8_TRAFFIC = 2 * (6195784 + 928303) * 1024 + (17121 + 5818) * 1024
 
 
 def log(*a):
@@ -120,8 +121,10 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     v, g = steps(W * B, (W + K) * B)
+    kap_t = torch.zeros(K * B, dtype=torch.float64, device=dev)
     _, _, info = interp_steps(prob, v, None, 'OK', precision='mixed',
-                              est=est[:K * B], var=var[:K * B], vg_dev=g)
+                              est=est[:K * B], var=var[:K * B], vg_dev=g,
+                              kappa=kap_t)
     infos = [info]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -153,9 +156,13 @@ def main():
     ms8, c8 = kern['gemm_tri_i8']
     ach32 = fl32 / (ms32 * 1e-3) / 1e12 if ms32 > 0 else 0.0
     ach64 = fl64 / (ms64 * 1e-3) / 1e12 if ms64 > 0 else 0.0
-    # int8: 15 plane products of the triangle per step (int8 ops) and the
-    # FP64-equivalent rate of the same L^-1 delta
-    ach8 = 15 * fl8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
+    # int8: 15 plane products of the triangle per step up to condition
+    # estimate 1e7, 21 above (int8 ops), and the FP64-equivalent rate of
+    # the same L^-1 delta
+    kap_np = kap_t.cpu().numpy()
+    prods = np.where(kap_np <= 1e7, 15.0, 21.0)
+    ops8 = float(np.sum((prods * nsq)[i8_steps]) * C)
+    ach8 = ops8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
     eq8 = fl8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
     # kgen: algorithmic HBM bytes = delta written (4 or 8 B) + cell coords
     kb = float(np.sum(timed_n[f32_steps]) * C * 4 +
@@ -325,7 +332,8 @@ def main():
           'unit': 'TOPS', 'frac': ach8 / 5000.0, 'traffic': TRI8_TRAFFIC,
           'fp64_equivalent_tflops': eq8,
           'kernel': 'k_tri8 (L^-1 delta, exact int8 plane products, int8 '
-                    'MFMA)', 'traffic_note': note}
+                    'MFMA; 5- and 6-plane launches together)',
+          'traffic_note': note}
     dominant = max(((ms32, r32), (ms64, r64), (ms8, r8)),
                    key=lambda x: x[0])[1]
     out = {
