@@ -106,6 +106,7 @@ SIGNATURES = {
     'spk_prof_enable': (c_int, [c_void_p, c_int]),
     'spk_prof_reset': (c_int, [c_void_p]),
     'spk_prof_get': (c_int, [c_void_p, c_char_p, _PD, _PI64]),
+    'spk_prof_work': (c_int, [c_void_p, c_char_p, _PD]),
     'spk_vgfit_pairs_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_int64,
                                     c_void_p, c_void_p, c_void_p]),
     'spk_vgfit_evg_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,

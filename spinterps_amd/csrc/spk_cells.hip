@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void k_gemm(
     const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
     double *__restrict__ est, double *__restrict__ var, int64_t ldo,
     int sel, double *__restrict__ qpart, int nrb,
-    const uint8_t *__restrict__ nzb, int64_t nzk) {
+    const uint8_t *__restrict__ nzb, int64_t nzk,
+    unsigned long long *__restrict__ nexec) {
   using MF = Mfma<T>;
   using V = typename MF::V;
   constexpr int BM = 64, BN = 128;
@@ -276,6 +277,9 @@ __global__ __launch_bounds__(256) void k_gemm(
       __syncthreads();
       nch = tot;
     }
+    // executed chunks, counted when profiling
+    if (!PLAIN && nexec && tid == 0)
+      atomicAdd(nexec, (unsigned long long)nch);
     auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
     load_tiles(r0, nch > 0 ? kchunk(0) : 0, kend);
     __syncthreads();  // previous row block's readers are done with LDS
@@ -1140,6 +1144,9 @@ struct Prof {
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<std::string, std::pair<double, int64_t>>> stats;
+  // device counters of executed work: [0] 16-row chunks multiplied by the
+  // FP32 triangular contraction
+  unsigned long long *work = nullptr;
 };
 
 static Prof &prof_of(spk_ctx *ctx) {
@@ -1216,14 +1223,42 @@ extern "C" {
 
 int spk_prof_enable(spk_ctx *ctx, int on) {
   if (!ctx) return set_error(SPK_EARG, "null ctx");
-  prof_of(ctx).on = on != 0;
+  Prof &pf = prof_of(ctx);
+  pf.on = on != 0;
+  if (pf.on && !pf.work) {
+    SPK_HIP(hipSetDevice(ctx->device));
+    SPK_TRY(ws_get(ctx, "prof_work", 64, (void **)&pf.work));
+    SPK_HIP(hipMemsetAsync(pf.work, 0, 64, ctx->stream));
+    SPK_TRY(sync(ctx));
+  }
   return SPK_OK;
 }
 
 int spk_prof_reset(spk_ctx *ctx) {
   if (!ctx) return set_error(SPK_EARG, "null ctx");
   SPK_TRY(prof_collect(ctx));
-  prof_of(ctx).stats.clear();
+  Prof &pf = prof_of(ctx);
+  pf.stats.clear();
+  if (pf.work) {
+    SPK_HIP(hipMemsetAsync(pf.work, 0, 64, ctx->stream));
+    SPK_TRY(sync(ctx));
+  }
+  return SPK_OK;
+}
+
+int spk_prof_work(spk_ctx *ctx, const char *name, double *value) {
+  if (!ctx || !name || !value) return set_error(SPK_EARG, "null argument");
+  Prof &pf = prof_of(ctx);
+  *value = 0.0;
+  if (std::string(name) != "gemm_tri_f32_macs")
+    return set_error(SPK_EARG, "unknown work counter %s", name);
+  if (!pf.work) return SPK_OK;
+  SPK_TRY(prof_collect(ctx));
+  unsigned long long chunks = 0;
+  SPK_TRY(d2h(ctx, &chunks, pf.work, sizeof(chunks)));
+  SPK_TRY(sync(ctx));
+  // 16 k x 64 rows x 128 cells multiply-adds per chunk
+  *value = (double)chunks * 16.0 * 64.0 * 128.0;
   return SPK_OK;
 }
 
@@ -1620,7 +1655,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                eb + s * C + cb,
                                vbo ? vbo + s * C + cb : nullptr, C, 0,
                                (double *)nullptr, 0, (const uint8_t *)nullptr,
-                               (int64_t)0);
+                               (int64_t)0, (unsigned long long *)nullptr);
           }
           SPK_TRY(check_launch("k_gemm<double,plain>"));
         }
@@ -1697,7 +1732,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              ld * ldk, q.sc, p, method, 1, 0, q.zact, ldz,
                              (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
                              cn, eb + cb, vbo + cb, C, 1, qpart, nrb,
-                             (const uint8_t *)nzb, nzk);
+                             (const uint8_t *)nzb, nzk,
+                             prof_of(ctx).on ? prof_of(ctx).work : nullptr);
         }
         {
           ProfScope ps(ctx, "gemm_tri_f64");

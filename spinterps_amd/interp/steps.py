@@ -212,6 +212,15 @@ def prof_get(name, device=0):
     return ms.value, cnt.value
 
 
+def prof_work(name, device=0):
+    """Executed work counted while profiling is on (e.g.
+    'gemm_tri_f32_macs')."""
+    ctx = context(device)
+    v = ctypes.c_double()
+    check(lib().spk_prof_work(ctx.h, name.encode(), ctypes.byref(v)))
+    return v.value
+
+
 def interpolate(prob: DeviceProblem, vals, vgs, methods, *,
                 min_var_thr=-np.inf, min_var_cut=None, max_var_cut=None,
                 min_vg_val=0.0, est_vars_flag=True, precision='fp64'):
