@@ -319,8 +319,10 @@ int spk_prof_reset(spk_ctx *ctx);
 int spk_prof_get(spk_ctx *ctx, const char *name, double *total_ms,
                  int64_t *count);
 /* Executed work counted while profiling is on: "gemm_tri_f32_macs", the
- * multiply-adds the FP32 triangular contraction executed (it skips the
- * zero chunks of delta, so this is below the algorithmic count). */
+ * multiply-adds the FP32 triangular contraction executed, and
+ * "gemm_tri_i8_ops", the int8 operations of the plane products the int8
+ * contraction executed (both skip the zero chunks of delta, so these are
+ * below the algorithmic counts). */
 int spk_prof_work(spk_ctx *ctx, const char *name, double *value);
 
 #ifdef __cplusplus

@@ -1145,7 +1145,8 @@ struct Prof {
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<std::string, std::pair<double, int64_t>>> stats;
   // device counters of executed work: [0] 16-row chunks multiplied by the
-  // FP32 triangular contraction
+  // FP32 triangular contraction, [1] 64-k chunks times plane products of
+  // the int8 contraction
   unsigned long long *work = nullptr;
 };
 
@@ -1250,15 +1251,24 @@ int spk_prof_work(spk_ctx *ctx, const char *name, double *value) {
   if (!ctx || !name || !value) return set_error(SPK_EARG, "null argument");
   Prof &pf = prof_of(ctx);
   *value = 0.0;
-  if (std::string(name) != "gemm_tri_f32_macs")
+  const std::string nm(name);
+  int slot;
+  double per;
+  if (nm == "gemm_tri_f32_macs") {
+    slot = 0;  // 16 k x 64 rows x 128 cells multiply-adds per chunk
+    per = 16.0 * 64.0 * 128.0;
+  } else if (nm == "gemm_tri_i8_ops") {
+    slot = 1;  // 2 x 64 k x 128 rows x 64 cells ops per chunk and product
+    per = 2.0 * 64.0 * 128.0 * 64.0;
+  } else {
     return set_error(SPK_EARG, "unknown work counter %s", name);
+  }
   if (!pf.work) return SPK_OK;
   SPK_TRY(prof_collect(ctx));
-  unsigned long long chunks = 0;
-  SPK_TRY(d2h(ctx, &chunks, pf.work, sizeof(chunks)));
+  unsigned long long cnt = 0;
+  SPK_TRY(d2h(ctx, &cnt, pf.work + slot, sizeof(cnt)));
   SPK_TRY(sync(ctx));
-  // 16 k x 64 rows x 128 cells multiply-adds per chunk
-  *value = (double)chunks * 16.0 * 64.0 * 128.0;
+  *value = (double)cnt * per;
   return SPK_OK;
 }
 
@@ -1703,23 +1713,29 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
             const dim3 gs((unsigned)(ldk / 64), (unsigned)(KS8 / 64), Sb);
             hipLaunchKernelGGL((k_slice_cols<double, 5>), gs, dim3(256), 0, sm,
                                (const double *)K64, ld, ldk, cn, q.sc,
-                               (const double *)dmax, B8, KS8, eb8);
+                               (const double *)dmax, B8, KS8, eb8,
+                               (const uint8_t *)nzb, nzk);
             hipLaunchKernelGGL((k_slice_cols<double, 6>), gs, dim3(256), 0, sm,
                                (const double *)K64, ld, ldk, cn, q.sc,
-                               (const double *)dmax, B8, KS8, eb8);
+                               (const double *)dmax, B8, KS8, eb8,
+                               (const uint8_t *)nzb, nzk);
           }
           {
             ProfScope ps(ctx, "gemm_tri_i8");
             const dim3 g8((unsigned)ceil_div(N, 128), (unsigned)(ldk / 64),
                           Sb);
+            unsigned long long *w8 =
+                prof_of(ctx).on ? prof_of(ctx).work + 1 : nullptr;
             hipLaunchKernelGGL((k_tri8<5, 4, 2>), g8, dim3(256), 0, sm,
                                (const int8_t *)q.L8, R8, (const int *)q.ea,
                                (const int8_t *)B8, ldk, (const int *)eb8, KS8,
-                               q.sc, cn, qpart, nrb);
+                               q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
+                               w8);
             hipLaunchKernelGGL((k_tri8<6, 5, 2>), g8, dim3(256), 0, sm,
                                (const int8_t *)q.L8, R8, (const int *)q.ea,
                                (const int8_t *)B8, ldk, (const int *)eb8, KS8,
-                               q.sc, cn, qpart, nrb);
+                               q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
+                               w8);
           }
           SPK_TRY(check_launch("k_tri8"));
         }

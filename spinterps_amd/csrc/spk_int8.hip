@@ -57,6 +57,23 @@ __device__ inline bool int8_step(const StepSc &st) {
          !(st.flags & SPK_ST_FP64VAR);
 }
 
+// Whether the 64 k from k0 (a multiple of 64) hold a nonzero delta for the
+// 128-cell tile of cell c0 of step sl: nzb[step][tile][16-row chunk], tiles
+// ceil(Cs / 128) per step and nzk chunks per tile, a nonzero byte marking a
+// chunk that k_kgen found a nonzero delta in.
+__device__ inline bool chunk_nonzero(const uint8_t *nzb, int sl, int64_t Cs,
+                                     int64_t nzk, int64_t c0, int k0) {
+  const int64_t nzt = (Cs + 127) >> 7;
+  const uint8_t *nz = nzb + ((int64_t)sl * nzt + (c0 >> 7)) * nzk;
+  bool f = false;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int64_t i = (k0 >> 4) + b;
+    f = f || (i < nzk && nz[i] != 0);
+  }
+  return f;
+}
+
 // L^-1 of each int8 step (FP64, column-major lower triangle, ld) -> planes
 // A8[step][s][R][KS] (row i, k contiguous; zero above the diagonal and
 // beyond n) and row exponents ea[step][R].  Grid (R / 64, steps), 256
@@ -124,12 +141,15 @@ __global__ __launch_bounds__(256) void k_slice_rows(
 // eb[step][ldk] from the per-cell maxima dmax[step][ldk] that k_kgen
 // leaves (for FP32 input raised by 2^-22 so that the rounded values stay
 // below 2^e).  Grid (ldk / 64, KS / 64, steps), 256 threads: 64 cells x 64
-// k per block, transposed through LDS.
+// k per block, transposed through LDS.  With a chunk map (nzb, see
+// chunk_nonzero) a block whose 64 k hold no nonzero delta in its 128-cell
+// tile writes no planes: k_tri8 skips the same chunks.
 template <typename T, int S>
 __global__ __launch_bounds__(256) void k_slice_cols(
     const T *__restrict__ K, int64_t ld, int64_t ldk, int64_t Cs,
     const StepSc *__restrict__ sc, const double *__restrict__ dmax,
-    int8_t *__restrict__ B8, int64_t KS, int *__restrict__ eb) {
+    int8_t *__restrict__ B8, int64_t KS, int *__restrict__ eb,
+    const uint8_t *__restrict__ nzb, int64_t nzk) {
   const int sl = blockIdx.z;
   const StepSc st = sc[sl];
   if (!int8_step(st) || st.planes != S) return;
@@ -148,6 +168,7 @@ __global__ __launch_bounds__(256) void k_slice_cols(
     cexp[tid] = e;
     if (blockIdx.y == 0) eb[(int64_t)sl * ldk + c] = e;
   }
+  if (nzb && !chunk_nonzero(nzb, sl, Cs, nzk, c0, k0)) return;
   for (int e = tid; e < 64 * 64; e += 256) {
     const int kk = e >> 6, cc = e & 63;
     const int k = k0 + kk;
@@ -173,10 +194,10 @@ __global__ __launch_bounds__(256) void k_slice_cols(
 }
 template __global__ void k_slice_cols<double, 5>(
     const double *, int64_t, int64_t, int64_t, const StepSc *,
-    const double *, int8_t *, int64_t, int *);
+    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t);
 template __global__ void k_slice_cols<double, 6>(
     const double *, int64_t, int64_t, int64_t, const StepSc *,
-    const double *, int8_t *, int64_t, int *);
+    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t);
 
 // q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the first S
 // planes, diagonals s + t <= P.  Block tile 128 rows x 32 TJ cells, k in
@@ -186,14 +207,18 @@ template __global__ void k_slice_cols<double, 6>(
 // planes are loaded into registers while the previous chunk is multiplied
 // out of LDS (rows padded to 80 bytes so the 16-byte reads of 16 rows hit
 // distinct banks).  XCD remap and the 64-row partial layout are those of
-// k_tri64.
+// k_tri64.  With a chunk map (nzb) only the 64-k chunks that hold a nonzero
+// delta in the block's 128-cell tile are multiplied (the others add
+// exactly nothing); nexec, when given, counts the plane products executed.
 template <int S, int P, int TJ>
 __global__ __launch_bounds__(256, 1) void k_tri8(
     const int8_t *__restrict__ A8, int64_t R, const int *__restrict__ ea,
     const int8_t *__restrict__ B8, int64_t ldk, const int *__restrict__ eb,
     int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
-    double *__restrict__ qpart, int nrb) {
+    double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nzb,
+    int64_t nzk, unsigned long long *nexec) {
   constexpr int BM = 128, BN = 32 * TJ, BK = 64, RS = 80;
+  static_assert(BN <= 128 && 128 % BN == 0, "cell tile inside a map tile");
   constexpr int NB = BN / 64;  // 16-byte B loads per plane and thread
   static_assert(S <= kPlanes && P <= 2 * (S - 1) && NB >= 1, "shape");
   const int G = gridDim.x * gridDim.y;
@@ -213,6 +238,8 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   const int8_t *B = B8 + (int64_t)sl * kPlanes * ldk * KS;
   __shared__ __align__(16) int8_t As[S][BM * RS];
   __shared__ __align__(16) int8_t Bs[S][BN * RS];
+  __shared__ int s_list[256];  // nonzero 64-k chunks (with a chunk map)
+  __shared__ int s_cnt[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
@@ -259,12 +286,33 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < TJ; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
-  const int nch = (kend + BK - 1) / BK;
-  load(0);
-  store();
+  int nch = (kend + BK - 1) / BK;
+  const bool skip = nzb != nullptr && nch <= 256;
+  if (skip) {
+    const bool f = tid < nch && chunk_nonzero(nzb, sl, Cs, nzk, c0, tid * BK);
+    const unsigned long long bm = __ballot(f);
+    if (lane == 0) s_cnt[wave] = __popcll(bm);
+    __syncthreads();
+    int off = 0;
+    for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
+    if (f) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] = tid;
+    nch = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    __syncthreads();
+  }
+  if (nexec && tid == 0) {
+    int prods = 0;
+    for (int s = 0; s < S; ++s)
+      for (int t = 0; t < S; ++t) prods += s + t <= P;
+    atomicAdd(nexec, (unsigned long long)nch * prods);
+  }
+  auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
+  if (nch > 0) {
+    load(kchunk(0));
+    store();
+  }
   __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
-    if (ch + 1 < nch) load((ch + 1) * BK);
+    if (ch + 1 < nch) load(kchunk(ch + 1));
     i4_t af[S][4];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -336,10 +384,12 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
 template __global__ void k_tri8<5, 4, 2>(const int8_t *, int64_t, const int *,
                                          const int8_t *, int64_t, const int *,
                                          int64_t, const StepSc *, int64_t,
-                                         double *, int);
+                                         double *, int, const uint8_t *,
+                                         int64_t, unsigned long long *);
 template __global__ void k_tri8<6, 5, 2>(const int8_t *, int64_t, const int *,
                                          const int8_t *, int64_t, const int *,
                                          int64_t, const StepSc *, int64_t,
-                                         double *, int);
+                                         double *, int, const uint8_t *,
+                                         int64_t, unsigned long long *);
 
 }  // namespace spk

@@ -213,8 +213,8 @@ def prof_get(name, device=0):
 
 
 def prof_work(name, device=0):
-    """Executed work counted while profiling is on (e.g.
-    'gemm_tri_f32_macs')."""
+    """Executed work counted while profiling is on ('gemm_tri_f32_macs',
+    'gemm_tri_i8_ops')."""
     ctx = context(device)
     v = ctypes.c_double()
     check(lib().spk_prof_work(ctx.h, name.encode(), ctypes.byref(v)))

@@ -414,14 +414,18 @@ def test_int8_plane_contraction_matches_fp64(gpu):
     maxima)."""
     import torch
     from spinterps_amd import _lib
-    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    from spinterps_amd.interp.steps import (DeviceProblem, interp_steps,
+                                            prof_enable, prof_work)
     w = configs.hourly(n_side=48, cell=500.0)
     T = 40
     vals, vgs = configs.hourly_steps(w, 0, T)
     prob = DeviceProblem.from_workload(w)
     kap = torch.zeros(T, dtype=torch.float64, device='cuda')
+    prof_enable(0, True)
     _, v8, inf8 = interp_steps(prob, vals, vgs, 'OK', precision='mixed',
                                kappa=kap, fp64var_kappa=1e-300)
+    ops = prof_work('gemm_tri_i8_ops')
+    prof_enable(0, False)
     _, v64, i64 = interp_steps(prob, vals, vgs, 'OK', precision='fp64b')
     k = kap.cpu().numpy()
     inf8, i64 = inf8.cpu().numpy(), i64.cpu().numpy()
@@ -429,6 +433,19 @@ def test_int8_plane_contraction_matches_fp64(gpu):
     int8 = ((inf8 & _lib.ST_INT8VAR) != 0) & ((inf8 & _lib.ST_FP64VAR) == 0)
     sel = solved & int8
     assert sel.sum() >= 20, (k, inf8)
+    # the cells cover 24 km x 24 km of the 500 km station field, so most
+    # 64-station chunks of delta are exactly zero and the contraction skips
+    # them: its executed plane products stay well below those of every
+    # chunk of the triangle (128-row x 64-cell tiles, 64-k chunks)
+    C = len(w.cell_x)
+    full = 0.0
+    for t in np.nonzero(sel)[0]:
+        n = int(np.sum(~np.isnan(vals[t])))
+        prods = 15 if k[t] <= 1e7 else 21
+        nch = sum(-(-min(n, r0 + 128) // 64) for r0 in range(0, n, 128))
+        full += nch * -(-C // 64) * prods * 2.0 * 64 * 128 * 64
+    print(f'int8 ops executed {ops:.3e} of {full:.3e} without skipping')
+    assert 0 < ops < 0.8 * full
     v8, v64 = v8.cpu().numpy(), v64.cpu().numpy()
     floor = 1e-3 * np.abs(v64).mean(axis=1, keepdims=True)
     with np.errstate(invalid='ignore'):
