@@ -564,7 +564,8 @@ constexpr int kJ = 4, kSlots = 16, kChunk = 128;
 // nugget, the common form of fitted variograms, so that k_kgen's station
 // loop can be specialised on the model with no component loop inside.  The
 // operations are those of vg_simple_entry for these components (its sum of
-// two terms commutes exactly).
+// two terms commutes exactly), and neither contracts a product and a sum
+// into one rounding, so both give the same values.
 struct VgOne {
   double nug, s, r, c1, c2;
 };
@@ -592,6 +593,7 @@ __device__ inline int vg_one_of(const VgFast &f, VgOne &o) {
 }
 template <int MT>
 __device__ inline double vg_one(const VgOne &o, double h, double min_vg_val) {
+#pragma clang fp contract(off)
   double g;
   if constexpr (MT == SPK_VG_SPH)
     g = (h >= o.r) ? o.s : o.s * (h * o.c1 - (h * h * h) * o.c2);
@@ -696,7 +698,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
     double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
     double *__restrict__ abuf, double *__restrict__ est, int64_t ldo,
     double *__restrict__ dmax, uint8_t *__restrict__ nzb, int64_t nzt,
-    int64_t nzk) {
+    int64_t nzk, int zskip) {
   const int sl = blockIdx.y;
   const StepSc *stp = sc + sl;
   if (stp->status) return;
@@ -859,6 +861,52 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   double dm = 0.0;  // max |delta| of the cell (int8 plane scaling)
   VgOne vo;
   const int one = vg_one_of(vf, vo);
+  // Culling.  For a single bounded model whose far value nug + s equals c0
+  // (and is above min_vg_val), k = c0 - gamma(h) is exactly 0 beyond a
+  // distance: h >= r (spherical), h c1 <= -42 (exponential) or h^2 c1 <= -42
+  // (Gaussian), where exp() < 2^-54 so that 1 - exp() rounds to 1.  A
+  // sub-chunk of 16 stations whose bounding box is that far from every cell
+  // of the block (the gap measured by dist2 itself, so that by monotone
+  // rounding no cell-station distance the loop computes is below it), and
+  // whose staged c0 - G entries against the block's slot stations are all
+  // 0, gives k = 0 and delta = 0 exactly for every cell: it adds nothing to
+  // the estimate, its delta chunk stays unmarked in the chunk map, and it is
+  // not evaluated.  Its delta is not written either where every consumer
+  // of this step's delta skips the unmarked chunks (zskip, and not an FP64
+  // step); otherwise zeros are written.
+  __shared__ double s_bb[4][4];  // per wave, then the block's in row 0
+  __shared__ int s_cull[kChunk / 16];
+  const bool cull_on = var_on && one >= 0 && c0 == vo.nug + vo.s &&
+                       vo.nug + vo.s > min_vg_val;
+  const bool zwrite = !(zskip && !all64 && !(stp->flags & SPK_ST_FP64VAR));
+  if (cull_on) {
+    double v0 = valid ? x : INFINITY, v1 = valid ? x : -INFINITY;
+    double v2 = valid ? y : INFINITY, v3 = valid ? y : -INFINITY;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      v0 = fmin(v0, __shfl_xor(v0, off, 64));
+      v1 = fmax(v1, __shfl_xor(v1, off, 64));
+      v2 = fmin(v2, __shfl_xor(v2, off, 64));
+      v3 = fmax(v3, __shfl_xor(v3, off, 64));
+    }
+    if ((tid & 63) == 0) {
+      s_bb[tid >> 6][0] = v0;
+      s_bb[tid >> 6][1] = v1;
+      s_bb[tid >> 6][2] = v2;
+      s_bb[tid >> 6][3] = v3;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+      for (int w2 = 1; w2 < 4; ++w2) {
+        s_bb[0][0] = fmin(s_bb[0][0], s_bb[w2][0]);
+        s_bb[0][1] = fmax(s_bb[0][1], s_bb[w2][1]);
+        s_bb[0][2] = fmin(s_bb[0][2], s_bb[w2][2]);
+        s_bb[0][3] = fmax(s_bb[0][3], s_bb[w2][3]);
+      }
+    }
+    // read after the station loop's first barrier
+  }
   double best = INFINITY;
   int bi = 0;
   double e = 0.0, a = 0.0;
@@ -883,32 +931,82 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
       }
     }
     __syncthreads();
+    if (cull_on) {
+      if (tid < kChunk) {
+        const bool have = tid < cnt;
+        double u0 = have ? stx[tid] : INFINITY;
+        double u1 = have ? stx[tid] : -INFINITY;
+        double u2 = have ? sty[tid] : INFINITY;
+        double u3 = have ? sty[tid] : -INFINITY;
+        int z = 1;
+        if (have)
+          for (int k = 0; k < n_slots; ++k) z &= sG[k][tid] == 0.0 ? 1 : 0;
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) {
+          u0 = fmin(u0, __shfl_xor(u0, off, 64));
+          u1 = fmax(u1, __shfl_xor(u1, off, 64));
+          u2 = fmin(u2, __shfl_xor(u2, off, 64));
+          u3 = fmax(u3, __shfl_xor(u3, off, 64));
+          z &= __shfl_xor(z, off, 64);
+        }
+        if ((tid & 15) == 0) {
+          const double gx =
+              fmax(0.0, fmax(u0 - s_bb[0][1], s_bb[0][0] - u1));
+          const double gy =
+              fmax(0.0, fmax(u2 - s_bb[0][3], s_bb[0][2] - u3));
+          const double D = dist2(gx, gy, 0.0, 0.0);
+          bool far;
+          if (one == SPK_VG_SPH)
+            far = D >= vo.r;
+          else if (one == SPK_VG_EXP)
+            far = D * vo.c1 <= -42.0;
+          else
+            far = (D * D) * vo.c1 <= -42.0;
+          s_cull[tid >> 4] = (z && far) ? 1 : 0;
+        }
+      }
+      __syncthreads();
+    }
     unsigned nzf = 0;  // 16-station chunks of this pass with delta != 0
     if (valid) {
       if (var_on) {
-        // the station loop, specialised on a single bounded model
+        // the station loop, specialised on a single bounded model, over
+        // sub-chunks of 16 stations
         auto var_loop = [&](auto tag) {
           constexpr int MT = decltype(tag)::value;
-          for (int j = 0; j < cnt; ++j) {
-            const double d = dist2(x, y, stx[j], sty[j]);
-            double g;
-            if constexpr (MT < 0)
-              g = kg_vg<GEN>(vf, d, 0, min_vg_val);
-            else
-              g = vg_one<MT>(vo, d, min_vg_val);
-            const double k = c0 - g;
-            e += k * sal[j];
-            a += k * sw[j];
-            double delta = k;
+          for (int j0 = 0; j0 < cnt; j0 += 16) {
+            const int j1 = min(cnt, j0 + 16);
+            if (MT >= 0 && cull_on && s_cull[j0 >> 4]) {
+              if (zwrite)
+                for (int j = j0; j < j1; ++j) {
+                  if (K32s)
+                    K32s[(int64_t)(base + j) * ldk + c] = 0.0f;
+                  else
+                    K64s[(int64_t)(base + j) * ldk + c] = 0.0;
+                }
+              continue;
+            }
+            for (int j = j0; j < j1; ++j) {
+              const double d = dist2(x, y, stx[j], sty[j]);
+              double g;
+              if constexpr (MT < 0)
+                g = kg_vg<GEN>(vf, d, 0, min_vg_val);
+              else
+                g = vg_one<MT>(vo, d, min_vg_val);
+              const double k = c0 - g;
+              e += k * sal[j];
+              a += k * sw[j];
+              double delta = k;
 #pragma unroll
-            for (int q = 0; q < kJ; ++q)
-              if (slt[q] >= 0) delta -= sG[slt[q]][j] * beta[q];
-            if (K32s)
-              K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
-            else
-              K64s[(int64_t)(base + j) * ldk + c] = delta;
-            dm = fmax(dm, fabs(delta));
-            if (delta != 0.0) nzf |= 1u << (j >> 4);
+              for (int q = 0; q < kJ; ++q)
+                if (slt[q] >= 0) delta -= sG[slt[q]][j] * beta[q];
+              if (K32s)
+                K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
+              else
+                K64s[(int64_t)(base + j) * ldk + c] = delta;
+              dm = fmax(dm, fabs(delta));
+              if (delta != 0.0) nzf |= 1u << (j >> 4);
+            }
           }
         };
         switch (one) {
@@ -1473,6 +1571,11 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // 128-cell tile (mixed path): the FP32 contraction skips the others
   uint8_t *nzb = nullptr;
   const int64_t nzk = ld / 16;
+  // every consumer of the mixed path's delta (FP32 and int8 contractions,
+  // int8 slicing) skips the unmarked chunks when the FP32 contraction's
+  // chunk list (256 chunks of 16 rows) covers the triangle: kgen then
+  // leaves culled chunks unwritten
+  const int zskip = prec == SPK_P_MIXED && ceil_div(N, 16) <= 256 ? 1 : 0;
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -1703,7 +1806,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              prec == SPK_P_MIXED ? 0 : 1, skm, K32, K64, ldk,
                              q.Gm,
                              abuf, perm ? eb : eb + cb, C, dmax, nzb,
-                             nzt, nzk);
+                             nzt, nzk, zskip);
         }
         SPK_TRY(check_launch("k_kgen"));
         if (!want_var) continue;

@@ -27,6 +27,9 @@ typedef int i4_t __attribute__((ext_vector_type(4)));
 
 // planes stored per operand (the 5-plane cut)
 constexpr int kPlanes = kInt8Planes;
+// 64-k chunks up to which the int8 kernels skip the zero chunks of delta
+// (the int8 steps have at most 20 000 stations, 313 chunks)
+constexpr int kMaxChunks8 = 512;
 
 // Cut x / 2^e (|x| < 2^e) into kPlanes 7-bit planes: the base-128 digits
 // of trunc(|x| 2^(7 kPlanes - e)), most significant first, with the sign of
@@ -142,8 +145,10 @@ __global__ __launch_bounds__(256) void k_slice_rows(
 // leaves (for FP32 input raised by 2^-22 so that the rounded values stay
 // below 2^e).  Grid (ldk / 64, KS / 64, steps), 256 threads: 64 cells x 64
 // k per block, transposed through LDS.  With a chunk map (nzb, see
-// chunk_nonzero) a block whose 64 k hold no nonzero delta in its 128-cell
-// tile writes no planes: k_tri8 skips the same chunks.
+// chunk_nonzero) and at most kMaxChunks8 chunks of 64 k, a block whose 64 k
+// hold no nonzero delta in its 128-cell tile writes no planes: k_tri8
+// skips the same chunks.  Within a block, the 16-row chunks the map leaves
+// unmarked are taken as zeros without reading delta.
 template <typename T, int S>
 __global__ __launch_bounds__(256) void k_slice_cols(
     const T *__restrict__ K, int64_t ld, int64_t ldk, int64_t Cs,
@@ -168,12 +173,29 @@ __global__ __launch_bounds__(256) void k_slice_cols(
     cexp[tid] = e;
     if (blockIdx.y == 0) eb[(int64_t)sl * ldk + c] = e;
   }
-  if (nzb && !chunk_nonzero(nzb, sl, Cs, nzk, c0, k0)) return;
+  if (nzb && (KS >> 6) <= kMaxChunks8 &&
+      !chunk_nonzero(nzb, sl, Cs, nzk, c0, k0))
+    return;
+  // the 16-row chunks the map leaves unmarked hold delta = 0 for every cell
+  // of the tile (k_kgen may leave them unwritten): zeros, not reads
+  unsigned live = 15u;
+  if (nzb) {
+    const uint8_t *nz =
+        nzb + ((int64_t)sl * ((Cs + 127) >> 7) + (c0 >> 7)) * nzk;
+    live = 0u;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t i = (k0 >> 4) + b;
+      if (i < nzk && nz[i] != 0) live |= 1u << b;
+    }
+  }
   for (int e = tid; e < 64 * 64; e += 256) {
     const int kk = e >> 6, cc = e & 63;
     const int k = k0 + kk;
     const int64_t c = c0 + cc;
-    tile[kk][cc] = (k < n && c < Cs) ? (double)D[(int64_t)k * ldk + c] : 0.0;
+    tile[kk][cc] = (k < n && c < Cs && ((live >> (kk >> 4)) & 1u))
+                       ? (double)D[(int64_t)k * ldk + c]
+                       : 0.0;
   }
   __syncthreads();
   const int wc = tid >> 2, wseg = tid & 3;
@@ -238,7 +260,7 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   const int8_t *B = B8 + (int64_t)sl * kPlanes * ldk * KS;
   __shared__ __align__(16) int8_t As[S][BM * RS];
   __shared__ __align__(16) int8_t Bs[S][BN * RS];
-  __shared__ int s_list[256];  // nonzero 64-k chunks (with a chunk map)
+  __shared__ int s_list[kMaxChunks8];  // nonzero 64-k chunks (with a map)
   __shared__ int s_cnt[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
@@ -287,17 +309,23 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
 #pragma unroll
       for (int b = 0; b < TJ; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
   int nch = (kend + BK - 1) / BK;
-  const bool skip = nzb != nullptr && nch <= 256;
+  // the same condition as k_slice_cols' (nch <= KS / 64)
+  const bool skip = nzb != nullptr && (KS >> 6) <= kMaxChunks8;
   if (skip) {
-    const bool f = tid < nch && chunk_nonzero(nzb, sl, Cs, nzk, c0, tid * BK);
-    const unsigned long long bm = __ballot(f);
-    if (lane == 0) s_cnt[wave] = __popcll(bm);
-    __syncthreads();
-    int off = 0;
-    for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
-    if (f) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] = tid;
-    nch = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    __syncthreads();
+    int tot = 0;
+    for (int b0 = 0; b0 < nch; b0 += 256) {
+      const int ch = b0 + tid;
+      const bool f = ch < nch && chunk_nonzero(nzb, sl, Cs, nzk, c0, ch * BK);
+      const unsigned long long bm = __ballot(f);
+      if (lane == 0) s_cnt[wave] = __popcll(bm);
+      __syncthreads();
+      int off = tot;
+      for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
+      if (f) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] = ch;
+      tot += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+      __syncthreads();
+    }
+    nch = tot;
   }
   if (nexec && tid == 0) {
     int prods = 0;

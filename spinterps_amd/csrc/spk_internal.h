@@ -179,6 +179,7 @@ __host__ __device__ inline void vg_fast_init(const spk_vg &vg, VgFast &f) {
 
 __device__ inline double vg_fast_entry(const VgFast &f, double h,
                                        int covar_flag, double min_vg_val) {
+#pragma clang fp contract(off)
   double acc = 0.0;
   for (int k = 0; k < f.n; ++k) {
     const double s = f.s[k];
@@ -215,6 +216,7 @@ __device__ inline double vg_fast_entry(const VgFast &f, double h,
 // out, which keeps register use low in the cell kernels.
 __device__ inline double vg_simple_entry(const VgFast &f, double h,
                                          int covar_flag, double min_vg_val) {
+#pragma clang fp contract(off)
   double acc = 0.0;
   for (int k = 0; k < f.n; ++k) {
     const double s = f.s[k];

@@ -404,6 +404,43 @@ def test_mixed_slot_overflow_promotes(gpu):
     assert np.max(np.abs(v - v64) / np.maximum(np.abs(v64), floor)) < 1e-9
 
 
+@pytest.mark.parametrize('model', ['Sph', 'Exp', 'Gau'])
+def test_mixed_culled_station_chunks_bit_exact(gpu, model):
+    """kgen culls the 16-station sub-chunks that are beyond the model's
+    exact-zero distance from a block's cells: their covariances and delta are
+    exactly 0, so the culled path gives bit-identical estimates and
+    variances.  The reference run appends a zero-sill component of the same
+    family ('+ 0.0 M(1.0)'), which leaves every value unchanged but takes
+    the general evaluation, which culls nothing.  Between the two runs a
+    long-range exponential call fills the delta workspace with nonzero
+    values, so a consumer that read a chunk the culled run left unwritten
+    would differ."""
+    import torch
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    # 96 x 96 cells at 500 m in one corner of the 500 km station field
+    w = configs.hourly(n_side=96, cell=500.0)
+    T = 12
+    vals, _ = configs.hourly_steps(w, 0, T)
+    rng = np.random.default_rng(31)
+    vgs, vgs0, vgs_far = [], [], []
+    for t in range(T):
+        rg = float(rng.uniform(5e3, 8e4))
+        f = float(rng.uniform(0.0, 0.3))
+        v = float(np.nanvar(vals[t]))
+        base = f'{f * v!r} Nug(0.0) + {(1 - f) * v!r} {model}({rg!r})'
+        vgs.append(base)
+        vgs0.append(base + f' + 0.0 {model}(1.0)')
+        vgs_far.append(f'{v!r} Exp(1000000.0)')
+    prob = DeviceProblem.from_workload(w)
+    e0, v0, i0 = interp_steps(prob, vals, vgs0, 'OK', precision='mixed')
+    interp_steps(prob, vals, vgs_far, 'OK', precision='mixed')
+    e1, v1, i1 = interp_steps(prob, vals, vgs, 'OK', precision='mixed')
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(i1.cpu().numpy(), i0.cpu().numpy())
+    np.testing.assert_array_equal(e1.cpu().numpy(), e0.cpu().numpy())
+    np.testing.assert_array_equal(v1.cpu().numpy(), v0.cpu().numpy())
+
+
 def test_int8_plane_contraction_matches_fp64(gpu):
     """Exact int8 plane products (every step forced above the FP32 limit)
     against the batched FP64 path, whose contraction is the same
