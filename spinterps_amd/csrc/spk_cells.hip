@@ -1829,12 +1829,12 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                           Sb);
             unsigned long long *w8 =
                 prof_of(ctx).on ? prof_of(ctx).work + 1 : nullptr;
-            hipLaunchKernelGGL((k_tri8<5, 4, 2>), g8, dim3(256), 0, sm,
+            hipLaunchKernelGGL((k_tri8<5, 4>), g8, dim3(512), 0, sm,
                                (const int8_t *)q.L8, R8, (const int *)q.ea,
                                (const int8_t *)B8, ldk, (const int *)eb8, KS8,
                                q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
                                w8);
-            hipLaunchKernelGGL((k_tri8<6, 5, 2>), g8, dim3(256), 0, sm,
+            hipLaunchKernelGGL((k_tri8<6, 5>), g8, dim3(512), 0, sm,
                                (const int8_t *)q.L8, R8, (const int *)q.ea,
                                (const int8_t *)B8, ldk, (const int *)eb8, KS8,
                                q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,

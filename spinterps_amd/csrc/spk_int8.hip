@@ -222,27 +222,31 @@ template __global__ void k_slice_cols<double, 6>(
     const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t);
 
 // q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the first S
-// planes, diagonals s + t <= P.  Block tile 128 rows x 32 TJ cells, k in
-// chunks of 64 (one MFMA depth) up to the tile's last row; 4 waves in
-// 2 x 2, each 64 rows x 16 TJ cells = 4 x TJ MFMA tiles per diagonal, the
-// P + 1 diagonal sums held in int32 for the whole k range.  Each chunk's
-// planes are loaded into registers while the previous chunk is multiplied
-// out of LDS (rows padded to 80 bytes so the 16-byte reads of 16 rows hit
-// distinct banks).  XCD remap and the 64-row partial layout are those of
-// k_tri64.  With a chunk map (nzb) only the 64-k chunks that hold a nonzero
-// delta in the block's 128-cell tile are multiplied (the others add
-// exactly nothing); nexec, when given, counts the plane products executed.
-template <int S, int P, int TJ>
-__global__ __launch_bounds__(256, 1) void k_tri8(
+// planes, diagonals s + t <= P.  Block tile 128 rows x 64 cells, k in
+// chunks of 64 (one MFMA depth) up to the tile's last row; 8 waves in
+// 4 x 2, each 32 rows x 32 cells = 2 x 2 MFMA tiles per diagonal, the
+// P + 1 diagonal sums held in int32 for the whole k range.  The wave tile
+// keeps a wave near 200 registers, so two waves share each SIMD and one
+// wave's loads, barriers and epilogue overlap the other's products.  Each
+// chunk's planes are loaded into registers while the previous chunk is
+// multiplied out of LDS (rows padded to 80 bytes so the 16-byte reads of 16
+// rows hit distinct banks).  The two 32-row halves of a 64-row partial are
+// added through LDS in a fixed order.  XCD remap and the 64-row partial
+// layout are those of k_tri64.  With a chunk map (nzb) only the 64-k chunks
+// that hold a nonzero delta in the block's 128-cell tile are multiplied (the
+// others add exactly nothing); nexec, when given, counts the plane
+// products executed.
+template <int S, int P>
+__global__ __launch_bounds__(512) void k_tri8(
     const int8_t *__restrict__ A8, int64_t R, const int *__restrict__ ea,
     const int8_t *__restrict__ B8, int64_t ldk, const int *__restrict__ eb,
     int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
     double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nzb,
     int64_t nzk, unsigned long long *nexec) {
-  constexpr int BM = 128, BN = 32 * TJ, BK = 64, RS = 80;
-  static_assert(BN <= 128 && 128 % BN == 0, "cell tile inside a map tile");
-  constexpr int NB = BN / 64;  // 16-byte B loads per plane and thread
-  static_assert(S <= kPlanes && P <= 2 * (S - 1) && NB >= 1, "shape");
+  constexpr int BM = 128, BN = 64, BK = 64, RS = 80, NT = 512;
+  // B chunk: S planes x 64 cells x 4 x 16 B = 256 S loads over NT threads
+  constexpr int NBQ = (S * 256 + NT - 1) / NT;
+  static_assert(S <= kPlanes && P <= 2 * (S - 1), "shape");
   const int G = gridDim.x * gridDim.y;
   int lin = blockIdx.x + blockIdx.y * gridDim.x;
   if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
@@ -261,59 +265,52 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   __shared__ __align__(16) int8_t As[S][BM * RS];
   __shared__ __align__(16) int8_t Bs[S][BN * RS];
   __shared__ int s_list[kMaxChunks8];  // nonzero 64-k chunks (with a map)
-  __shared__ int s_cnt[4];
+  __shared__ int s_cnt[NT / 64];
+  __shared__ double s_half[2][2][2][16];  // [row pair][wc][tj][column]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
-  // staging: A chunk = 128 rows x 64 B per plane (2 x 16 B per thread),
-  // B chunk = BN cells x 64 B per plane (NB x 16 B per thread)
-  i4_t ra[S][2], rb[S][NB];
+  // staging: A chunk = 128 rows x 64 B per plane (one 16 B load per thread
+  // and plane), B chunk = 64 cells x 64 B per plane
+  i4_t ra[S], rb[NBQ];
   auto load = [&](int k0) {
+    const int row = tid >> 2, seg = tid & 3;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s < S; ++s)
+      ra[s] = *reinterpret_cast<const i4_t *>(
+          A + (int64_t)s * R * KS + (int64_t)(r0 + row) * KS + k0 + seg * 16);
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
-        ra[s][q] = *reinterpret_cast<const i4_t *>(
-            A + (int64_t)s * R * KS + (int64_t)(r0 + row) * KS + k0 +
-            seg * 16);
-      }
-#pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
-        rb[s][q] = *reinterpret_cast<const i4_t *>(
-            B + (int64_t)s * ldk * KS + (c0 + row) * KS + k0 + seg * 16);
-      }
+    for (int q = 0; q < NBQ; ++q) {
+      const int e = tid + q * NT, t = e >> 8, cell = (e & 255) >> 2;
+      if (t < S)
+        rb[q] = *reinterpret_cast<const i4_t *>(
+            B + (int64_t)t * ldk * KS + (c0 + cell) * KS + k0 + seg * 16);
     }
   };
   auto store = [&]() {
+    const int row = tid >> 2, seg = tid & 3;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s < S; ++s)
+      *reinterpret_cast<i4_t *>(&As[s][row * RS + seg * 16]) = ra[s];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
-        *reinterpret_cast<i4_t *>(&As[s][row * RS + seg * 16]) = ra[s][q];
-      }
-#pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        const int e = tid + q * 256, row = e >> 2, seg = e & 3;
-        *reinterpret_cast<i4_t *>(&Bs[s][row * RS + seg * 16]) = rb[s][q];
-      }
+    for (int q = 0; q < NBQ; ++q) {
+      const int e = tid + q * NT, t = e >> 8, cell = (e & 255) >> 2;
+      if (t < S) *reinterpret_cast<i4_t *>(&Bs[t][cell * RS + seg * 16]) = rb[q];
     }
   };
-  i4_t acc[P + 1][4][TJ];
+  i4_t acc[P + 1][2][2];
 #pragma unroll
   for (int d = 0; d <= P; ++d)
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < TJ; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
+      for (int b = 0; b < 2; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
   int nch = (kend + BK - 1) / BK;
   // the same condition as k_slice_cols' (nch <= KS / 64)
   const bool skip = nzb != nullptr && (KS >> 6) <= kMaxChunks8;
   if (skip) {
     int tot = 0;
-    for (int b0 = 0; b0 < nch; b0 += 256) {
+    for (int b0 = 0; b0 < nch; b0 += NT) {
       const int ch = b0 + tid;
       const bool f = ch < nch && chunk_nonzero(nzb, sl, Cs, nzk, c0, ch * BK);
       const unsigned long long bm = __ballot(f);
@@ -322,7 +319,8 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
       int off = tot;
       for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
       if (f) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] = ch;
-      tot += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+#pragma unroll
+      for (int w2 = 0; w2 < NT / 64; ++w2) tot += s_cnt[w2];
       __syncthreads();
     }
     nch = tot;
@@ -341,27 +339,27 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
     if (ch + 1 < nch) load(kchunk(ch + 1));
-    i4_t af[S][4];
+    i4_t af[S][2];
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
+      for (int ti = 0; ti < 2; ++ti)
         af[s][ti] = *reinterpret_cast<const i4_t *>(
-            &As[s][(wr * 64 + ti * 16 + li) * RS + lg * 16]);
+            &As[s][(wr * 32 + ti * 16 + li) * RS + lg * 16]);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-      i4_t bf[TJ];
+      i4_t bf[2];
 #pragma unroll
-      for (int tj = 0; tj < TJ; ++tj)
+      for (int tj = 0; tj < 2; ++tj)
         bf[tj] = *reinterpret_cast<const i4_t *>(
-            &Bs[t][(wc * 16 * TJ + tj * 16 + li) * RS + lg * 16]);
+            &Bs[t][(wc * 32 + tj * 16 + li) * RS + lg * 16]);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         if (s + t > P) continue;
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
+        for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-          for (int tj = 0; tj < TJ; ++tj)
+          for (int tj = 0; tj < 2; ++tj)
             acc[s + t][ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                 af[s][ti], bf[tj], acc[s + t][ti][tj], 0, 0, 0);
       }
@@ -375,18 +373,16 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
   // y = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d; result slot r of lane group lg
   // is row 4 lg + r of the 16 x 16 tile
   const int *er = ea + (int64_t)sl * R;
-  double part[TJ];
+  double part[2] = {0.0, 0.0};
 #pragma unroll
-  for (int tj = 0; tj < TJ; ++tj) part[tj] = 0.0;
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = r0 + wr * 64 + ti * 16 + 4 * lg + r;
+      const int row = r0 + wr * 32 + ti * 16 + 4 * lg + r;
       if (row >= n) continue;
       const int e2 = 2 * er[row];
 #pragma unroll
-      for (int tj = 0; tj < TJ; ++tj) {
+      for (int tj = 0; tj < 2; ++tj) {
         double y = 0.0;
 #pragma unroll
         for (int d = P; d >= 0; --d)
@@ -395,29 +391,37 @@ __global__ __launch_bounds__(256, 1) void k_tri8(
       }
     }
 #pragma unroll
-  for (int tj = 0; tj < TJ; ++tj) {
+  for (int tj = 0; tj < 2; ++tj) {
     part[tj] += __shfl_xor(part[tj], 16, 64);
     part[tj] += __shfl_xor(part[tj], 32, 64);
   }
-  const int prow = 2 * rti + wr;
-  if (lg != 0 || prow >= nrb) return;
+  // rows 32-63 of each 64-row partial go through LDS to the wave holding
+  // rows 0-31 of it, which adds them to its own
+  if ((wr & 1) && lg == 0) {
 #pragma unroll
-  for (int tj = 0; tj < TJ; ++tj) {
-    const int64_t col = c0 + wc * 16 * TJ + tj * 16 + li;
+    for (int tj = 0; tj < 2; ++tj) s_half[wr >> 1][wc][tj][li] = part[tj];
+  }
+  __syncthreads();
+  const int prow = 2 * rti + (wr >> 1);
+  if ((wr & 1) || lg != 0 || prow >= nrb) return;
+#pragma unroll
+  for (int tj = 0; tj < 2; ++tj) {
+    const int64_t col = c0 + wc * 32 + tj * 16 + li;
+    const double v = part[tj] + s_half[wr >> 1][wc][tj][li];
     if (col < Cs)
       qpart[((int64_t)sl * nrb + prow) * ldk + col] =
-          ldexp(part[tj], 2 * eb[(int64_t)sl * ldk + col]);
+          ldexp(v, 2 * eb[(int64_t)sl * ldk + col]);
   }
 }
-template __global__ void k_tri8<5, 4, 2>(const int8_t *, int64_t, const int *,
-                                         const int8_t *, int64_t, const int *,
-                                         int64_t, const StepSc *, int64_t,
-                                         double *, int, const uint8_t *,
-                                         int64_t, unsigned long long *);
-template __global__ void k_tri8<6, 5, 2>(const int8_t *, int64_t, const int *,
-                                         const int8_t *, int64_t, const int *,
-                                         int64_t, const StepSc *, int64_t,
-                                         double *, int, const uint8_t *,
-                                         int64_t, unsigned long long *);
+template __global__ void k_tri8<5, 4>(const int8_t *, int64_t, const int *,
+                                      const int8_t *, int64_t, const int *,
+                                      int64_t, const StepSc *, int64_t,
+                                      double *, int, const uint8_t *, int64_t,
+                                      unsigned long long *);
+template __global__ void k_tri8<6, 5>(const int8_t *, int64_t, const int *,
+                                      const int8_t *, int64_t, const int *,
+                                      int64_t, const StepSc *, int64_t,
+                                      double *, int, const uint8_t *, int64_t,
+                                      unsigned long long *);
 
 }  // namespace spk
