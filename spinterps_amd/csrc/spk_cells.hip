@@ -136,7 +136,7 @@ struct Vec4<double> {
 };
 
 template <typename T, bool PLAIN, int BK>
-__global__ __launch_bounds__(256) void k_gemm(
+__global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
     const T *__restrict__ A, int64_t lda, int64_t strideA,
     const T *__restrict__ B, int64_t ldb, int64_t strideB,
     const StepSc *__restrict__ sc, int p, int method, int want_var, int skm,
@@ -163,7 +163,9 @@ __global__ __launch_bounds__(256) void k_gemm(
     cti = lin / gridDim.x;
   }
   constexpr int VW = 16 / sizeof(T);         // elements per 16-byte load
-  constexpr int PAD = VW;
+  // row padding: FP32 rows 16 banks apart, so the four 16-lane groups of a
+  // 32-bit LDS read (one row each) fall on distinct banks
+  constexpr int PAD = sizeof(T) == 4 ? 16 : VW;
   constexpr int A_LD = BM * BK / VW / 256;   // vector loads per thread
   constexpr int B_LD = BN * BK / VW / 256;
   static_assert(A_LD >= 1 && B_LD >= 1, "tile too small");
@@ -195,9 +197,13 @@ __global__ __launch_bounds__(256) void k_gemm(
   double acc_e[2] = {0.0, 0.0}, acc_l[2] = {0.0, 0.0};
   double acc_v[2] = {0.0, 0.0}, acc_m[2] = {0.0, 0.0};
 
-  VT ra[A_LD], rb[B_LD];
+  // register stages of the next chunks' tiles; the FP32 contraction keeps
+  // two chunks in flight, which covers an L2 miss behind the chunk being
+  // multiplied
+  constexpr bool PF2 = !PLAIN && sizeof(T) == 4;
+  VT ra[A_LD], rb[B_LD], ra2[PF2 ? A_LD : 1], rb2[PF2 ? B_LD : 1];
   // A tile element group q: rows r..r+VW-1 of column k
-  auto load_tiles = [&](int r0, int k0, int kend) {
+  auto load_tiles = [&](auto &ra, auto &rb, int r0, int k0, int kend) {
 #pragma unroll
     for (int q = 0; q < A_LD; ++q) {
       const int e = tid + q * 256;
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(256) void k_gemm(
         rb[q] = VT{};
     }
   };
-  auto store_tiles = [&](int buf) {
+  auto store_tiles = [&](const auto &ra, const auto &rb, int buf) {
 #pragma unroll
     for (int q = 0; q < A_LD; ++q) {
       const int e = tid + q * 256;
@@ -281,13 +287,8 @@ __global__ __launch_bounds__(256) void k_gemm(
     if (!PLAIN && nexec && tid == 0)
       atomicAdd(nexec, (unsigned long long)nch);
     auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
-    load_tiles(r0, nch > 0 ? kchunk(0) : 0, kend);
-    __syncthreads();  // previous row block's readers are done with LDS
-    store_tiles(0);
-    __syncthreads();
-    for (int ch = 0; ch < nch; ++ch) {
-      const int buf = ch & 1;
-      if (ch + 1 < nch) load_tiles(r0, kchunk(ch + 1), kend);
+    // multiply chunk ch out of LDS buffer ch & 1 (FP32: then add to FP64)
+    auto mma_chunk = [&](int buf) {
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
         const int ka = kk * 4 + lg;
@@ -297,11 +298,14 @@ __global__ __launch_bounds__(256) void k_gemm(
 #pragma unroll
         for (int tj = 0; tj < 2; ++tj)
           bf[tj] = Bs[buf][ka][wave * 32 + tj * 16 + li];
+        // FP32 chunk sums start from zero (the previous chunk's went to FP64)
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
           for (int tj = 0; tj < 2; ++tj)
-            acc[ti][tj] = MF::op(af[ti], bf[tj], acc[ti][tj]);
+            acc[ti][tj] = MF::op(af[ti], bf[tj],
+                                 ACC64 && kk == 0 ? V{0, 0, 0, 0}
+                                                  : acc[ti][tj]);
       }
       if (ACC64) {
 #pragma unroll
@@ -311,11 +315,35 @@ __global__ __launch_bounds__(256) void k_gemm(
 #pragma unroll
             for (int r = 0; r < 4; ++r)
               acc64[ti][tj][r] += (double)acc[ti][tj][r];
-            acc[ti][tj] = V{0, 0, 0, 0};
           }
       }
-      if (ch + 1 < nch) store_tiles(buf ^ 1);
+    };
+    load_tiles(ra, rb, r0, nch > 0 ? kchunk(0) : 0, kend);
+    __syncthreads();  // previous row block's readers are done with LDS
+    store_tiles(ra, rb, 0);
+    if constexpr (PF2) {
+      // stage ra holds chunk ch + 1 on entry to chunk ch, even ch
+      if (nch > 1) load_tiles(ra, rb, r0, kchunk(1), kend);
       __syncthreads();
+      auto step = [&](int ch, auto &la, auto &lb, const auto &sa,
+                      const auto &sb) {
+        if (ch + 2 < nch) load_tiles(la, lb, r0, kchunk(ch + 2), kend);
+        mma_chunk(ch & 1);
+        if (ch + 1 < nch) store_tiles(sa, sb, (ch & 1) ^ 1);
+        __syncthreads();
+      };
+      for (int ch = 0; ch < nch; ch += 2) {
+        step(ch, ra2, rb2, ra, rb);
+        if (ch + 1 < nch) step(ch + 1, ra, rb, ra2, rb2);
+      }
+    } else {
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        if (ch + 1 < nch) load_tiles(ra, rb, r0, kchunk(ch + 1), kend);
+        mma_chunk(ch & 1);
+        if (ch + 1 < nch) store_tiles(ra, rb, (ch & 1) ^ 1);
+        __syncthreads();
+      }
     }
     // epilogue for this row block
 #pragma unroll
