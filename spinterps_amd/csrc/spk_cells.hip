@@ -1599,6 +1599,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // 128-cell tile (mixed path): the FP32 contraction skips the others
   uint8_t *nzb = nullptr;
   const int64_t nzk = ld / 16;
+  // map of the 64 x 64 (cells x stations) blocks whose int8 planes hold a
+  // nonzero digit, written by k_slice_cols and read by k_tri8
+  uint8_t *nz8 = nullptr;
   // every consumer of the mixed path's delta (FP32 and int8 contractions,
   // int8 slicing) skips the unmarked chunks when the FP32 contraction's
   // chunk list (256 chunks of 16 rows) covers the triangle: kgen then
@@ -1617,6 +1620,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       SPK_TRY(ws_get(ctx, "eb8", S * ldk * sizeof(int), (void **)&eb8));
       SPK_TRY(ws_get(ctx, "dmax", S * ldk * 8, (void **)&dmax));
       SPK_TRY(ws_get(ctx, "nzb", S * (ldk / 128) * nzk, (void **)&nzb));
+      SPK_TRY(ws_get(ctx, "nz8", S * (ldk / 64) * (KS8 / 64), (void **)&nz8));
     }
     SPK_TRY(ws_get(ctx, "abuf", S * ldk * 8, (void **)&abuf));
     SPK_TRY(ws_get(ctx, "qpart", S * nrb * ldk * 8, (void **)&qpart));
@@ -1845,11 +1849,11 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
             hipLaunchKernelGGL((k_slice_cols<double, 5>), gs, dim3(256), 0, sm,
                                (const double *)K64, ld, ldk, cn, q.sc,
                                (const double *)dmax, B8, KS8, eb8,
-                               (const uint8_t *)nzb, nzk);
+                               (const uint8_t *)nzb, nzk, nz8);
             hipLaunchKernelGGL((k_slice_cols<double, 6>), gs, dim3(256), 0, sm,
                                (const double *)K64, ld, ldk, cn, q.sc,
                                (const double *)dmax, B8, KS8, eb8,
-                               (const uint8_t *)nzb, nzk);
+                               (const uint8_t *)nzb, nzk, nz8);
           }
           {
             ProfScope ps(ctx, "gemm_tri_i8");
@@ -1860,12 +1864,12 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
             hipLaunchKernelGGL((k_tri8<5, 4>), g8, dim3(512), 0, sm,
                                (const int8_t *)q.L8, R8, (const int *)q.ea,
                                (const int8_t *)B8, ldk, (const int *)eb8, KS8,
-                               q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
+                               q.sc, cn, qpart, nrb, (const uint8_t *)nz8,
                                w8);
             hipLaunchKernelGGL((k_tri8<6, 5>), g8, dim3(512), 0, sm,
                                (const int8_t *)q.L8, R8, (const int *)q.ea,
                                (const int8_t *)B8, ldk, (const int *)eb8, KS8,
-                               q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
+                               q.sc, cn, qpart, nrb, (const uint8_t *)nz8,
                                w8);
           }
           SPK_TRY(check_launch("k_tri8"));

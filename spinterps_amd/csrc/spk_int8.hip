@@ -148,13 +148,16 @@ __global__ __launch_bounds__(256) void k_slice_rows(
 // chunk_nonzero) and at most kMaxChunks8 chunks of 64 k, a block whose 64 k
 // hold no nonzero delta in its 128-cell tile writes no planes: k_tri8
 // skips the same chunks.  Within a block, the 16-row chunks the map leaves
-// unmarked are taken as zeros without reading delta.
+// unmarked are taken as zeros without reading delta.  Every block writes
+// nz8[step][cell tile][chunk] (ldk / 64 tiles of 64 cells, KS / 64 chunks):
+// 1 where one of its S planes holds a nonzero digit, which is what k_tri8
+// multiplies.
 template <typename T, int S>
 __global__ __launch_bounds__(256) void k_slice_cols(
     const T *__restrict__ K, int64_t ld, int64_t ldk, int64_t Cs,
     const StepSc *__restrict__ sc, const double *__restrict__ dmax,
     int8_t *__restrict__ B8, int64_t KS, int *__restrict__ eb,
-    const uint8_t *__restrict__ nzb, int64_t nzk) {
+    const uint8_t *__restrict__ nzb, int64_t nzk, uint8_t *__restrict__ nz8) {
   const int sl = blockIdx.z;
   const StepSc st = sc[sl];
   if (!int8_step(st) || st.planes != S) return;
@@ -173,9 +176,13 @@ __global__ __launch_bounds__(256) void k_slice_cols(
     cexp[tid] = e;
     if (blockIdx.y == 0) eb[(int64_t)sl * ldk + c] = e;
   }
+  uint8_t *nz8e =
+      nz8 + ((int64_t)sl * (ldk >> 6) + blockIdx.x) * (KS >> 6) + blockIdx.y;
   if (nzb && (KS >> 6) <= kMaxChunks8 &&
-      !chunk_nonzero(nzb, sl, Cs, nzk, c0, k0))
+      !chunk_nonzero(nzb, sl, Cs, nzk, c0, k0)) {
+    if (tid == 0) *nz8e = 0;
     return;
+  }
   // the 16-row chunks the map leaves unmarked hold delta = 0 for every cell
   // of the tile (k_kgen may leave them unwritten): zeros, not reads
   unsigned live = 15u;
@@ -208,18 +215,25 @@ __global__ __launch_bounds__(256) void k_slice_cols(
 #pragma unroll
     for (int s = 0; s < kPlanes; ++s) p[s][j] = q[s];
   }
+  int any = 0;
 #pragma unroll
-  for (int s = 0; s < S; ++s)
+  for (int s = 0; s < S; ++s) {
+    const i4_t v = *reinterpret_cast<const i4_t *>(p[s]);
+    any |= v[0] | v[1] | v[2] | v[3];
     *reinterpret_cast<i4_t *>(Bp + (int64_t)s * ldk * KS +
-                              (c0 + wc) * KS + k0 + wseg * 16) =
-        *reinterpret_cast<const i4_t *>(p[s]);
+                              (c0 + wc) * KS + k0 + wseg * 16) = v;
+  }
+  any = __syncthreads_or(any);
+  if (tid == 0) *nz8e = any ? 1 : 0;
 }
 template __global__ void k_slice_cols<double, 5>(
     const double *, int64_t, int64_t, int64_t, const StepSc *,
-    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t);
+    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t,
+    uint8_t *);
 template __global__ void k_slice_cols<double, 6>(
     const double *, int64_t, int64_t, int64_t, const StepSc *,
-    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t);
+    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t,
+    uint8_t *);
 
 // q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the first S
 // planes, diagonals s + t <= P.  Block tile 128 rows x 64 cells, k in
@@ -232,17 +246,17 @@ template __global__ void k_slice_cols<double, 6>(
 // multiplied out of LDS (rows padded to 80 bytes so the 16-byte reads of 16
 // rows hit distinct banks).  The two 32-row halves of a 64-row partial are
 // added through LDS in a fixed order.  XCD remap and the 64-row partial
-// layout are those of k_tri64.  With a chunk map (nzb) only the 64-k chunks
-// that hold a nonzero delta in the block's 128-cell tile are multiplied (the
-// others add exactly nothing); nexec, when given, counts the plane
-// products executed.
+// layout are those of k_tri64.  With the digit map nz8 of k_slice_cols only
+// the 64-k chunks whose delta planes hold a nonzero digit in the block's 64
+// cells are multiplied (the others add exactly nothing); nexec, when given,
+// counts the plane products executed.
 template <int S, int P>
 __global__ __launch_bounds__(512) void k_tri8(
     const int8_t *__restrict__ A8, int64_t R, const int *__restrict__ ea,
     const int8_t *__restrict__ B8, int64_t ldk, const int *__restrict__ eb,
     int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
-    double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nzb,
-    int64_t nzk, unsigned long long *nexec) {
+    double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nz8,
+    unsigned long long *nexec) {
   constexpr int BM = 128, BN = 64, BK = 64, RS = 80, NT = 512;
   // B chunk: S planes x 64 cells x 4 x 16 B = 256 S loads over NT threads
   constexpr int NBQ = (S * 256 + NT - 1) / NT;
@@ -306,13 +320,15 @@ __global__ __launch_bounds__(512) void k_tri8(
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
   int nch = (kend + BK - 1) / BK;
-  // the same condition as k_slice_cols' (nch <= KS / 64)
-  const bool skip = nzb != nullptr && (KS >> 6) <= kMaxChunks8;
+  // the list holds up to kMaxChunks8 chunks (nch <= KS / 64)
+  const bool skip = nz8 != nullptr && (KS >> 6) <= kMaxChunks8;
+  const uint8_t *nzd = skip ? nz8 + ((int64_t)sl * (ldk >> 6) + cti) * (KS >> 6)
+                            : nullptr;
   if (skip) {
     int tot = 0;
     for (int b0 = 0; b0 < nch; b0 += NT) {
       const int ch = b0 + tid;
-      const bool f = ch < nch && chunk_nonzero(nzb, sl, Cs, nzk, c0, ch * BK);
+      const bool f = ch < nch && nzd[ch] != 0;
       const unsigned long long bm = __ballot(f);
       if (lane == 0) s_cnt[wave] = __popcll(bm);
       __syncthreads();
@@ -416,12 +432,12 @@ __global__ __launch_bounds__(512) void k_tri8(
 template __global__ void k_tri8<5, 4>(const int8_t *, int64_t, const int *,
                                       const int8_t *, int64_t, const int *,
                                       int64_t, const StepSc *, int64_t,
-                                      double *, int, const uint8_t *, int64_t,
+                                      double *, int, const uint8_t *,
                                       unsigned long long *);
 template __global__ void k_tri8<6, 5>(const int8_t *, int64_t, const int *,
                                       const int8_t *, int64_t, const int *,
                                       int64_t, const StepSc *, int64_t,
-                                      double *, int, const uint8_t *, int64_t,
+                                      double *, int, const uint8_t *,
                                       unsigned long long *);
 
 }  // namespace spk

@@ -69,12 +69,12 @@ template <typename T, int S>
 __global__ void k_slice_cols(const T *K, int64_t ld, int64_t ldk, int64_t Cs,
                              const StepSc *sc, const double *dmax, int8_t *B8,
                              int64_t KS, int *eb, const uint8_t *nzb,
-                             int64_t nzk);
+                             int64_t nzk, uint8_t *nz8);
 template <int S, int P>
 __global__ void k_tri8(const int8_t *A8, int64_t R, const int *ea,
                        const int8_t *B8, int64_t ldk, const int *eb,
                        int64_t KS, const StepSc *sc, int64_t Cs,
-                       double *qpart, int nrb, const uint8_t *nzb, int64_t nzk,
+                       double *qpart, int nrb, const uint8_t *nz8,
                        unsigned long long *nexec);
 
 }  // namespace spk
