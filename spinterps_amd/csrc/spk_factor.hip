@@ -490,11 +490,15 @@ __device__ void small_inverse(double *B, int p, double *Binv) {
 }
 
 // Y[:, r] = Li X[:, r] (r < nr), X and Y column-major with leading dim ld.
-__device__ void tri_lower_mv(const double *Li, int64_t ld, int n,
-                             const double *X, double *Y, int nr) {
+// The k loops are unrolled so that a thread's loads of the next entries are
+// in flight while it adds up the current ones (in the same order).
+__device__ void tri_lower_mv(const double *__restrict__ Li, int64_t ld, int n,
+                             const double *__restrict__ X,
+                             double *__restrict__ Y, int nr) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     double acc[kMaxP + 1];
     for (int r = 0; r < nr; ++r) acc[r] = 0.0;
+#pragma unroll 8
     for (int k = 0; k <= i; ++k) {
       const double l = Li[i + (int64_t)k * ld];
       for (int r = 0; r < nr; ++r) acc[r] += l * X[k + (int64_t)r * ld];
@@ -504,13 +508,15 @@ __device__ void tri_lower_mv(const double *Li, int64_t ld, int n,
 }
 
 // Y[:, r] = Li^T X[:, r]: one wave per output entry, lanes over rows.
-__device__ void tri_lower_tmv(const double *Li, int64_t ld, int n,
-                              const double *X, double *Y, int nr) {
+__device__ void tri_lower_tmv(const double *__restrict__ Li, int64_t ld,
+                              int n, const double *__restrict__ X,
+                              double *__restrict__ Y, int nr) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
   for (int k = wave; k < n; k += nw) {
     double acc[kMaxP + 1];
     for (int r = 0; r < nr; ++r) acc[r] = 0.0;
+#pragma unroll 4
     for (int i = k + lane; i < n; i += 64) {
       const double l = Li[i + (int64_t)k * ld];
       for (int r = 0; r < nr; ++r) acc[r] += l * X[i + (int64_t)r * ld];
