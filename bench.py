@@ -31,8 +31,8 @@ sys.path.insert(0, str(ROOT))
 # from a run in which k_tri64 contracted every step above the FP32 limit
 # (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
-TRI32_TRAFFIC = 2 * 1804649 * 1024 + 68983 * 1024
-TRI8_TRAFFIC = 2 * (6195784 + 928303) * 1024 + (17121 + 5818) * 1024
+TRI32_TRAFFIC = 2 * 1811231 * 1024 + 71089 * 1024
+TRI8_TRAFFIC = 2 * (2382306 + 61343) * 1024 + (17656 + 6000) * 1024
 
 
 def log(*a):
