@@ -335,7 +335,7 @@ def main():
            'unit': 'TFLOP/s', 'frac': ach32 / 157.3,
            'algorithmic_tflops': alg32, 'traffic': TRI32_TRAFFIC,
            'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA, FP64 sums '
-                     'every 16 rows)', 'traffic_note': note}
+                     'every 32 rows)', 'traffic_note': note}
     r8 = {'bound': 'mfma', 'achieved': ach8, 'peak': 5000.0,
           'unit': 'TOPS', 'frac': ach8 / 5000.0, 'traffic': TRI8_TRAFFIC,
           'algorithmic_tops': alg8, 'fp64_equivalent_tflops': eq8,

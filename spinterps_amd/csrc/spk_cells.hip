@@ -247,7 +247,8 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
     }
   };
 
-  // FP32 MFMA sums only BK products per chunk; chunk sums are added in FP64
+  // FP32 MFMA sums are added in FP64 after every chunk (BK products), or
+  // for the FP32 contraction after every second chunk (2 BK products)
   constexpr bool ACC64 = sizeof(T) == 4;
   for (int r0 = r_lo; r0 < r_hi; r0 += BM) {
     V acc[4][2];
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
       atomicAdd(nexec, (unsigned long long)nch);
     auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
     // multiply chunk ch out of LDS buffer ch & 1 (FP32: then add to FP64)
-    auto mma_chunk = [&](int buf) {
+    auto mma_chunk = [&](int buf, bool zs = true, bool fl = true) {
 #pragma unroll
       for (int kk = 0; kk < BK / 4; ++kk) {
         const int ka = kk * 4 + lg;
@@ -304,10 +305,10 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
 #pragma unroll
           for (int tj = 0; tj < 2; ++tj)
             acc[ti][tj] = MF::op(af[ti], bf[tj],
-                                 ACC64 && kk == 0 ? V{0, 0, 0, 0}
-                                                  : acc[ti][tj]);
+                                 ACC64 && zs && kk == 0 ? V{0, 0, 0, 0}
+                                                        : acc[ti][tj]);
       }
-      if (ACC64) {
+      if (ACC64 && fl) {
 #pragma unroll
         for (int ti = 0; ti < (ACC64 ? 4 : 1); ++ti)
 #pragma unroll
@@ -328,7 +329,9 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
       auto step = [&](int ch, auto &la, auto &lb, const auto &sa,
                       const auto &sb) {
         if (ch + 2 < nch) load_tiles(la, lb, r0, kchunk(ch + 2), kend);
-        mma_chunk(ch & 1);
+        // FP32 sums of two chunks (32 products) start from zero and then
+        // go to FP64
+        mma_chunk(ch & 1, (ch & 1) == 0, (ch & 1) || ch + 1 >= nch);
         if (ch + 1 < nch) store_tiles(sa, sb, (ch & 1) ^ 1);
         __syncthreads();
       };
@@ -1877,7 +1880,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
         if (prec == SPK_P_MIXED) {
           ProfScope ps(ctx, "gemm_tri_f32");
-          // k chunks of 16: FP32 sums of 16 products, 3 waves per SIMD
+          // k chunks of 16: FP32 sums of 32 products, 3 waves per SIMD
           hipLaunchKernelGGL((k_gemm<float, false, 16>), grid, dim3(256), 0, sm,
                              q.L32, ld, strideM, (const float *)K32, ldk,
                              ld * ldk, q.sc, p, method, 1, 0, q.zact, ldz,
