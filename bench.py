@@ -55,6 +55,11 @@ def main():
                     help='simulated fields for the variogram-fit leg '
                          '(0 skips it)')
     ap.add_argument('--cpu-vg-fields', type=int, default=4)
+    ap.add_argument('--idw-steps', type=int, default=365,
+                    help='IDW leg: daily steps (1000 stations, 500 x 500 '
+                         'cells at 1 km, p = 2); 0 skips it')
+    ap.add_argument('--cpu-idw-cells', type=int, default=2000)
+    ap.add_argument('--cpu-idw-steps', type=int, default=8)
     ap.add_argument('--stress', type=int, default=0,
                     help='also run this many steps of the ill-conditioned '
                          'stress set (daily inputs, Gaussian model, nugget '
@@ -320,6 +325,7 @@ def main():
         log('calibration copy done: k_clip_cast, 2 GiB read + 1 GiB write')
 
     vgfit = vgfit_leg(args, log) if args.vg_fields > 0 else None
+    idw = idw_leg(args, log) if args.idw_steps > 0 else None
     stress = stress_leg(args, log) if args.stress > 0 else None
     e2e = e2e_leg(args, log) if args.e2e > 0 else None
 
@@ -392,6 +398,7 @@ def main():
         'steps_fp32_variance': int(f32_steps.sum()),
         'cpu_baseline': cpu,
         'variograms_fitted': vgfit,
+        'idw': idw,
     }
     if stress is not None:
         out['stress'] = stress
@@ -513,6 +520,82 @@ def stress_leg(args, log):
                      '|estimate| and 1e-3 x mean variance')
     log(f'stress: {res}')
     return res
+
+
+def idw_leg(args, log):
+    """IDW cell-estimates/s on the daily workload (1 000 stations, 0-20 %
+    missing per step, 500 x 500 cells at 1 km, p = 2), inputs resident in
+    HBM; one k_idw_nnb launch per batch of steps (for p = 2 one pass over
+    the active stations per cell, weights 1 / d^2: the reference's
+    normalisation by the max distance cancels in the ratio).  The
+    oracle's IDW (interp/steps.py:293-313 vectorised with NumPy, one BLAS
+    thread) on a cell sample is the CPU baseline and the reference values
+    for the error."""
+    import numpy as np
+    import torch
+    from spinterps_amd import configs
+    from spinterps_amd.interp.steps import (DeviceProblem, interp_steps,
+                                            prof_enable, prof_get)
+    w = configs.daily()
+    T = min(args.idw_steps, w.n_steps)
+    vals, _ = configs.daily_steps(w, 0, T)
+    C = len(w.cell_x)
+    dev = torch.device('cuda:0')
+    prob = DeviceProblem.from_workload(w)
+    vals_d = torch.as_tensor(vals, device=dev)
+    est = torch.empty((T, C), dtype=torch.float64, device=dev)
+    interp_steps(prob, vals_d[:2], None, 'IDW', idw_exp=2.0, est=est[:2])
+    prof_enable(0, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    interp_steps(prob, vals_d, None, 'IDW', idw_exp=2.0, est=est)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_k, cnt = prof_get('idw_nnb')
+    prof_enable(0, False)
+    rate = T * C / (t1 - t0)
+    n_act = np.sum(~np.isnan(vals), axis=1).astype(np.float64)
+    # 4 n C flops per step: num and den, a multiply-add each per station
+    fl = float(np.sum(4.0 * n_act * C))
+    ach = fl / (ms_k * 1e-3) / 1e12 if ms_k > 0 else 0.0
+    log(f'idw: {(t1 - t0) * 1e3:.1f} ms for {T} steps, {rate:.3e} '
+        f'cell-estimates/s, kernel {ms_k:.1f} ms / {cnt}')
+    out = {'value': rate, 'unit': 'cell-estimates/s', 'steps': T,
+           'cells': C, 'ms': (t1 - t0) * 1e3,
+           'config': 'daily: 1000 stations, 0-20 % missing per step, '
+                     '500x500 cells at 1 km, p = 2',
+           'roofline': {'bound': 'valu', 'achieved': ach, 'peak': 78.6,
+                        'unit': 'TFLOP/s', 'frac': ach / 78.6,
+                        'note': 'algorithmic 4 n C flops per step (the '
+                                'num/den contraction); the kernel also '
+                                'forms n C squared distances and '
+                                'reciprocals'}}
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import spk_oracle as O
+        rng = np.random.default_rng(9)
+        cidx = np.sort(rng.choice(C, args.cpu_idw_cells, replace=False))
+        S = min(args.cpu_idw_steps, T)
+        with threadpool_limits(1):
+            tc0 = time.perf_counter()
+            _, f64, _ = O.interp_all(w.stn_x, w.stn_y, w.cell_x[cidx],
+                                     w.cell_y[cidx], vals[:S], None,
+                                     [('IDW', 'IDW', 2.0)])
+            tc1 = time.perf_counter()
+        ref = f64['IDW']
+        got = est[:S].cpu().numpy()[:, cidx]
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+        out['error_vs_oracle'] = {'max': float(rel.max()),
+                                  'median': float(np.median(rel)),
+                                  'steps': S, 'cells': int(len(cidx))}
+        out['cpu_baseline'] = {
+            'value': S * len(cidx) / (tc1 - tc0), 'unit': 'cell-estimates/s',
+            'cores': 1, 'kind': 'port',
+            'sample': f'{S} daily steps x {len(cidx)} cells, the oracle\'s '
+                      'NumPy IDW with its distances, one BLAS thread'}
+        log(f"idw vs oracle: {out['error_vs_oracle']}, cpu "
+            f"{out['cpu_baseline']['value']:.3e}/s")
+    return out
 
 
 def vgfit_leg(args, log):

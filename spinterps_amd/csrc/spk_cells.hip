@@ -1126,6 +1126,40 @@ __global__ __launch_bounds__(256) void k_idw_nnb(
   const int64_t c = blockIdx.x * (int64_t)256 + tid;
   const bool valid = c < C;
   const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  double *eo = est + (int64_t)sl * C;
+  if (method != SPK_M_NNB && p == 2.0) {
+    // p = 2: the reference's weights 1 / (d / dmax)^2 are dmax^2 / s with
+    // s = dx^2 + dy^2, and the per-cell factor dmax^2 cancels in
+    // sum(w z) / sum(w); one pass with w = 1 / s (no square root, one
+    // division).  The weight-sum check is made on the normalised sum
+    // max(s) sum(1 / s).
+    double ws = 0.0, wz = 0.0, smax = 0.0;
+    for (int base = 0; base < n; base += 256) {
+      __syncthreads();
+      if (base + tid < n) {
+        const int g = my[base + tid];
+        stx[tid] = sx[g];
+        sty[tid] = sy[g];
+        sz[tid] = z[base + tid];
+      }
+      __syncthreads();
+      const int cnt = min(256, n - base);
+      if (valid)
+        for (int j = 0; j < cnt; ++j) {
+          const double dx = x - stx[j], dy = y - sty[j];
+          const double s2 = (dx * dx) + (dy * dy);
+          smax = fmax(smax, s2);
+          const double wj = 1.0 / s2;
+          ws = ws + wj;
+          wz = wz + wj * sz[j];
+        }
+    }
+    if (!valid) return;
+    const double wn = smax > 0.0 ? ws * smax : ws;
+    if (!(wn >= 1e-14)) atomicOr(bad, 1);  // interp/steps.py:305
+    eo[c] = wz / ws;
+    return;
+  }
   double dmax = 0.0, best = INFINITY;
   int bi = 0;
   // pass 1: max distance (IDW) / nearest station (NNB)
@@ -1149,7 +1183,6 @@ __global__ __launch_bounds__(256) void k_idw_nnb(
         }
       }
   }
-  double *eo = est + (int64_t)sl * C;
   if (method == SPK_M_NNB) {
     if (valid) eo[c] = z[bi];
     return;
