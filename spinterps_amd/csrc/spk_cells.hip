@@ -1610,9 +1610,19 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       }
     }
   }
-  // cell stripes for the materialised right-hand sides / deflated covariances
+  // cell stripes for the materialised right-hand sides / deflated covariances:
+  // the budget is an eighth of the device's memory, at least 12 GiB and at
+  // most 48 GiB (36 GB of a 288 GB MI355X).  Longer stripes mean fewer
+  // launches and fewer partly idle launch tails; a cell's results do not
+  // depend on the stripe length (stripes are whole 128-cell tiles).
   int64_t Cs = C;
-  const size_t rhs_budget = (size_t)12 << 30;
+  size_t rhs_budget = (size_t)12 << 30;
+  {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+      rhs_budget = std::min<size_t>(std::max<size_t>(rhs_budget, tot / 8),
+                                    (size_t)48 << 30);
+  }
   if (!batched) {
     Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
   } else if (want_var) {
