@@ -1631,7 +1631,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     const int64_t eb = prec == SPK_P_MIXED ? 12 + kInt8Planes : 8;
     Cs = std::min<int64_t>(C, (int64_t)(2 * rhs_budget / (S * ld * eb)));
   }
-  Cs = std::max<int64_t>(128, Cs / 128 * 128);
+  // whole 256-cell blocks of k_kgen: a block's cells share a table of
+  // nearest stations for their local fits, so with stripes of whole blocks
+  // a cell's block, and its results, do not depend on the stripe length
+  Cs = std::max<int64_t>(256, Cs / 256 * 256);
   if (Cs > C) Cs = C;
   const int64_t ldk = round_up(Cs, 128);
   const int nrb = (int)ceil_div(N, 64);

@@ -404,6 +404,26 @@ def test_mixed_slot_overflow_promotes(gpu):
     assert np.max(np.abs(v - v64) / np.maximum(np.abs(v64), floor)) < 1e-9
 
 
+def test_mixed_results_independent_of_call_length(gpu):
+    """The same 16 steps alone and inside a 64-step call give bit-identical
+    estimates and variances.  The cell stripes depend on the batch length
+    (the workspace budget is shared by the batch's steps): on a 200 x 200
+    grid the 64-step call splits the cells into stripes, the 16-step call
+    does not; stripes are whole 256-cell blocks, so every cell keeps its
+    block of nearest stations and its local fit."""
+    import torch
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.hourly(n_side=200, cell=500.0)
+    vals, vgs = configs.hourly_steps(w, 0, 64)
+    prob = DeviceProblem.from_workload(w)
+    e64, v64, _ = interp_steps(prob, vals, vgs, 'OK', precision='mixed')
+    e16, v16, _ = interp_steps(prob, vals[:16], vgs[:16], 'OK',
+                               precision='mixed')
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(e16.cpu().numpy(), e64[:16].cpu().numpy())
+    np.testing.assert_array_equal(v16.cpu().numpy(), v64[:16].cpu().numpy())
+
+
 @pytest.mark.parametrize('model', ['Sph', 'Exp', 'Gau'])
 def test_mixed_culled_station_chunks_bit_exact(gpu, model):
     """kgen culls the 16-station sub-chunks that are beyond the model's
