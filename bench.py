@@ -25,14 +25,15 @@ sys.path.insert(0, str(ROOT))
 
 
 # HBM bytes per dispatch of the variance contractions, measured by
-# rocprofv3 --pmc on this workload (batches of 64 steps, stripes of 10 752
-# cells; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with k_clip_cast); the
+# rocprofv3 --pmc on this workload (batches of 64 steps, 31 stripes per
+# batch on MI355X; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with
+# k_clip_cast); the
 # int8 figure adds its two launches (5 and 6 planes).  The FP64 figure is
 # from a run in which k_tri64 contracted every step above the FP32 limit
 # (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
-TRI32_TRAFFIC = 2 * 1811231 * 1024 + 71089 * 1024
-TRI8_TRAFFIC = 2 * (2382306 + 61343) * 1024 + (17656 + 6000) * 1024
+TRI32_TRAFFIC = 2 * 5401173 * 1024 + 202497 * 1024
+TRI8_TRAFFIC = 2 * (7194895 + 151185) * 1024 + (50293 + 17090) * 1024
 
 
 def log(*a):
