@@ -268,19 +268,37 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
     if (mt <= 0) break;
     const int nt = (mt + 63) / 64;
     const int ty = tid >> 4, tx = tid & 15;
+    // tiles (bi, bj <= bi) in order; the next tile's panel rows are loaded
+    // into registers while this tile is updated (the panel columns are not
+    // written by the trailing update)
+    double qa[8], qb[8];
+    auto fetch = [&](int bi, int bj) {
+      const int i0 = m0 + bi * 64, j0 = m0 + bj * 64;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 256;
+        const int r = e & 63, c = e >> 6;
+        qa[q] = (i0 + r < n && c < jb) ? As[(i0 + r) + (int64_t)(k0 + c) * ld]
+                                       : 0.0;
+        qb[q] = (j0 + r < n && c < jb) ? As[(j0 + r) + (int64_t)(k0 + c) * ld]
+                                       : 0.0;
+      }
+    };
+    fetch(0, 0);
     for (int bi = 0; bi < nt; ++bi) {
       for (int bj = 0; bj <= bi; ++bj) {
         const int i0 = m0 + bi * 64, j0 = m0 + bj * 64;
-        for (int e = tid; e < 64 * 32; e += 256) {
-          const int r = e & 63, c = e >> 6;
-          Pa[r][c] = (i0 + r < n && c < jb)
-                         ? As[(i0 + r) + (int64_t)(k0 + c) * ld]
-                         : 0.0;
-          Pb[r][c] = (j0 + r < n && c < jb)
-                         ? As[(j0 + r) + (int64_t)(k0 + c) * ld]
-                         : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int e = tid + q * 256;
+          Pa[e & 63][e >> 6] = qa[q];
+          Pb[e & 63][e >> 6] = qb[q];
         }
         __syncthreads();
+        if (bj < bi)
+          fetch(bi, bj + 1);
+        else if (bi + 1 < nt)
+          fetch(bi + 1, 0);
         double acc[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -373,20 +391,34 @@ __global__ __launch_bounds__(256) void k_trtri(const double *__restrict__ L,
 #pragma unroll
         for (int b = 0; b < 4; ++b)
           acc[a][b] = (rb == cb && ty * 4 + a == tx * 4 + b) ? 1.0 : 0.0;
-      // acc -= L[r, k] X[k, c] for k in [c0, r0)
-      for (int k0 = c0; k0 < r0; k0 += 32) {
-        for (int e = tid; e < 64 * 32; e += 256) {
+      // acc -= L[r, k] X[k, c] for k in [c0, r0); the next chunk's tiles
+      // are loaded into registers while this one is multiplied
+      double pa_[8], pb_[8];
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int e = tid + q * 256;
           const int r = e & 63, k = e >> 6;
-          sm.t.a[r][k] =
-              (r < rn) ? Ls[(r0 + r) + (int64_t)(k0 + k) * ld] : 0.0;
+          pa_[q] = (r < rn) ? Ls[(r0 + r) + (int64_t)(k0 + k) * ld] : 0.0;
         }
-        for (int e = tid; e < 32 * 64; e += 256) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int e = tid + q * 256;
           const int k = e & 31, c = e >> 5;
-          sm.t.b[k][c] = (c0 + c < n)
-                             ? Xs[(k0 + k) + (int64_t)(c0 + c) * ld]
-                             : 0.0;
+          pb_[q] =
+              (c0 + c < n) ? Xs[(k0 + k) + (int64_t)(c0 + c) * ld] : 0.0;
+        }
+      };
+      if (c0 < r0) fetch(c0);
+      for (int k0 = c0; k0 < r0; k0 += 32) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int e = tid + q * 256;
+          sm.t.a[e & 63][e >> 6] = pa_[q];
+          sm.t.b[e & 31][e >> 5] = pb_[q];
         }
         __syncthreads();
+        if (k0 + 32 < r0) fetch(k0 + 32);
         for (int k = 0; k < 32; ++k) {
           double pa[4], pb[4];
 #pragma unroll
