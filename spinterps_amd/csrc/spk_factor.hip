@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
     // tiles (bi, bj <= bi) in order; the next tile's panel rows are loaded
     // into registers while this tile is updated (the panel columns are not
     // written by the trailing update)
-    double qa[8], qb[8];
+    double qa[8], qb[8], qc[4][4];
     auto fetch = [&](int bi, int bj) {
       const int i0 = m0 + bi * 64, j0 = m0 + bj * 64;
 #pragma unroll
@@ -283,6 +283,14 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
         qb[q] = (j0 + r < n && c < jb) ? As[(j0 + r) + (int64_t)(k0 + c) * ld]
                                        : 0.0;
       }
+      // the tile's own entries (tiles are disjoint)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int i = i0 + ty * 4 + a, j = j0 + tx * 4 + b;
+          qc[a][b] = (i < n && j < n && i >= j) ? As[i + (int64_t)j * ld] : 0.0;
+        }
     };
     fetch(0, 0);
     for (int bi = 0; bi < nt; ++bi) {
@@ -294,6 +302,11 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
           Pa[e & 63][e >> 6] = qa[q];
           Pb[e & 63][e >> 6] = qb[q];
         }
+        double cur[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) cur[a][b] = qc[a][b];
         __syncthreads();
         if (bj < bi)
           fetch(bi, bj + 1);
@@ -321,7 +334,8 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
 #pragma unroll
           for (int b = 0; b < 4; ++b) {
             const int j = j0 + tx * 4 + b;
-            if (i < n && j < n && i >= j) As[i + (int64_t)j * ld] -= acc[a][b];
+            if (i < n && j < n && i >= j)
+              As[i + (int64_t)j * ld] = cur[a][b] - acc[a][b];
           }
         }
         __syncthreads();
