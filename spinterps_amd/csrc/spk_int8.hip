@@ -243,8 +243,9 @@ template __global__ void k_slice_cols<double, 6>(
 // keeps a wave near 200 registers, so two waves share each SIMD and one
 // wave's loads, barriers and epilogue overlap the other's products.  Each
 // chunk's planes are loaded into registers while the previous chunk is
-// multiplied out of LDS (rows padded to 80 bytes so the 16-byte reads of 16
-// rows hit distinct banks).  The two 32-row halves of a 64-row partial are
+// multiplied out of LDS (64-byte rows with their 16-byte segments swizzled
+// by row, so that both the 16-row reads and the 4-row stores of a 16-lane
+// group hit distinct banks).  The two 32-row halves of a 64-row partial are
 // added through LDS in a fixed order.  XCD remap and the 64-row partial
 // layout are those of k_tri64.  With the digit map nz8 of k_slice_cols only
 // the 64-k chunks whose delta planes hold a nonzero digit in the block's 64
@@ -257,7 +258,11 @@ __global__ __launch_bounds__(512) void k_tri8(
     int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
     double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nz8,
     unsigned long long *nexec) {
-  constexpr int BM = 128, BN = 64, BK = 64, RS = 80, NT = 512;
+  constexpr int BM = 128, BN = 64, BK = 64, RS = 64, NT = 512;
+  // 16-byte segment seg of LDS row r is kept at seg ^ ((r >> 2) & 3): the
+  // 16-row reads and the 4-row stores of a 16-lane group then touch 64
+  // distinct banks
+  auto swz = [](int r, int seg) { return r * RS + ((seg ^ ((r >> 2) & 3)) << 4); };
   // B chunk: S planes x 64 cells x 4 x 16 B = 256 S loads over NT threads
   constexpr int NBQ = (S * 256 + NT - 1) / NT;
   static_assert(S <= kPlanes && P <= 2 * (S - 1), "shape");
@@ -305,11 +310,11 @@ __global__ __launch_bounds__(512) void k_tri8(
     const int row = tid >> 2, seg = tid & 3;
 #pragma unroll
     for (int s = 0; s < S; ++s)
-      *reinterpret_cast<i4_t *>(&As[s][row * RS + seg * 16]) = ra[s];
+      *reinterpret_cast<i4_t *>(&As[s][swz(row, seg)]) = ra[s];
 #pragma unroll
     for (int q = 0; q < NBQ; ++q) {
       const int e = tid + q * NT, t = e >> 8, cell = (e & 255) >> 2;
-      if (t < S) *reinterpret_cast<i4_t *>(&Bs[t][cell * RS + seg * 16]) = rb[q];
+      if (t < S) *reinterpret_cast<i4_t *>(&Bs[t][swz(cell, seg)]) = rb[q];
     }
   };
   i4_t acc[P + 1][2][2];
@@ -361,14 +366,14 @@ __global__ __launch_bounds__(512) void k_tri8(
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)
         af[s][ti] = *reinterpret_cast<const i4_t *>(
-            &As[s][(wr * 32 + ti * 16 + li) * RS + lg * 16]);
+            &As[s][swz(wr * 32 + ti * 16 + li, lg)]);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       i4_t bf[2];
 #pragma unroll
       for (int tj = 0; tj < 2; ++tj)
         bf[tj] = *reinterpret_cast<const i4_t *>(
-            &Bs[t][(wc * 32 + tj * 16 + li) * RS + lg * 16]);
+            &Bs[t][swz(wc * 32 + tj * 16 + li, lg)]);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         if (s + t > P) continue;
