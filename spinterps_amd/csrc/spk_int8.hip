@@ -243,7 +243,8 @@ template __global__ void k_slice_cols<double, 6>(
 // keeps a wave near 200 registers, so two waves share each SIMD and one
 // wave's loads, barriers and epilogue overlap the other's products.  Each
 // chunk's planes are loaded into registers while the previous chunk is
-// multiplied out of LDS (64-byte rows with their 16-byte segments swizzled
+// multiplied out of one of two LDS buffers and then stored into the other,
+// one barrier per chunk (64-byte rows with their 16-byte segments swizzled
 // by row, so that both the 16-row reads and the 4-row stores of a 16-lane
 // group hit distinct banks).  The two 32-row halves of a 64-row partial are
 // added through LDS in a fixed order.  XCD remap and the 64-row partial
@@ -281,8 +282,9 @@ __global__ __launch_bounds__(512) void k_tri8(
   const int kend = min(n, r0 + BM);
   const int8_t *A = A8 + (int64_t)sl * kPlanes * R * KS;
   const int8_t *B = B8 + (int64_t)sl * kPlanes * ldk * KS;
-  __shared__ __align__(16) int8_t As[S][BM * RS];
-  __shared__ __align__(16) int8_t Bs[S][BN * RS];
+  // two LDS buffers: chunk ch + 1 is stored while chunk ch is multiplied
+  __shared__ __align__(16) int8_t As[2][S][BM * RS];
+  __shared__ __align__(16) int8_t Bs[2][S][BN * RS];
   __shared__ int s_list[kMaxChunks8];  // nonzero 64-k chunks (with a map)
   __shared__ int s_cnt[NT / 64];
   __shared__ double s_half[2][2][2][16];  // [row pair][wc][tj][column]
@@ -306,15 +308,16 @@ __global__ __launch_bounds__(512) void k_tri8(
             B + (int64_t)t * ldk * KS + (c0 + cell) * KS + k0 + seg * 16);
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
     const int row = tid >> 2, seg = tid & 3;
 #pragma unroll
     for (int s = 0; s < S; ++s)
-      *reinterpret_cast<i4_t *>(&As[s][swz(row, seg)]) = ra[s];
+      *reinterpret_cast<i4_t *>(&As[buf][s][swz(row, seg)]) = ra[s];
 #pragma unroll
     for (int q = 0; q < NBQ; ++q) {
       const int e = tid + q * NT, t = e >> 8, cell = (e & 255) >> 2;
-      if (t < S) *reinterpret_cast<i4_t *>(&Bs[t][swz(cell, seg)]) = rb[q];
+      if (t < S)
+        *reinterpret_cast<i4_t *>(&Bs[buf][t][swz(cell, seg)]) = rb[q];
     }
   };
   i4_t acc[P + 1][2][2];
@@ -355,10 +358,11 @@ __global__ __launch_bounds__(512) void k_tri8(
   auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
   if (nch > 0) {
     load(kchunk(0));
-    store();
+    store(0);
   }
   __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
     if (ch + 1 < nch) load(kchunk(ch + 1));
     i4_t af[S][2];
 #pragma unroll
@@ -366,14 +370,14 @@ __global__ __launch_bounds__(512) void k_tri8(
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)
         af[s][ti] = *reinterpret_cast<const i4_t *>(
-            &As[s][swz(wr * 32 + ti * 16 + li, lg)]);
+            &As[buf][s][swz(wr * 32 + ti * 16 + li, lg)]);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       i4_t bf[2];
 #pragma unroll
       for (int tj = 0; tj < 2; ++tj)
         bf[tj] = *reinterpret_cast<const i4_t *>(
-            &Bs[t][swz(wc * 32 + tj * 16 + li, lg)]);
+            &Bs[buf][t][swz(wc * 32 + tj * 16 + li, lg)]);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         if (s + t > P) continue;
@@ -385,11 +389,10 @@ __global__ __launch_bounds__(512) void k_tri8(
                 af[s][ti], bf[tj], acc[s + t][ti][tj], 0, 0, 0);
       }
     }
+    // the other buffer was last read in chunk ch - 1, before the barrier
+    // that ended it
+    if (ch + 1 < nch) store(buf ^ 1);
     __syncthreads();
-    if (ch + 1 < nch) {
-      store();
-      __syncthreads();
-    }
   }
   // y = 2^(e_i + f_c) sum_d 2^-7(d+2) Z_d; result slot r of lane group lg
   // is row 4 lg + r of the 16 x 16 tile
