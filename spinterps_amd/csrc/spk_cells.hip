@@ -1894,12 +1894,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         if (B8) {
           {
             ProfScope ps(ctx, "slice_cols");
-            const dim3 gs((unsigned)(ldk / 64), (unsigned)(KS8 / 64), Sb);
-            hipLaunchKernelGGL((k_slice_cols<double, 5>), gs, dim3(256), 0, sm,
-                               (const double *)K64, ld, ldk, cn, q.sc,
-                               (const double *)dmax, B8, KS8, eb8,
-                               (const uint8_t *)nzb, nzk, nz8);
-            hipLaunchKernelGGL((k_slice_cols<double, 6>), gs, dim3(256), 0, sm,
+            const dim3 gs((unsigned)(ldk / 64), Sb);
+            hipLaunchKernelGGL((k_slice_cols<double>), gs, dim3(256), 0, sm,
                                (const double *)K64, ld, ldk, cn, q.sc,
                                (const double *)dmax, B8, KS8, eb8,
                                (const uint8_t *)nzb, nzk, nz8);

@@ -60,23 +60,6 @@ __device__ inline bool int8_step(const StepSc &st) {
          !(st.flags & SPK_ST_FP64VAR);
 }
 
-// Whether the 64 k from k0 (a multiple of 64) hold a nonzero delta for the
-// 128-cell tile of cell c0 of step sl: nzb[step][tile][16-row chunk], tiles
-// ceil(Cs / 128) per step and nzk chunks per tile, a nonzero byte marking a
-// chunk that k_kgen found a nonzero delta in.
-__device__ inline bool chunk_nonzero(const uint8_t *nzb, int sl, int64_t Cs,
-                                     int64_t nzk, int64_t c0, int k0) {
-  const int64_t nzt = (Cs + 127) >> 7;
-  const uint8_t *nz = nzb + ((int64_t)sl * nzt + (c0 >> 7)) * nzk;
-  bool f = false;
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int64_t i = (k0 >> 4) + b;
-    f = f || (i < nzk && nz[i] != 0);
-  }
-  return f;
-}
-
 // L^-1 of each int8 step (FP64, column-major lower triangle, ld) -> planes
 // A8[step][s][R][KS] (row i, k contiguous; zero above the diagonal and
 // beyond n) and row exponents ea[step][R].  Grid (R / 64, steps), 256
@@ -139,31 +122,33 @@ __global__ __launch_bounds__(256) void k_slice_rows(
 }
 
 // delta of each int8 step (T = double, [step][k][c] with ldk, rows < n
-// valid, cells < Cs valid) -> the first S planes B8[step][t][ldk][KS]
-// (cell c, k contiguous; zero beyond n and beyond Cs) and cell exponents
-// eb[step][ldk] from the per-cell maxima dmax[step][ldk] that k_kgen
-// leaves (for FP32 input raised by 2^-22 so that the rounded values stay
-// below 2^e).  Grid (ldk / 64, KS / 64, steps), 256 threads: 64 cells x 64
-// k per block, transposed through LDS.  With a chunk map (nzb, see
-// chunk_nonzero) and at most kMaxChunks8 chunks of 64 k, a block whose 64 k
-// hold no nonzero delta in its 128-cell tile writes no planes: k_tri8
-// skips the same chunks.  Within a block, the 16-row chunks the map leaves
-// unmarked are taken as zeros without reading delta.  Every block writes
-// nz8[step][cell tile][chunk] (ldk / 64 tiles of 64 cells, KS / 64 chunks):
-// 1 where one of its S planes holds a nonzero digit, which is what k_tri8
-// multiplies.
-template <typename T, int S>
+// valid, cells < Cs valid) -> its first S planes (S = the step's plane
+// count) B8[step][t][ldk][KS] (cell c, k contiguous; zero beyond n and beyond
+// Cs) and cell exponents eb[step][ldk] from the per-cell maxima dmax[step][ldk]
+// that k_kgen leaves (for FP32 input raised by 2^-22 so that the rounded
+// values stay below 2^e).  Grid (ldk / 64, steps), 256 threads: each block
+// takes its 64 cells through the k range in chunks of 64, transposed through
+// LDS (one launch for both plane counts, and few blocks that only exit).
+// With a chunk map (nzb[step][128-cell tile][16-row chunk], a nonzero byte
+// where k_kgen found a nonzero delta) and at most kMaxChunks8 chunks of 64 k,
+// a chunk that holds no nonzero delta in the block's 128-cell tile gets no
+// planes: k_tri8 skips the same chunks.  Within a chunk, the
+// 16-row chunks the map leaves unmarked are taken as zeros without reading
+// delta.  Every chunk's entry of nz8[step][cell tile][chunk] (ldk / 64 tiles
+// of 64 cells, KS / 64 chunks) is written: 1 where one of the S planes holds
+// a nonzero digit, which is what k_tri8 multiplies.
+template <typename T>
 __global__ __launch_bounds__(256) void k_slice_cols(
     const T *__restrict__ K, int64_t ld, int64_t ldk, int64_t Cs,
     const StepSc *__restrict__ sc, const double *__restrict__ dmax,
     int8_t *__restrict__ B8, int64_t KS, int *__restrict__ eb,
     const uint8_t *__restrict__ nzb, int64_t nzk, uint8_t *__restrict__ nz8) {
-  const int sl = blockIdx.z;
+  const int sl = blockIdx.y;
   const StepSc st = sc[sl];
-  if (!int8_step(st) || st.planes != S) return;
+  if (!int8_step(st)) return;
+  const int S = st.planes;
   const int n = st.n;
   const int64_t c0 = blockIdx.x * (int64_t)64;
-  const int k0 = blockIdx.y * 64;
   const T *D = K + (int64_t)sl * ld * ldk;
   int8_t *Bp = B8 + (int64_t)sl * kPlanes * ldk * KS;
   __shared__ double tile[64][65];
@@ -174,63 +159,66 @@ __global__ __launch_bounds__(256) void k_slice_cols(
     const double m = c < Cs ? dmax[(int64_t)sl * ldk + c] : 0.0;
     const int e = exp_above(sizeof(T) == 4 ? m * (1.0 + 0x1p-22) : m);
     cexp[tid] = e;
-    if (blockIdx.y == 0) eb[(int64_t)sl * ldk + c] = e;
+    eb[(int64_t)sl * ldk + c] = e;
   }
-  uint8_t *nz8e =
-      nz8 + ((int64_t)sl * (ldk >> 6) + blockIdx.x) * (KS >> 6) + blockIdx.y;
-  if (nzb && (KS >> 6) <= kMaxChunks8 &&
-      !chunk_nonzero(nzb, sl, Cs, nzk, c0, k0)) {
-    if (tid == 0) *nz8e = 0;
-    return;
-  }
-  // the 16-row chunks the map leaves unmarked hold delta = 0 for every cell
-  // of the tile (k_kgen may leave them unwritten): zeros, not reads
-  unsigned live = 15u;
-  if (nzb) {
-    const uint8_t *nz =
-        nzb + ((int64_t)sl * ((Cs + 127) >> 7) + (c0 >> 7)) * nzk;
-    live = 0u;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int64_t i = (k0 >> 4) + b;
-      if (i < nzk && nz[i] != 0) live |= 1u << b;
-    }
-  }
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int kk = e >> 6, cc = e & 63;
-    const int k = k0 + kk;
-    const int64_t c = c0 + cc;
-    tile[kk][cc] = (k < n && c < Cs && ((live >> (kk >> 4)) & 1u))
-                       ? (double)D[(int64_t)k * ldk + c]
-                       : 0.0;
-  }
-  __syncthreads();
+  const int nkc = (int)(KS >> 6);
+  uint8_t *nz8r = nz8 + ((int64_t)sl * (ldk >> 6) + blockIdx.x) * nkc;
+  const bool mapped = nzb != nullptr;
+  const bool skip = mapped && nkc <= kMaxChunks8;
+  const uint8_t *nz =
+      mapped ? nzb + ((int64_t)sl * ((Cs + 127) >> 7) + (c0 >> 7)) * nzk
+             : nullptr;
   const int wc = tid >> 2, wseg = tid & 3;
-  int8_t p[kPlanes][16];
-  const int e = cexp[wc];
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int k0 = kc * 64;
+    // the 16-row chunks the map leaves unmarked hold delta = 0 for every
+    // cell of the tile (k_kgen may leave them unwritten): zeros, not reads
+    unsigned live = 15u;
+    if (mapped) {
+      live = 0u;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    int8_t q[kPlanes];
-    planes_of(tile[wseg * 16 + j][wc], e, q);
+      for (int b = 0; b < 4; ++b) {
+        const int64_t i = (k0 >> 4) + b;
+        if (i < nzk && nz[i] != 0) live |= 1u << b;
+      }
+    }
+    if (skip && live == 0u) {
+      if (tid == 0) nz8r[kc] = 0;
+      continue;
+    }
+    // the previous chunk's readers finished at its closing barrier
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int kk = e >> 6, cc = e & 63;
+      const int k = k0 + kk;
+      const int64_t c = c0 + cc;
+      tile[kk][cc] = (k < n && c < Cs && ((live >> (kk >> 4)) & 1u))
+                         ? (double)D[(int64_t)k * ldk + c]
+                         : 0.0;
+    }
+    __syncthreads();
+    int8_t p[kPlanes][16];
+    const int e = cexp[wc];
 #pragma unroll
-    for (int s = 0; s < kPlanes; ++s) p[s][j] = q[s];
+    for (int j = 0; j < 16; ++j) {
+      int8_t q[kPlanes];
+      planes_of(tile[wseg * 16 + j][wc], e, q);
+#pragma unroll
+      for (int s2 = 0; s2 < kPlanes; ++s2) p[s2][j] = q[s2];
+    }
+    int any = 0;
+#pragma unroll
+    for (int s2 = 0; s2 < kPlanes; ++s2) {
+      if (s2 >= S) break;
+      const i4_t v = *reinterpret_cast<const i4_t *>(p[s2]);
+      any |= v[0] | v[1] | v[2] | v[3];
+      *reinterpret_cast<i4_t *>(Bp + (int64_t)s2 * ldk * KS +
+                                (c0 + wc) * KS + k0 + wseg * 16) = v;
+    }
+    any = __syncthreads_or(any);
+    if (tid == 0) nz8r[kc] = any ? 1 : 0;
   }
-  int any = 0;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const i4_t v = *reinterpret_cast<const i4_t *>(p[s]);
-    any |= v[0] | v[1] | v[2] | v[3];
-    *reinterpret_cast<i4_t *>(Bp + (int64_t)s * ldk * KS +
-                              (c0 + wc) * KS + k0 + wseg * 16) = v;
-  }
-  any = __syncthreads_or(any);
-  if (tid == 0) *nz8e = any ? 1 : 0;
 }
-template __global__ void k_slice_cols<double, 5>(
-    const double *, int64_t, int64_t, int64_t, const StepSc *,
-    const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t,
-    uint8_t *);
-template __global__ void k_slice_cols<double, 6>(
+template __global__ void k_slice_cols<double>(
     const double *, int64_t, int64_t, int64_t, const StepSc *,
     const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t,
     uint8_t *);

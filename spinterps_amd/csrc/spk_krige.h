@@ -65,7 +65,7 @@ constexpr double kInt8Deep = 1e7;
 __global__ void k_slice_rows(const double *Li, int64_t ld, int64_t stride,
                              const StepSc *sc, int8_t *A8, int64_t KS,
                              int64_t R, int *ea);
-template <typename T, int S>
+template <typename T>
 __global__ void k_slice_cols(const T *K, int64_t ld, int64_t ldk, int64_t Cs,
                              const StepSc *sc, const double *dmax, int8_t *B8,
                              int64_t KS, int *eb, const uint8_t *nzb,
