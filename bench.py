@@ -32,8 +32,8 @@ sys.path.insert(0, str(ROOT))
 # from a run in which k_tri64 contracted every step above the FP32 limit
 # (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
-TRI32_TRAFFIC = 2 * 5401173 * 1024 + 202497 * 1024
-TRI8_TRAFFIC = 2 * (7194895 + 151185) * 1024 + (50293 + 17090) * 1024
+TRI32_TRAFFIC = 2 * 5403411 * 1024 + 202497 * 1024
+TRI8_TRAFFIC = 2 * (7126920 + 150551) * 1024 + (50293 + 17090) * 1024
 
 
 def log(*a):
