@@ -142,13 +142,16 @@ def _oracle_fields(w, vals, vgs, methods, **kw):
                         **kw)
 
 
-@pytest.mark.parametrize('method', ['IDW', 'NNB'])
-def test_idw_nnb_steps(gpu, method):
+@pytest.mark.parametrize('method,p', [('IDW', 2.0), ('IDW', 1.5),
+                                      ('NNB', 2.0)])
+def test_idw_nnb_steps(gpu, method, p):
+    """IDW (p = 2: one pass with 1/d^2 weights; other p: the reference's
+    two passes) and NNB against the oracle."""
     from spinterps_amd.interp.steps import DeviceProblem, interp_steps
     w, vals, vgs = _small_case(mask=True)
     prob = DeviceProblem.from_workload(w)
-    est, _, info = interp_steps(prob, vals, None, method, idw_exp=2.0)
-    _, f64, _ = _oracle_fields(w, vals, None, [(method, method, 2.0)])
+    est, _, info = interp_steps(prob, vals, None, method, idw_exp=p)
+    _, f64, _ = _oracle_fields(w, vals, None, [(method, method, p)])
     got = est.cpu().numpy()
     if method == 'NNB':
         np.testing.assert_array_equal(got, f64[method])
