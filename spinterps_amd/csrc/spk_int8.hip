@@ -251,7 +251,9 @@ __global__ __launch_bounds__(512) void k_tri8(
   // 16-byte segment seg of LDS row r is kept at seg ^ ((r >> 2) & 3): the
   // 16-row reads and the 4-row stores of a 16-lane group then touch 64
   // distinct banks
-  auto swz = [](int r, int seg) { return r * RS + ((seg ^ ((r >> 2) & 3)) << 4); };
+  auto swz = [](int r, int seg) {
+    return r * RS + ((seg ^ ((r >> 2) & 3)) << 4);
+  };
   // B chunk: S planes x 64 cells x 4 x 16 B = 256 S loads over NT threads
   constexpr int NBQ = (S * 256 + NT - 1) / NT;
   static_assert(S <= kPlanes && P <= 2 * (S - 1), "shape");
