@@ -454,7 +454,9 @@ def stress_leg(args, log):
     """Ill-conditioned stress set on the daily inputs: OK estimate and
     variance on the full 500 x 500 grid by the plain FP64 and the mixed
     paths, and errors of both on sampled cells against the oracle's
-    numpy.linalg.solve of the reference's bordered system."""
+    numpy.linalg.solve of the reference's bordered system.  Steps a path
+    flags ill-conditioned (condition estimate above 1e12, where no FP64
+    formulation reproduces another) are reported apart from the rest."""
     import numpy as np
     import torch
 
@@ -487,6 +489,7 @@ def stress_leg(args, log):
     cidx = rng.choice(w.n_cells, args.stress_cells, replace=False)
     cx, cy = w.cell_x[cidx], w.cell_y[cidx]
     errs = {p: ([], []) for p in outs}
+    errs_ill = {p: ([], []) for p in outs}
     used = []
     for t in range(min(T, args.stress_oracle_steps)):
         act = ~np.isnan(vals[t])
@@ -502,8 +505,9 @@ def stress_leg(args, log):
             gv = v[t].cpu().numpy()[cidx]
             fe = 1e-3 * np.abs(est).mean()
             fv = 1e-3 * np.abs(var).mean()
-            errs[p][0].append(np.abs(ge - est) / np.maximum(np.abs(est), fe))
-            errs[p][1].append(np.abs(gv - var) / np.maximum(np.abs(var), fv))
+            dst = errs_ill if inf[t] & 16 else errs
+            dst[p][0].append(np.abs(ge - est) / np.maximum(np.abs(est), fe))
+            dst[p][1].append(np.abs(gv - var) / np.maximum(np.abs(var), fv))
         used.append(t)
     for p, (le, lv) in errs.items():
         if le:
@@ -513,6 +517,12 @@ def stress_leg(args, log):
                            'var_max': float(ev.max()),
                            'var_median': float(np.median(ev)),
                            'steps_compared': len(le)})
+        li, lvi = errs_ill[p]
+        if li:
+            res[p]['ill_conditioned_steps'] = {
+                'steps': len(li),
+                'est_max': float(np.concatenate(li).max()),
+                'var_max': float(np.concatenate(lvi).max())}
     kp = outs['fp64'][3]
     res['kappa_median'] = float(np.median(kp))
     res['kappa_max'] = float(kp.max())
