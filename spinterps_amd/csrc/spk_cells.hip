@@ -767,7 +767,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
     beta[q] = 0.0;
     slt[q] = -1;
   }
-  double qloc = 0.0, sbeta = 0.0;
+  double qloc = 0.0;
   if (var_on) {
     // the cell's nearest active station, from k_nearest
     const int bj = valid ? nrst[(int64_t)sl * ldn + c] : -1;
@@ -1252,7 +1252,7 @@ __global__ __launch_bounds__(256) void k_special(
   int bi = 0;
   for (int base = 0; base < n; base += 256) {
     __syncthreads();
-    if (base + threadIdx.x < n) {
+    if (base + (int)threadIdx.x < n) {
       stx[threadIdx.x] = sx[my[base + threadIdx.x]];
       sty[threadIdx.x] = sy[my[base + threadIdx.x]];
     }

@@ -76,7 +76,8 @@ __global__ __launch_bounds__(256) void k_sim_apply(
     for (int k0 = 0; k0 <= i0; k0 += 256) {
       __syncthreads();
       xi[threadIdx.x] =
-          (k0 + threadIdx.x < n) ? normal_at(sd, k0 + threadIdx.x) : 0.0;
+          (k0 + (int)threadIdx.x < n) ? normal_at(sd, k0 + threadIdx.x)
+                                      : 0.0;
       __syncthreads();
       if (i < n) {
         const int kmax = min(256, i - k0 + 1);

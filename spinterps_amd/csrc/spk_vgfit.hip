@@ -74,7 +74,6 @@ __global__ __launch_bounds__(256) void k_evg(
   const double *z = vals + (int64_t)s * N;
   int32_t *L = list + (int64_t)s * M;
   __shared__ int wcnt[4];
-  __shared__ int64_t cnt_s;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int64_t base = 0;
   for (int64_t b0 = 0; b0 < M; b0 += 256) {
@@ -138,7 +137,6 @@ __global__ __launch_bounds__(256) void k_evg(
       }
     }
     keep_s = keep;
-    cnt_s = base;
   }
   __syncthreads();
   const int64_t keep = keep_s;
