@@ -226,17 +226,36 @@ __device__ inline double fit_model(int t, double h, double r, double s) {
 constexpr int kDeMaxD = 8;
 constexpr int kDeMaxPop = 800;
 
+// Population of a candidate with D parameters: popmul * D^2, at most
+// kDeMaxPop (the host sizes the workgroup's LDS with the same function).
+__host__ __device__ inline int de_pop(int popmul, int D) {
+  const long long p = (long long)popmul * D * D;
+  return p < kDeMaxPop ? (int)p : kDeMaxPop;
+}
+
+// LDS of one k_de workgroup for populations of up to pcap members with a
+// row stride of ds parameters: population and trials (doubles), their
+// energies (doubles) and the Latin-hypercube permutation (ints).
+inline size_t de_lds_bytes(int pcap, int ds) {
+  return (size_t)pcap * ds * 2 * sizeof(double) +
+         (size_t)pcap * 2 * sizeof(double) + (size_t)pcap * sizeof(int);
+}
+
 __device__ double de_energy(const double *x, const double *lb,
                             const double *ub, const int *types, int ncomp,
                             const double *h, const double *g, int nb) {
-  double pr[kDeMaxD];
-  for (int d = 0; d < 2 * ncomp; ++d) pr[d] = lb[d] + x[d] * (ub[d] - lb[d]);
+  // the parameters are formed where they are used: an array of them,
+  // indexed by the component, would live in scratch memory
   double acc = 0.0;
   for (int b = 0; b < nb; ++b) {
     const double hb = h[b];
     double v = 0.0;
-    for (int c = 0; c < ncomp; ++c)
-      v += fit_model(types[c], hb, pr[2 * c], pr[2 * c + 1]);
+    for (int c = 0; c < ncomp; ++c) {
+      const int d = 2 * c;
+      const double rg = lb[d] + x[d] * (ub[d] - lb[d]);
+      const double sl = lb[d + 1] + x[d + 1] * (ub[d + 1] - lb[d + 1]);
+      v += fit_model(types[c], hb, rg, sl);
+    }
     const double r = g[b] - v;
     acc += r * r;
   }
@@ -251,6 +270,48 @@ __device__ inline double blk_sum(double v, double *red) {
   return red[0] + red[1] + red[2] + red[3];
 }
 
+// (bv, bi) <- the smaller of (bv, bi) and (ov, oi) by value, then index;
+// bi == INT_MAX marks an empty slot
+__device__ inline void argmin_take(double &bv, int &bi, double ov, int oi) {
+  if (oi != INT_MAX && (bi == INT_MAX || ov < bv || (ov == bv && oi < bi))) {
+    bv = ov;
+    bi = oi;
+  }
+}
+
+// Index of the smallest en[i], i < P, over the 256 threads of the block:
+// the first index among equal values, and NaN entries never chosen, which
+// is what the scan `bi = 0; for i: if (en[i] < en[bi]) bi = i` returns
+// (that scan keeps 0 when en[0] is NaN).  Every thread gets the result.
+__device__ inline int blk_argmin(const double *en, int P, double *rv,
+                                 int *ri) {
+  double bv = 0.0;
+  int bi = INT_MAX;
+  for (int i = threadIdx.x; i < P; i += 256) {
+    const double v = en[i];
+    if (!isnan(v) && (bi == INT_MAX || v < bv)) {
+      bv = v;
+      bi = i;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    argmin_take(bv, bi, ov, oi);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    rv[threadIdx.x >> 6] = bv;
+    ri[threadIdx.x >> 6] = bi;
+  }
+  __syncthreads();
+  bv = rv[0];
+  bi = ri[0];
+  for (int w = 1; w < 4; ++w) argmin_take(bv, bi, rv[w], ri[w]);
+  if (isnan(en[0]) || bi == INT_MAX) return 0;
+  return bi;
+}
+
 // One workgroup per (step, candidate).  cand[c]: {ncomp, type[4],
 // fixed_first}; with fixed_first the first component's (range, sill) are
 // pinned to the step's first bin (h0, g0), vgs.py:735-739.
@@ -260,29 +321,35 @@ __global__ __launch_bounds__(256) void k_de(
     const double *__restrict__ var, const int32_t *__restrict__ cand,
     int ncand, int maxiter, int popmul, double tol, double range_ub,
     unsigned long long seed, const double *__restrict__ init, double spread,
-    double *__restrict__ out_par,
+    int pcap, int ds, double *__restrict__ out_par,
     double *__restrict__ out_fun, int32_t *__restrict__ out_nit) {
   const int s = blockIdx.y, ci = blockIdx.x;
   const int tid = threadIdx.x;
   const int32_t *cd = cand + ci * 6;
   const int ncomp = cd[0];
   const int D = 2 * ncomp;
-  const int P = min(popmul * D * D, kDeMaxPop);
+  const int P = de_pop(popmul, D);
   const int nb = nbins[s];
   const double *h = bh + (int64_t)s * max_bins;
   const double *g = bg + (int64_t)s * max_bins;
-  __shared__ double pop[kDeMaxPop * kDeMaxD];
-  __shared__ double trial[kDeMaxPop * kDeMaxD];
-  __shared__ double en[kDeMaxPop];
-  __shared__ double trial_e[kDeMaxPop];
+  // LDS sized by the launch for its largest population (pcap) and
+  // parameter count (ds, the row stride), so that small candidates leave
+  // room for more workgroups per CU
+  extern __shared__ double de_lds[];
+  double *pop = de_lds;
+  double *trial = pop + (int64_t)pcap * ds;
+  double *en = trial + (int64_t)pcap * ds;
+  double *trial_e = en + pcap;
+  int *perm = (int *)(trial_e + pcap);
   __shared__ double lb[kDeMaxD], ub[kDeMaxD];
   __shared__ int types[4];
   __shared__ double red[4];
+  __shared__ double redv[4];
+  __shared__ int redi[4];
   __shared__ double Fs;
-  __shared__ int best_s, stop_s;
-  __shared__ int perm[kDeMaxPop];
+  __shared__ int stop_s;
   double *opar = out_par + ((int64_t)s * ncand + ci) * kDeMaxD;
-  if (nb <= 0) {
+  if (nb <= 0 || P > pcap || D > ds) {
     if (tid == 0) {
       out_fun[(int64_t)s * ncand + ci] = NAN;
       out_nit[(int64_t)s * ncand + ci] = 0;
@@ -327,7 +394,7 @@ __global__ __launch_bounds__(256) void k_de(
     __syncthreads();
     if (!init) {
       for (int i = tid; i < P; i += 256)
-        pop[i * kDeMaxD + d] = (perm[i] + rng.uni()) / P;
+        pop[i * ds + d] = (perm[i] + rng.uni()) / P;
     } else {
       // seeded start: member 0 is the given point; the others stratify a
       // log-box of half-width `spread` around it, clipped to the bounds
@@ -343,39 +410,33 @@ __global__ __launch_bounds__(256) void k_de(
           v = fmin(fmax(v, lb[d]), ub[d]);
           u = (v - lb[d]) / w;
         }
-        pop[i * kDeMaxD + d] = u;
+        pop[i * ds + d] = u;
       }
     }
     __syncthreads();
   }
   for (int i = tid; i < P; i += 256)
-    en[i] = de_energy(&pop[i * kDeMaxD], lb, ub, types, ncomp, h, g, nb);
+    en[i] = de_energy(&pop[i * ds], lb, ub, types, ncomp, h, g, nb);
   __syncthreads();
   int nit = 0;
   for (int it = 1; it <= maxiter; ++it) {
     nit = it;
-    if (tid == 0) {
-      Fs = 0.5 + 0.5 * rng.uni();
-      int bi = 0;
-      for (int i = 1; i < P; ++i)
-        if (en[i] < en[bi]) bi = i;
-      best_s = bi;
-    }
+    const int bi = blk_argmin(en, P, redv, redi);
+    if (tid == 0) Fs = 0.5 + 0.5 * rng.uni();
     __syncthreads();
     const double F = Fs;
-    const int bi = best_s;
     // trials and their energies first, selection after (deferred update)
     for (int i = tid; i < P; i += 256) {
       int r0, r1;
       do { r0 = (int)(rng.uni() * P); } while (r0 == i);
       do { r1 = (int)(rng.uni() * P); } while (r1 == i || r1 == r0);
       const int fill = (int)(rng.uni() * D);
-      double *tr = &trial[i * kDeMaxD];
+      double *tr = &trial[i * ds];
       for (int d = 0; d < D; ++d) {
-        double v = pop[i * kDeMaxD + d];
+        double v = pop[i * ds + d];
         if (d == fill || rng.uni() < 0.7) {
-          v = pop[bi * kDeMaxD + d] +
-              F * (pop[r0 * kDeMaxD + d] - pop[r1 * kDeMaxD + d]);
+          v = pop[bi * ds + d] +
+              F * (pop[r0 * ds + d] - pop[r1 * ds + d]);
           if (v < 0.0 || v > 1.0) v = rng.uni();
         }
         tr[d] = v;
@@ -386,7 +447,7 @@ __global__ __launch_bounds__(256) void k_de(
     for (int i = tid; i < P; i += 256) {
       if (trial_e[i] <= en[i]) {
         en[i] = trial_e[i];
-        for (int d = 0; d < D; ++d) pop[i * kDeMaxD + d] = trial[i * kDeMaxD + d];
+        for (int d = 0; d < D; ++d) pop[i * ds + d] = trial[i * ds + d];
       }
     }
     __syncthreads();
@@ -404,12 +465,10 @@ __global__ __launch_bounds__(256) void k_de(
     __syncthreads();
     if (stop_s) break;
   }
+  const int bi = blk_argmin(en, P, redv, redi);
   if (tid == 0) {
-    int bi = 0;
-    for (int i = 1; i < P; ++i)
-      if (en[i] < en[bi]) bi = i;
     for (int d = 0; d < kDeMaxD; ++d)
-      opar[d] = d < D ? lb[d] + pop[bi * kDeMaxD + d] * (ub[d] - lb[d]) : 0.0;
+      opar[d] = d < D ? lb[d] + pop[bi * ds + d] * (ub[d] - lb[d]) : 0.0;
     out_fun[(int64_t)s * ncand + ci] = en[bi];
     out_nit[(int64_t)s * ncand + ci] = nit;
   }
@@ -610,14 +669,27 @@ static int de_launch(spk_ctx *ctx, const double *bin_h, const double *bin_g,
   int32_t *dc;
   SPK_TRY(ws_get(ctx, "vgcand", n_cands * 6 * 4, (void **)&dc));
   SPK_TRY(h2d(ctx, dc, cands, n_cands * 6 * 4));
+  // LDS rows of the largest candidate: stride ds parameters, pcap members
+  int ds = 0, pcap = 0;
+  for (int c = 0; c < n_cands; ++c) {
+    const int D = 2 * cands[c * 6];
+    ds = std::max(ds, D);
+    pcap = std::max(pcap, de_pop(popmul, D));
+  }
+  const size_t lds = de_lds_bytes(pcap, ds);
+  if (lds > 64 * 1024)
+    SPK_HIP(hipFuncSetAttribute((const void *)k_de,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
   for (int64_t t0 = 0; t0 < T; t0 += 65535) {
     const int Tb = (int)std::min<int64_t>(65535, T - t0);
-    hipLaunchKernelGGL(k_de, dim3(n_cands, Tb), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_de, dim3(n_cands, Tb), dim3(256), lds, ctx->stream,
                        bin_h + t0 * max_bins, bin_g + t0 * max_bins,
                        nbins + t0, max_bins, var + t0, dc, n_cands, maxiter,
                        popmul, tol, range_ub, (unsigned long long)seed,
                        init ? init + t0 * n_cands * kDeMaxD : nullptr, spread,
-                       params + t0 * n_cands * kDeMaxD, fun + t0 * n_cands,
+                       pcap, ds, params + t0 * n_cands * kDeMaxD,
+                       fun + t0 * n_cands,
                        nit + t0 * n_cands);
     SPK_TRY(check_launch("k_de"));
   }
