@@ -70,6 +70,42 @@ def test_de_objective_matches_scipy(gpu):
         assert out[t] is not None and len(out[t]) == 1
 
 
+def test_de_candidates_independent_of_launch_lds(gpu):
+    """Each (step, candidate) DE is one workgroup whose LDS is sized for the
+    largest candidate of its launch: one-component candidates give the same
+    bits alone (rows of 2 parameters, 200 members) and beside three- and
+    four-component ones (rows of 8, 800 members, 118 KB per workgroup), and
+    every candidate's objective is the oracle's objective at its
+    parameters."""
+    from spinterps_amd.variograms.fitvgs import BatchVariogramFitter
+    x, y, _ = _field(0)
+    vals = np.stack([_field(s)[2] for s in (0, 1)])
+    f = BatchVariogramFitter(x, y)
+    # four model types: permutations of up to 4 components (the size is
+    # capped at the number of types, fitvgs.py's candidate rule)
+    kw = dict(mdr=0.5, nk=50, fit_vgs=['Sph', 'Exp', 'Lin', 'Gau'],
+              fil_nug_vg='Nug', range_ub=2e5, opt_iters=300, seed=3,
+              return_details=True)
+    _, d1 = f.fit(vals, perm_r_list=[1], **kw)
+    _, d4 = f.fit(vals, perm_r_list=[1, 3, 4], **kw)
+    n1 = len(d1['cands'])
+    assert list(d4['cands'][:n1]) == list(d1['cands'])
+    assert max(len(o) for o in d4['cands']) == 4
+    np.testing.assert_array_equal(d4['params'][:, :n1], d1['params'])
+    np.testing.assert_array_equal(d4['fun'][:, :n1], d1['fun'])
+    np.testing.assert_array_equal(d4['nit'][:, :n1], d1['nit'])
+    bh = d4['bins'][0].cpu().numpy()
+    bg = d4['bins'][1].cpu().numpy()
+    nb = d4['bins'][3].cpu().numpy()
+    for k in range(len(d4['steps'])):
+        h, g = bh[k, :nb[k]], bg[k, :nb[k]]
+        for c, o in enumerate(d4['cands']):
+            p = d4['params'][k, c, :2 * len(o)]
+            assert np.all(d4['params'][k, c, 2 * len(o):] == 0.0)
+            ref = V.objective(p, list(o), h, g)
+            np.testing.assert_allclose(d4['fun'][k, c], ref, rtol=1e-9)
+
+
 def test_fit_strings_parse_and_skip_rules(gpu):
     from spinterps_amd import _lib
     from spinterps_amd.variograms.fitvgs import BatchVariogramFitter
