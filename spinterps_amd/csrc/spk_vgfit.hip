@@ -233,12 +233,23 @@ __host__ __device__ inline int de_pop(int popmul, int D) {
   return p < kDeMaxPop ? (int)p : kDeMaxPop;
 }
 
+// Members per thread: member i belongs to thread i mod 256, as its k-th
+// member with k = i / 256; its energy and its trial's energy stay in that
+// thread's registers.
+constexpr int kDeOwn = (kDeMaxPop + 255) / 256;
+
 // LDS of one k_de workgroup for populations of up to pcap members with a
-// row stride of ds parameters: population and trials (doubles), their
-// energies (doubles) and the Latin-hypercube permutation (ints).
+// row stride of ds parameters: population and trials (doubles); the
+// Latin-hypercube permutation (pcap ints) shares the trials' space.
 inline size_t de_lds_bytes(int pcap, int ds) {
-  return (size_t)pcap * ds * 2 * sizeof(double) +
-         (size_t)pcap * 2 * sizeof(double) + (size_t)pcap * sizeof(int);
+  return (size_t)pcap * ds * 2 * sizeof(double);
+}
+
+// a[k] = v for a thread's k-th member (static indices keep a in registers)
+__device__ inline void set_own(double (&a)[kDeOwn], int k, double v) {
+#pragma unroll
+  for (int j = 0; j < kDeOwn; ++j)
+    if (j == k) a[j] = v;
 }
 
 __device__ double de_energy(const double *x, const double *lb,
@@ -279,19 +290,27 @@ __device__ inline void argmin_take(double &bv, int &bi, double ov, int oi) {
   }
 }
 
-// Index of the smallest en[i], i < P, over the 256 threads of the block:
-// the first index among equal values, and NaN entries never chosen, which
-// is what the scan `bi = 0; for i: if (en[i] < en[bi]) bi = i` returns
-// (that scan keeps 0 when en[0] is NaN).  Every thread gets the result.
-__device__ inline int blk_argmin(const double *en, int P, double *rv,
-                                 int *ri) {
+// Index of the smallest energy of members i < P (each thread holds its own
+// members' energies, en[k] of member threadIdx.x + 256 k) over the 256
+// threads of the block: the first index among equal values, and NaN
+// entries never chosen, which is what the scan
+// `bi = 0; for i: if (en[i] < en[bi]) bi = i` returns (that scan keeps 0
+// when the energy of member 0 is NaN).  Every thread gets the index and,
+// in best, its energy.
+__device__ inline int blk_argmin(const double (&en)[kDeOwn], int P,
+                                 double *rv, int *ri, double *e0,
+                                 double &best) {
   double bv = 0.0;
   int bi = INT_MAX;
-  for (int i = threadIdx.x; i < P; i += 256) {
-    const double v = en[i];
-    if (!isnan(v) && (bi == INT_MAX || v < bv)) {
-      bv = v;
-      bi = i;
+#pragma unroll
+  for (int k = 0; k < kDeOwn; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    if (i < P) {
+      const double v = en[k];
+      if (!isnan(v) && (bi == INT_MAX || v < bv)) {
+        bv = v;
+        bi = i;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -304,11 +323,16 @@ __device__ inline int blk_argmin(const double *en, int P, double *rv,
     rv[threadIdx.x >> 6] = bv;
     ri[threadIdx.x >> 6] = bi;
   }
+  if (threadIdx.x == 0) *e0 = en[0];
   __syncthreads();
   bv = rv[0];
   bi = ri[0];
   for (int w = 1; w < 4; ++w) argmin_take(bv, bi, rv[w], ri[w]);
-  if (isnan(en[0]) || bi == INT_MAX) return 0;
+  if (isnan(*e0) || bi == INT_MAX) {
+    best = *e0;
+    return 0;
+  }
+  best = bv;
   return bi;
 }
 
@@ -338,14 +362,13 @@ __global__ __launch_bounds__(256) void k_de(
   extern __shared__ double de_lds[];
   double *pop = de_lds;
   double *trial = pop + (int64_t)pcap * ds;
-  double *en = trial + (int64_t)pcap * ds;
-  double *trial_e = en + pcap;
-  int *perm = (int *)(trial_e + pcap);
+  int *perm = (int *)trial;  // start only, before the first trial
   __shared__ double lb[kDeMaxD], ub[kDeMaxD];
   __shared__ int types[4];
   __shared__ double red[4];
   __shared__ double redv[4];
   __shared__ int redi[4];
+  __shared__ double e0s;
   __shared__ double Fs;
   __shared__ int stop_s;
   double *opar = out_par + ((int64_t)s * ncand + ci) * kDeMaxD;
@@ -415,18 +438,25 @@ __global__ __launch_bounds__(256) void k_de(
     }
     __syncthreads();
   }
-  for (int i = tid; i < P; i += 256)
-    en[i] = de_energy(&pop[i * ds], lb, ub, types, ncomp, h, g, nb);
+  double en[kDeOwn], trial_e[kDeOwn];
+#pragma unroll
+  for (int k = 0; k < kDeOwn; ++k) en[k] = trial_e[k] = 0.0;
+  for (int k = 0; tid + 256 * k < P; ++k) {
+    const int i = tid + 256 * k;
+    set_own(en, k, de_energy(&pop[i * ds], lb, ub, types, ncomp, h, g, nb));
+  }
   __syncthreads();
   int nit = 0;
+  double best;
   for (int it = 1; it <= maxiter; ++it) {
     nit = it;
-    const int bi = blk_argmin(en, P, redv, redi);
+    const int bi = blk_argmin(en, P, redv, redi, &e0s, best);
     if (tid == 0) Fs = 0.5 + 0.5 * rng.uni();
     __syncthreads();
     const double F = Fs;
     // trials and their energies first, selection after (deferred update)
-    for (int i = tid; i < P; i += 256) {
+    for (int k = 0; tid + 256 * k < P; ++k) {
+      const int i = tid + 256 * k;
       int r0, r1;
       do { r0 = (int)(rng.uni() * P); } while (r0 == i);
       do { r1 = (int)(rng.uni() * P); } while (r1 == i || r1 == r0);
@@ -441,35 +471,42 @@ __global__ __launch_bounds__(256) void k_de(
         }
         tr[d] = v;
       }
-      trial_e[i] = de_energy(tr, lb, ub, types, ncomp, h, g, nb);
+      set_own(trial_e, k, de_energy(tr, lb, ub, types, ncomp, h, g, nb));
     }
     __syncthreads();
-    for (int i = tid; i < P; i += 256) {
-      if (trial_e[i] <= en[i]) {
-        en[i] = trial_e[i];
+#pragma unroll
+    for (int k = 0; k < kDeOwn; ++k) {
+      const int i = tid + 256 * k;
+      if (i < P && trial_e[k] <= en[k]) {
+        en[k] = trial_e[k];
         for (int d = 0; d < D; ++d) pop[i * ds + d] = trial[i * ds + d];
       }
     }
     __syncthreads();
     // convergence: std(E) <= tol * |mean(E)|
     double sm = 0.0;
-    for (int i = tid; i < P; i += 256) sm += en[i];
+#pragma unroll
+    for (int k = 0; k < kDeOwn; ++k)
+      if (tid + 256 * k < P) sm += en[k];
     const double mean = blk_sum(sm, red) / P;
     double sq = 0.0;
-    for (int i = tid; i < P; i += 256) {
-      const double dv = en[i] - mean;
-      sq += dv * dv;
+#pragma unroll
+    for (int k = 0; k < kDeOwn; ++k) {
+      if (tid + 256 * k < P) {
+        const double dv = en[k] - mean;
+        sq += dv * dv;
+      }
     }
     const double sd = sqrt(blk_sum(sq, red) / P);
     if (tid == 0) stop_s = (sd <= tol * fabs(mean)) ? 1 : 0;
     __syncthreads();
     if (stop_s) break;
   }
-  const int bi = blk_argmin(en, P, redv, redi);
+  const int bi = blk_argmin(en, P, redv, redi, &e0s, best);
   if (tid == 0) {
     for (int d = 0; d < kDeMaxD; ++d)
       opar[d] = d < D ? lb[d] + pop[bi * ds + d] * (ub[d] - lb[d]) : 0.0;
-    out_fun[(int64_t)s * ncand + ci] = en[bi];
+    out_fun[(int64_t)s * ncand + ci] = best;
     out_nit[(int64_t)s * ncand + ci] = nit;
   }
 }
