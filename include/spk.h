@@ -237,7 +237,10 @@ int spk_vgfit_evg_dev(spk_ctx *ctx, const int32_t *pi, const int32_t *pj,
  * bounds the sills at 2 var).  params: device [T x n_cands x 8] as (range,
  * sill) per component; fun, nit: device [T x n_cands].  range_ub is the
  * upper range bound (1e9 in vgs.py:747); popmul 50 and tol 0.01 are the
- * reference's settings. */
+ * reference's settings.  A candidate with D = 2 n_comp parameters evolves
+ * min(popmul D^2, 800) members; one workgroup per (step, candidate), whose
+ * LDS (16 bytes per member and parameter of the launch's largest
+ * candidate) sets how many share a CU. */
 int spk_vgfit_de_dev(spk_ctx *ctx, const double *bin_h, const double *bin_g,
                      const int32_t *nbins, int max_bins, const double *var,
                      int64_t T, const int32_t *cands, int n_cands, int maxiter,
