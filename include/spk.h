@@ -148,8 +148,17 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 #define SPK_ST_MEANFILL  2   /* interp flag off or nugget-only: mean fill  */
 #define SPK_ST_FAILED    4   /* factorisation failed: nearest-station fill */
 #define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
-#define SPK_ST_ILLCOND  16   /* condition estimate > 1e12: results are set
-                                by rounding (as the reference's pinv is) */
+#define SPK_ST_ILLCOND  16   /* results are set by rounding at the 1e-6
+                                level (as the reference's pinv's are):
+                                condition estimate > 1e12, or a residual
+                                check failed.  SPK_P_FP64: some cell's
+                                |sum of weights - 1| > 1e-9 (the sum is
+                                exactly one in exact arithmetic).
+                                SPK_P_MIXED / FP64B: the FP64 refinement
+                                step of the dual vectors changed some
+                                cell's estimate by more than 1e-7 of its
+                                magnitude (at least 1e-3 of the step's
+                                mean |value|). */
 #define SPK_ST_FP64VAR  32   /* SPK_P_MIXED: the variance contraction ran
                                 in FP64: condition estimate above a
                                 caller's int8 limit (spk_opts.int8var_kappa),
@@ -273,7 +282,10 @@ int spk_vgfit_de_seeded_dev(spk_ctx *ctx, const double *bin_h,
  * sill on the diagonal, plus 1e-6 of the sill sum as jitter), xi standard
  * normal from a counter-based generator keyed on (seed, t, station rank).
  * out [T x N]: NaN at masked stations.  info [T]: SPK_ST_* (FAILED when
- * the Cholesky failed; the step's row stays NaN). */
+ * the Cholesky failed; the step's row stays NaN).  Returns SPK_EARG on a
+ * context that has run spk_vgfit_pairs_dev: that call order stalled on the
+ * GPU with no cause found, so it is refused (simulate first, or use a
+ * second context). */
 int spk_simulate_fields_dev(spk_ctx *ctx, const spk_problem *prob, int64_t T,
                             const double *mask_vals, const spk_vg *vgs,
                             const double *means, uint64_t seed, double *out,

@@ -453,6 +453,82 @@ def krige_solve(K, R, z, dst_ref_dists, want_var):
     return est, var, ok
 
 
+def _refined_solve(C, B, its=2):
+    """X = C^-1 B by LU (float64) with `its` steps of iterative refinement
+    whose residuals B - C X are formed in long double (64-bit mantissa), so
+    that X is accurate to float64 rounding while kappa(C) u_64 << 1."""
+    import scipy.linalg as sla
+    lu = sla.lu_factor(C)
+    X = sla.lu_solve(lu, B)
+    Cl = C.astype(np.longdouble)
+    Bl = B.astype(np.longdouble)
+    for _ in range(its):
+        r = (Bl - Cl @ X.astype(np.longdouble)).astype(np.float64)
+        X = (X.astype(np.longdouble) +
+             sla.lu_solve(lu, r).astype(np.longdouble)).astype(np.float64)
+    return X
+
+
+def krige_exact(method, vg_str, ref_ref_dists, dst_ref_dists, z,
+                ref_drfts=None, dst_drfts=None, min_vg_val=0.0,
+                want_var=False, known_mean=None, its=2):
+    """The exact solution of the reference's kriging system of one step
+    (interp/steps.py:181-234, 403-435) without its per-cell sum-of-weights
+    test: the float64-rounded answer that pinv (steps.py:351) approximates.
+
+    For OK / EDK the semivariogram matrix G is turned into C = c0 - G
+    (c0 = the sill sum for bounded models, else 2 max|G| + |sill sum|).
+    Because the weights sum to one, G lambda = c0 1 - C lambda, so the
+    bordered system [[G, F], [F^T, 0]] [lambda; mu] = [g; f] (F = [1 | S],
+    f = [1; s_cell]) becomes C lambda = k + F mu, F^T lambda = f with
+    k = c0 - g: lambda = C^-1 k + W mu, mu = B^-1 (f - F^T C^-1 k),
+    W = C^-1 F, B = F^T W.  OK variance (steps.py:432-434):
+    lambda.g + 2 mu = c0 - lambda.k + 2 mu.  SK: lambda = Cov^-1 cov
+    (steps.py:218-219); with known_mean m: m + lambda.(z - m).
+    Every C^-1 is _refined_solve (long-double residuals), so the result is
+    exact to float64 rounding for kappa(C) up to about 1e15, where float64
+    pinv / solve lose kappa(C) u digits.  Returns (est [C], var [C] or
+    None, lambda_sum [C])."""
+    n = z.size
+    if method in ('SK', 'SKM'):
+        Cm = np.empty((n, n))
+        fill_vg_var_arr(ref_ref_dists, Cm, 1, 1, vg_str, min_vg_val)
+        k = np.empty(dst_ref_dists.shape)
+        fill_vg_var_arr(dst_ref_dists, k, 1, 0, vg_str, min_vg_val)
+        lam = _refined_solve(Cm, k.T, its).T
+        if method == 'SKM' or known_mean is not None:
+            m = z.mean() if known_mean is None else known_mean
+            return m + lam @ (z - m), None, lam.sum(1)
+        return lam @ z, None, lam.sum(1)
+    G = np.empty((n, n))
+    fill_vg_var_arr(ref_ref_dists, G, 0, 1, vg_str, min_vg_val)
+    g = np.empty(dst_ref_dists.shape)
+    fill_vg_var_arr(dst_ref_dists, g, 0, 0, vg_str, min_vg_val)
+    comps = parse_vg(vg_str)
+    bounded = all(nm not in ('Rng', 'Pow', 'Hol') for nm, _, _ in comps)
+    ssum = sum(s for _, s, _ in comps)
+    c0 = ssum if bounded else 2 * np.abs(G).max() + abs(ssum)
+    C = c0 - G
+    k = c0 - g                                       # [cells x n]
+    F = np.ones((n, 1))
+    f = np.ones((k.shape[0], 1))
+    if method == 'EDK':
+        F = np.concatenate([F, np.asarray(ref_drfts, np.float64)], axis=1)
+        f = np.concatenate([f, np.asarray(dst_drfts, np.float64).T], axis=1)
+    elif method != 'OK':
+        raise NotImplementedError(method)
+    X = _refined_solve(C, np.concatenate([k.T, F], axis=1), its)
+    Ck, W = X[:, :k.shape[0]], X[:, k.shape[0]:]
+    B = F.T @ W
+    mu = np.linalg.solve(B, (f - Ck.T @ F).T).T      # [cells x p]
+    lam = Ck.T + mu @ W.T                            # [cells x n]
+    est = lam @ z
+    var = None
+    if want_var and method == 'OK':
+        var = c0 - (lam * k).sum(1) + 2.0 * mu[:, 0]
+    return est, var, lam.sum(1)
+
+
 def get_interp(ref_data, interp_type, ref_drfts, dst_drfts, idw_exp, models,
                interp_steps_flags, nuggetness_flags, dst_ref_dists,
                ref_ref_dists, min_vg_val, want_var, prblm=None,

@@ -53,6 +53,17 @@ struct Mfma<float> {
   __device__ static constexpr int row(int g, int r) { return 4 * g + r; }
 };
 
+// |sum of weights - 1| of a cell above which the plain path flags the step
+// SPK_ST_ILLCOND (the sum is exactly one for OK / EDK).  On hourly steps
+// the plain path's error against an exact solution stays below 450 times
+// this deviation, so unflagged steps are within 5e-7.
+constexpr double kSumTol = 1e-9;
+// Relative change of a cell's estimate by the last refinement step above
+// which the step is flagged SPK_ST_ILLCOND.  On hourly steps the change
+// is within a factor 3 of the dual form's error against an exact solution
+// (long-double residuals), so unflagged steps stay well inside 1e-6.
+constexpr double kRefineTol = 1e-7;
+
 // np.isclose(x, 1.0) with rtol 1e-5, atol 1e-8 (interp/steps.py:418)
 __device__ inline bool isclose1(double x) {
   const double tol = 1e-8 + 1e-5 * 1.0;
@@ -139,7 +150,7 @@ template <typename T, bool PLAIN, int BK>
 __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
     const T *__restrict__ A, int64_t lda, int64_t strideA,
     const T *__restrict__ B, int64_t ldb, int64_t strideB,
-    const StepSc *__restrict__ sc, int p, int method, int want_var, int skm,
+    StepSc *__restrict__ sc, int p, int method, int want_var, int skm,
     const double *__restrict__ zact, int64_t ldz,
     const int32_t *__restrict__ nnb, int64_t ldn,
     const double *__restrict__ abuf, int64_t ldab, int64_t Cs,
@@ -390,11 +401,16 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
   if (lg != 0) return;
   double *eo = est + (int64_t)sl * ldo;
   double *vo = var ? var + (int64_t)sl * ldo : nullptr;
+  bool ill = false;
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj) {
     const int64_t col = c0 + wave * 32 + tj * 16 + li;
     if (col >= Cs) continue;
     if (PLAIN) {
+      // residual check: the weights of an OK / EDK system sum to one
+      // exactly, so their sum's distance from one is rounding of the
+      // explicit inverse
+      if (method != SPK_M_SK && fabs(acc_l[tj] - 1.0) > kSumTol) ill = true;
       if (skm) {
         // SK with the step mean m as known mean: m + lambda^T (z - m)
         eo[col] = acc_e[tj] + st.mean * (1.0 - acc_l[tj]);
@@ -410,6 +426,9 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
       qpart[((int64_t)sl * nrb + rbi) * ldab + col] = acc_v[tj];
     }
   }
+  if (PLAIN && __ballot(ill) != 0ull && li == 0 &&
+      !(st.flags & SPK_ST_ILLCOND))
+    atomicOr(&sc[sl].flags, SPK_ST_ILLCOND);
 }
 
 // OK variance lambda.g + 2 mu (interp/steps.py:432-434): the FP64 base
@@ -718,7 +737,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ zact, int64_t ldz,
-    const StepSc *__restrict__ sc, const spk_vg *__restrict__ vgs,
+    StepSc *__restrict__ sc, const spk_vg *__restrict__ vgs,
     const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
     int64_t C, const double *__restrict__ cell_drift,
     const int32_t *__restrict__ perm, const int32_t *__restrict__ nrst,
@@ -737,7 +756,9 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   const int32_t *my = idx + (int64_t)sl * N;
   const double *alpha = V + (int64_t)sl * ldv;
   const double *w = alpha + ld;
+  const double *dal = w + ld;   // refinement correction of alpha (k_dual)
   __shared__ double stx[kChunk], sty[kChunk], sal[kChunk], sw[kChunk];
+  __shared__ double sdal[kChunk];
   __shared__ double sG[kSlots][kChunk];
   __shared__ int slot_j[kSlots];
   __shared__ int n_slots;
@@ -941,6 +962,8 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   double best = INFINITY;
   int bi = 0;
   double e = 0.0, a = 0.0;
+  // the estimate's change by the last refinement step
+  double ke = 0.0;
   // delta_i = k_i - sum_q (c0 - G_{i,J_q}) beta_q
   for (int base = 0; base < n; base += kChunk) {
     __syncthreads();
@@ -950,6 +973,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
       sty[tid] = sy[g];
       sal[tid] = alpha[base + tid];
       sw[tid] = w[base + tid];
+      sdal[tid] = dal[base + tid];
     }
     const int cnt = min(kChunk, n - base);
     if (var_on) {
@@ -1027,6 +1051,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
               const double k = c0 - g;
               e += k * sal[j];
               a += k * sw[j];
+              ke += k * sdal[j];
               double delta = k;
 #pragma unroll
               for (int q = 0; q < kJ; ++q)
@@ -1064,6 +1089,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
           const double k = cflag ? g : c0 - g;
           e += k * sal[j];
           a += k * sw[j];
+          ke += k * sdal[j];
         }
       }
     }
@@ -1086,23 +1112,42 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   const int64_t oc = perm ? (int64_t)perm[c] : c;
   double *eo = est + (int64_t)sl * ldo;
   const double *z = zact + (int64_t)sl * ldz;
+  // Residual check: the last refinement step changed this cell's estimate
+  // by de.  It is of the size of the error left before it, and the refined
+  // result is at least as good, so a change above kRefineTol of the cell's
+  // scale (its estimate, at least 1e-3 of the step's mean |value|) marks
+  // the step ill-conditioned: no FP64 solve of it is reproducible to 1e-6.
+  // (The variance is far less sensitive: in the dual form its error stays
+  // near 1e-9 where the estimate's reaches 1e-6.)
+  double de = ke;
+  double ev;
   if (method == SPK_M_OK) {
-    eo[oc] = e + stp->a0;
+    ev = e + stp->a0;
+    de += stp->dh[0];
+  } else if (method == SPK_M_EDK) {
+    ev = e + stp->h[0];
+    de += stp->dh[0];
+    for (int k = 0; k < m; ++k) {
+      const double f = cell_drift[(int64_t)k * C + oc];
+      ev += stp->h[1 + k] * f;
+      de += stp->dh[1 + k] * f;
+    }
+  } else if (skm) {
+    // SK with the step mean m as known mean: m + k^T C^-1 (z - m 1)
+    // = k.alpha + m (1 - k.w)
+    ev = e + stp->mean * (1.0 - a);
+  } else {  // the reference's SK
+    ev = isclose1(a) ? e : z[bi];
+  }
+  const bool ill = fabs(de) > kRefineTol * fmax(fabs(ev), 1e-3 * stp->zabs);
+  if (ill && !(stp->flags & SPK_ST_ILLCOND))
+    atomicOr(&sc[sl].flags, SPK_ST_ILLCOND);
+  eo[oc] = ev;
+  if (method == SPK_M_OK) {
     if (var_on)
       abuf[(int64_t)sl * ldk + c] =
           c0 - qloc + (a - 1.0) * (a - 2.0) / stp->b;
     if (var_on && dmax) dmax[(int64_t)sl * ldk + c] = dm;
-  } else if (method == SPK_M_EDK) {
-    double v = e + stp->h[0];
-    for (int k = 0; k < m; ++k)
-      v += stp->h[1 + k] * cell_drift[(int64_t)k * C + oc];
-    eo[oc] = v;
-  } else if (skm) {
-    // SK with the step mean m as known mean: m + k^T C^-1 (z - m 1)
-    // = k.alpha + m (1 - k.w)
-    eo[oc] = e + stp->mean * (1.0 - a);
-  } else {  // the reference's SK
-    eo[oc] = isclose1(a) ? e : z[bi];
   }
 }
 
@@ -1597,11 +1642,11 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(get("sc", S * sizeof(StepSc), (void **)&q.sc));
     SPK_TRY(get("A", S * strideM * 8, (void **)&q.A));
     SPK_TRY(get("Li", S * strideM * 8, (void **)&q.Li));
-    SPK_TRY(get("T0", S * ld * 2 * (kMaxP + 1) * 8, (void **)&q.T0));
+    SPK_TRY(get("T0", S * ld * kT0Cols * 8, (void **)&q.T0));
     if (!batched) {
       SPK_TRY(get("M", S * strideM * 8, (void **)&q.M));
     } else {
-      SPK_TRY(get("V", S * 2 * ld * 8, (void **)&q.V));
+      SPK_TRY(get("V", S * 4 * ld * 8, (void **)&q.V));
       if (want_var) SPK_TRY(get("Gm", S * strideM * 8, (void **)&q.Gm));
       if (prec == SPK_P_MIXED && want_var) {
         SPK_TRY(get("L32", S * strideM * 4, (void **)&q.L32));
@@ -1799,8 +1844,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         ProfScope ps(ctx, "dual", sf);
         hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
                            strideM, q.sc, q.zact, ldz, q.idx, N,
-                           prob->stn_drift, m, method, q.V, 2 * ld, q.T0, gb,
-                           mvg, fp64var_thr, fp64var_thr_gau, int8var_thr);
+                           prob->stn_drift, m, method, q.V, 4 * ld, q.T0, gb,
+                           mvg, fp64var_thr, fp64var_thr_gau, int8var_thr,
+                           (const double *)sx, (const double *)sy);
       }
       if (q.L8) {
         ProfScope ps(ctx, "slice_rows", sf);
@@ -1883,7 +1929,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              (perm ? pcy : cy) + cb, cn, C, cd,
                              perm ? perm + cb : (const int32_t *)nullptr,
                              (const int32_t *)nrst, ldk, q.V,
-                             2 * ld, ld, method, m, mvg, want_var ? 1 : 0,
+                             4 * ld, ld, method, m, mvg, want_var ? 1 : 0,
                              prec == SPK_P_MIXED ? 0 : 1, skm, K32, K64, ldk,
                              q.Gm,
                              abuf, perm ? eb : eb + cb, C, dmax, nzb,

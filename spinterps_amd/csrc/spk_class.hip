@@ -250,7 +250,7 @@ int spk_krige_class(spk_ctx *ctx, int kind, const char *model,
   SPK_TRY(ws_get(ctx, "cls_A", slab * 8, (void **)&A));
   SPK_TRY(ws_get(ctx, "cls_Li", slab * 8, (void **)&Li));
   SPK_TRY(ws_get(ctx, "cls_M", slab * 8, (void **)&Mm));
-  SPK_TRY(ws_get(ctx, "cls_T0", ld * 2 * (kMaxP + 1) * 8, (void **)&T0));
+  SPK_TRY(ws_get(ctx, "cls_T0", ld * kT0Cols * 8, (void **)&T0));
   SPK_TRY(ws_get(ctx, "cls_R", Cr * P * 8, (void **)&R));
   SPK_TRY(ws_get(ctx, "cls_lam", Cr * P * 8, (void **)&lam));
   SPK_TRY(ws_get(ctx, "cls_K", (int64_t)P * P * 8, (void **)&K));

@@ -29,11 +29,11 @@ __global__ __launch_bounds__(256) void k_compact(
   int32_t *my_idx = idx + (int64_t)s * N;
   double *my_z = zact + (int64_t)s * ldz;
   __shared__ int wave_cnt[4];
-  __shared__ double red[256];
+  __shared__ double red[256], red_abs[256];
   __shared__ int red_any[256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int base_off = 0;
-  double sum = 0.0;
+  double sum = 0.0, sabs = 0.0;
   int any = 0;
   for (int64_t b = 0; b < N; b += 256) {
     const int64_t i = (b + tid < N && sorder) ? (int64_t)sorder[b + tid]
@@ -51,6 +51,7 @@ __global__ __launch_bounds__(256) void k_compact(
       my_idx[pos] = (int32_t)i;
       my_z[pos] = v;
       sum += v;
+      sabs += fabs(v);
       any |= (v >= thr) ? 1 : 0;
     }
     const int tot = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
@@ -58,11 +59,13 @@ __global__ __launch_bounds__(256) void k_compact(
     base_off += tot;
   }
   red[tid] = sum;
+  red_abs[tid] = sabs;
   red_any[tid] = any;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (tid < w) {
       red[tid] += red[tid + w];
+      red_abs[tid] += red_abs[tid + w];
       red_any[tid] |= red_any[tid + w];
     }
     __syncthreads();
@@ -72,6 +75,7 @@ __global__ __launch_bounds__(256) void k_compact(
     StepSc &o = sc[s];
     o.n = n;
     o.mean = n > 0 ? red[0] / n : NAN;
+    o.zabs = n > 0 ? red_abs[0] / n : 0.0;
     int st = 0;
     if (n == 0) {
       st = SPK_ST_NOSTNS;
@@ -94,7 +98,10 @@ __global__ __launch_bounds__(256) void k_compact(
     o.c0 = 0.0;
     o.b = 0.0;
     o.a0 = 0.0;
-    for (int k = 0; k < kMaxP; ++k) o.h[k] = 0.0;
+    for (int k = 0; k < kMaxP; ++k) {
+      o.h[k] = 0.0;
+      o.dh[k] = 0.0;
+    }
   }
 }
 
@@ -492,7 +499,15 @@ __global__ __launch_bounds__(256) void k_trtri(const double *__restrict__ L,
 //        interp/steps.py:418)
 //   EDK: F = [1 | S], U = Li F, W = Li^T U, B = U^T U, h = B^-1 U^T (Li z),
 //        alpha = C^-1 z - W h  ->  est = k.alpha + h.[1; s_cell]
-// Vectors are stored per step in V[s][0:ld] (alpha) and V[s][ld:2 ld] (w).
+// alpha and h solve the bordered system [C F; F^T 0] [alpha; h] = [z; 0].
+// One step of iterative refinement follows, with FP64 residuals against
+// the covariances the Cholesky factorised (recomputed from the
+// coordinates): r1 = z - C alpha - F h, r2 = -F^T alpha, rw = 1 - C w;
+// dh = B^-1 (F^T C^-1 r1 - r2), dalpha = C^-1 r1 - W dh, dw = C^-1 rw.
+// The corrections are added, and kept: k.dalpha + f.dh is the change of a
+// cell's estimate, which the cell kernel uses as its error indicator.
+// Vectors are stored per step in V[s][0:ld] (alpha), V[s][ld:2 ld] (w),
+// V[s][2 ld:3 ld] (dalpha) and V[s][3 ld:4 ld] (dw); dh in the StepSc.
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -583,7 +598,8 @@ __global__ __launch_bounds__(256) void k_dual(
     const double *__restrict__ stn_drift, int m, int method,
     double *__restrict__ V, int64_t ldv, double *__restrict__ T0,
     const spk_vg *__restrict__ vgs, double min_vg_val, double fp64var_thr,
-    double fp64var_thr_gau, double int8var_thr) {
+    double fp64var_thr_gau, double int8var_thr,
+    const double *__restrict__ sx, const double *__restrict__ sy) {
   const int64_t ldt = ld;
   const int s = blockIdx.x;
   if (sc[s].status) return;
@@ -591,11 +607,16 @@ __global__ __launch_bounds__(256) void k_dual(
   const double *L = Li + (int64_t)s * stride;
   const double *z = zact + (int64_t)s * ldz;
   const int32_t *my = idx + (int64_t)s * N;
-  // per-step work area: 2 (kMaxP + 1) columns of ld
-  double *X = T0 + (int64_t)s * ldt * 2 * (kMaxP + 1);
+  // per-step work area: kT0Cols columns of ld
+  double *X = T0 + (int64_t)s * ldt * kT0Cols;
   double *U = X + ldt * (kMaxP + 1);
+  double *R2 = U + ldt * (kMaxP + 1);  // residuals r1, rw
+  double *Y2 = R2 + 2 * ldt;           // Li [r1 rw]
+  double *Z2 = Y2 + 2 * ldt;           // C^-1 [r1 rw]
   double *alpha = V + (int64_t)s * ldv;
   double *w = alpha + ldt;
+  double *dalpha = w + ldt;
+  double *dw = dalpha + ldt;
   const int p = (method == SPK_M_EDK) ? 1 + m : 1;
   const int tid = threadIdx.x;
   // Condition estimate kappa = lambda_max(C) lambda_max(C^-1) by power
@@ -685,8 +706,9 @@ __global__ __launch_bounds__(256) void k_dual(
   }
   __syncthreads();
   __shared__ double hcoef[kMaxP];
+  __shared__ double Bi[kMaxP * kMaxP];
   if (tid == 0) {
-    double B[kMaxP * kMaxP], Bi[kMaxP * kMaxP];
+    double B[kMaxP * kMaxP];
     for (int a = 0; a < p; ++a)
       for (int b = 0; b < p; ++b) B[a * p + b] = G[a * (p + 1) + b];
     if (method == SPK_M_SK) {
@@ -700,10 +722,6 @@ __global__ __launch_bounds__(256) void k_dual(
         hcoef[a] = v;
       }
     }
-    StepSc &o = sc[s];
-    o.b = G[0];  // 1^T C^-1 1
-    for (int a = 0; a < p; ++a) o.h[a] = hcoef[a];
-    o.a0 = (method == SPK_M_SK) ? 0.0 : hcoef[0];
   }
   __syncthreads();
   for (int i = tid; i < n; i += 256) {
@@ -712,6 +730,117 @@ __global__ __launch_bounds__(256) void k_dual(
       for (int k = 0; k < p; ++k) a -= X[i + (int64_t)k * ldt] * hcoef[k];
     alpha[i] = a;
     w[i] = X[i];
+  }
+  __syncthreads();
+
+  // ---- one step of iterative refinement with FP64 residuals ----------------
+  // r1 = z - C alpha - F h, rw = 1 - C w, with C the covariances the
+  // Cholesky factorised: c0 - gamma (OK, EDK) or the covariance sum (SK),
+  // recomputed from the re-centred coordinates with the same arithmetic.
+  {
+    __shared__ double stx[256], sty[256], sal[256], sw[256];
+    __shared__ spk_vg vg;
+    if (tid == 0) vg = vgs[s];
+    const bool sk = method == SPK_M_SK;
+    const double c0 = sc[s].c0;
+    for (int i0 = 0; i0 < n; i0 += 256) {
+      const int i = i0 + tid;
+      double xi = 0.0, yi = 0.0;
+      if (i < n) {
+        xi = sx[my[i]];
+        yi = sy[my[i]];
+      }
+      double ca = 0.0, cw = 0.0;
+      for (int j0 = 0; j0 < n; j0 += 256) {
+        __syncthreads();
+        if (j0 + tid < n) {
+          const int g = my[j0 + tid];
+          stx[tid] = sx[g];
+          sty[tid] = sy[g];
+          sal[tid] = alpha[j0 + tid];
+          sw[tid] = w[j0 + tid];
+        }
+        __syncthreads();
+        const int cnt = min(256, n - j0);
+        if (i < n)
+          for (int j = 0; j < cnt; ++j) {
+            const double d = dist2(xi, yi, stx[j], sty[j]);
+            const double e = vg_entry(vg, d, sk ? 1 : 0, min_vg_val);
+            const double c = sk ? e : c0 - e;
+            ca += c * sal[j];
+            cw += c * sw[j];
+          }
+      }
+      if (i < n) {
+        double r1 = z[i] - ca;
+        if (!sk) {
+          r1 -= hcoef[0];
+          for (int k = 1; k < p; ++k)
+            r1 -= stn_drift[(int64_t)my[i] * m + (k - 1)] * hcoef[k];
+        }
+        R2[i] = r1;
+        R2[i + ldt] = 1.0 - cw;
+      }
+    }
+    __syncthreads();
+    tri_lower_mv(L, ld, n, R2, Y2, 2);  // Li [r1 rw]
+    __syncthreads();
+    tri_lower_tmv(L, ld, n, Y2, Z2, 2);  // C^-1 [r1 rw]
+    __syncthreads();
+    // gF[a] = U_F[:, a] . (Li r1) = F^T C^-1 r1, r2[a] = -F[:, a] . alpha
+    __shared__ double gF[kMaxP], r2s[kMaxP], dhs[kMaxP];
+    if (!sk) {
+      for (int a = tid >> 6; a < 2 * p; a += 4) {
+        double acc = 0.0;
+        const int col = a < p ? a : a - p;
+        for (int i = (tid & 63); i < n; i += 64) {
+          if (a < p) {
+            acc += U[i + (int64_t)col * ldt] * Y2[i];
+          } else {
+            const double f =
+                col == 0 ? 1.0 : stn_drift[(int64_t)my[i] * m + (col - 1)];
+            acc -= f * alpha[i];
+          }
+        }
+        acc = wave_sum(acc);
+        if ((tid & 63) == 0) {
+          if (a < p)
+            gF[col] = acc;
+          else
+            r2s[col] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int a = 0; a < kMaxP; ++a) dhs[a] = 0.0;
+      if (!sk)
+        for (int a = 0; a < p; ++a) {
+          double v = 0.0;
+          for (int b = 0; b < p; ++b) v += Bi[a * p + b] * (gF[b] - r2s[b]);
+          dhs[a] = v;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) {
+      double da = Z2[i];
+      if (!sk)
+        for (int k = 0; k < p; ++k) da -= X[i + (int64_t)k * ldt] * dhs[k];
+      const double dwi = Z2[i + ldt];
+      alpha[i] += da;
+      w[i] += dwi;
+      dalpha[i] = da;
+      dw[i] = dwi;
+    }
+    if (tid == 0) {
+      StepSc &o = sc[s];
+      o.b = G[0];  // 1^T C^-1 1
+      for (int a = 0; a < p; ++a) {
+        o.h[a] = hcoef[a] + dhs[a];
+        o.dh[a] = dhs[a];
+      }
+      o.a0 = sk ? 0.0 : hcoef[0] + dhs[0];
+    }
   }
 }
 
@@ -808,8 +937,8 @@ __global__ __launch_bounds__(256) void k_minv(
   }
   if (method == SPK_M_SK) return;
   const int p = (method == SPK_M_EDK) ? 1 + m : 1;
-  // per-step work area: 2 (kMaxP + 1) columns of ld (W, then Q)
-  double *W = T0 + (int64_t)s * ldt * 2 * (kMaxP + 1);
+  // per-step work area: kT0Cols columns of ld (W, then Q)
+  double *W = T0 + (int64_t)s * ldt * kT0Cols;
   // W = Cinv F
   for (int i = tid; i < n; i += 256) {
     double acc[kMaxP];

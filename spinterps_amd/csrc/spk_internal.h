@@ -53,6 +53,10 @@ struct spk_ctx {
   std::vector<std::string> stage_names;
   int n_marks = 0;
   std::vector<double> last_ms;
+  // set by spk_vgfit_pairs_dev: spk_simulate_fields_dev refuses to run on
+  // a context that has sorted station pairs (that order stalled with no
+  // cause found; simulations before any pair sort are unaffected)
+  bool pairs_sorted = false;
 };
 
 namespace spk {

@@ -6,6 +6,10 @@
 namespace spk {
 
 constexpr int kMaxP = 8;  // constraint rows: 1 + up to 7 drifts
+// Columns of ld doubles per step in the factor work array T0: the
+// right-hand sides [F z] and their images (2 (kMaxP + 1)), then six for
+// the refinement's residuals and their images.
+constexpr int kT0Cols = 2 * (kMaxP + 1) + 6;
 
 // Per-step scalars kept on the device.
 struct StepSc {
@@ -18,7 +22,9 @@ struct StepSc {
   double c0;      // covariance shift of the OK/EDK system
   double b;       // 1^T C^-1 1
   double a0;      // constant term of the dual-form estimate
+  double zabs;    // mean |value| of the active stations
   double h[kMaxP];
+  double dh[kMaxP];  // last refinement correction of h (dual form)
 };
 
 __global__ void k_compact(const double *vals, int64_t N, int32_t *idx,
@@ -41,7 +47,8 @@ __global__ void k_dual(const double *Lc, const double *Li, int64_t ld,
                        int m, int method, double *V, int64_t ldv, double *T0,
                        const spk_vg *vgs, double min_vg_val,
                        double fp64var_thr, double fp64var_thr_gau,
-                       double int8var_thr);
+                       double int8var_thr, const double *sx,
+                       const double *sy);
 __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const int32_t *idx, int64_t N,
                        const double *stn_drift, int m, int method, double *M,

@@ -413,6 +413,12 @@ int spk_simulate_fields_dev(spk_ctx *ctx, const spk_problem *prob, int64_t T,
     return set_error(SPK_EARG, "null argument");
   const int64_t N = prob->n_stns;
   if (N < 2) return set_error(SPK_ESHAPE, "need stations");
+  if (ctx->pairs_sorted)
+    return set_error(SPK_EARG,
+                     "field simulation after a station-pair sort on the same "
+                     "context is disabled (that order stalled on the GPU with "
+                     "no cause found): simulate first, or use another "
+                     "context");
   if (T <= 0) return SPK_OK;
   SPK_HIP(hipSetDevice(ctx->device));
   hipStream_t sm = ctx->stream;

@@ -654,6 +654,7 @@ int spk_vgfit_pairs_dev(spk_ctx *ctx, const double *x, const double *y,
                                              (int)M, 0, 64, ctx->stream));
   hipLaunchKernelGGL(k_pair_unpack, dim3(g), dim3(256), 0, ctx->stream, k1, v1,
                      M, pi, pj, pd);
+  ctx->pairs_sorted = true;
   return check_launch("k_pair_unpack");
 }
 

@@ -1,0 +1,281 @@
+"""The workloads at their stated sizes against the oracle (GPU).
+
+small  : 100 stations, 50 x 50 cells at 2 km, 24 steps (all of it).
+daily  : 1000 stations, 500 x 500 cells at 1 km; OK (+ variance), SK, SKM,
+         EDK with the DEM drift and IDW p = 2 on sampled cells of 3 steps.
+hourly : 2000 stations, 1000 x 1000 cells at 500 m; steps chosen to span
+         the condition number of their covariance matrix from 2 to 1e12
+         (Sph, Exp, Gau), every path against the exact solution of the
+         reference's system (oracle.krige_exact) on sampled cells.
+stress : daily stations with 10 % in 1-10 m pairs, Gaussian model with
+         nugget 1e-6 (condition numbers 1e8 to singular).
+
+Tolerances.  Relative errors use a floor of 1e-3 x the step's mean |value|
+(estimates) and 1e-3 x the step's mean variance (variances).  A step the
+library does not flag SPK_ST_ILLCOND must be within 1e-6 of the exact
+solution on every path; the plain FP64 path and the batched FP64 path
+within 1e-9 where the condition estimate is at most 1e5 (there the
+reference's own pinv is within 1e-10 of the exact solution).  Flagged
+steps are reported, and the reference's pinv error on them is printed
+beside the library's.
+"""
+import numpy as np
+import pytest
+
+from oracle import spk_oracle as O
+from spinterps_amd import _lib, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def _sample(C, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(C, k, replace=False))
+
+
+# ---- small (cfg1): the whole workload --------------------------------------
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp64b', 'mixed'])
+def test_small_full_workload(gpu, precision):
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.small()
+    vals, vgs = configs.small_steps(w)
+    assert vals.shape == (24, 100) and w.n_cells == 2500
+    _, ref, _ = O.interp_all(w.stn_x, w.stn_y, w.cell_x, w.cell_y, vals, vgs,
+                             [('OK', 'OK', None), ('IDW', 'IDW', 2.0)])
+    prob = DeviceProblem.from_workload(w)
+    e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=precision)
+    e, v = e.cpu().numpy(), v.cpu().numpy()
+    assert not (info.cpu().numpy() & _lib.ST_ILLCOND).any()
+    fe = 1e-3 * np.abs(ref['OK']).mean(axis=1, keepdims=True)
+    fv = 1e-3 * np.abs(ref['EST_VARS_OK']).mean(axis=1, keepdims=True)
+    re = _rel(e, ref['OK'], fe).max()
+    rv = _rel(v, ref['EST_VARS_OK'], fv).max()
+    print(f'small {precision}: est {re:.2e} var {rv:.2e}')
+    assert re <= 1e-9
+    assert rv <= (1e-9 if precision != 'mixed' else 1e-6)
+    if precision == 'fp64':
+        ei, _, _ = interp_steps(prob, vals, None, 'IDW', idw_exp=2.0)
+        np.testing.assert_allclose(ei.cpu().numpy(), ref['IDW'], rtol=1e-12)
+
+
+# ---- daily (cfg2): 1000 stations, full grid ----------------------------------
+
+@pytest.fixture(scope='module')
+def daily_case(gpu):
+    from spinterps_amd.interp.steps import DeviceProblem
+    w = configs.daily()
+    assert w.n_stns == 1000 and w.n_cells == 250000
+    vals, vgs = configs.daily_steps(w, 40, 43)
+    prob = DeviceProblem.from_workload(w)
+    cidx = _sample(w.n_cells, 200, 3)
+    _, ref, prblm = O.interp_all(
+        w.stn_x, w.stn_y, w.cell_x[cidx], w.cell_y[cidx], vals, vgs,
+        [('OK', 'OK', None), ('SK', 'SK', None), ('SKM', 'SKM', None),
+         ('EDK', 'EDK', None), ('IDW', 'IDW', 2.0)],
+        stn_drfts=w.stn_drift, cell_drfts=w.cell_drift[:, cidx])
+    assert not prblm
+    return w, vals, vgs, prob, cidx, ref
+
+
+@pytest.mark.parametrize('method', ['OK', 'SK', 'SKM', 'EDK', 'IDW'])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_daily_1000_stations(daily_case, method, precision):
+    from spinterps_amd.interp.steps import interp_steps
+    w, vals, vgs, prob, cidx, ref = daily_case
+    if method == 'IDW' and precision == 'mixed':
+        pytest.skip('IDW has one path')
+    e, v, info = interp_steps(prob, vals, None if method == 'IDW' else vgs,
+                              method, precision=precision, idw_exp=2.0)
+    e = e.cpu().numpy()[:, cidx]
+    assert not (info.cpu().numpy() & (_lib.ST_ILLCOND | _lib.ST_FAILED)).any()
+    fe = 1e-3 * np.abs(ref[method]).mean(axis=1, keepdims=True)
+    re = _rel(e, ref[method], fe).max()
+    msg = f'daily {method} {precision}: est {re:.2e}'
+    tol = 1e-12 if method == 'IDW' else 1e-9
+    if method == 'OK':
+        v = v.cpu().numpy()[:, cidx]
+        vr = ref['EST_VARS_OK']
+        rv = _rel(v, vr, 1e-3 * np.abs(vr).mean(axis=1, keepdims=True)).max()
+        msg += f' var {rv:.2e}'
+        assert rv <= (1e-9 if precision == 'fp64' else 1e-6), msg
+    print(msg)
+    assert re <= tol, msg
+
+
+# ---- hourly (cfg3): condition numbers 2 .. 1e12 ------------------------------
+
+# (step, model, condition number of C = c0 - G, exact eigenvalues)
+HOURLY_STEPS = [
+    (40, 'Gau', 1.6), (1619, 'Exp', 6.2), (1616, 'Exp', 2.4e1),
+    (1038, 'Gau', 7.4e2), (1601, 'Sph', 1.4e3), (1541, 'Gau', 1.4e4),
+    (1612, 'Sph', 4.7e4), (1036, 'Gau', 2.2e5), (966, 'Gau', 1.7e6),
+    (305, 'Gau', 3.2e7), (973, 'Gau', 5.5e8), (1089, 'Gau', 1.1e10),
+    (1600, 'Gau', 6.0e10), (248, 'Gau', 9.3e11)]
+
+
+@pytest.fixture(scope='module')
+def hourly_case(gpu):
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.hourly()
+    assert w.n_stns == 2000 and w.n_cells == 10 ** 6
+    steps = [s for s, _, _ in HOURLY_STEPS]
+    parts = [configs.hourly_steps(w, s, s + 1) for s in steps]
+    vals = np.concatenate([p[0] for p in parts])
+    vgs = [p[1][0] for p in parts]
+    for (s, m, _), g in zip(HOURLY_STEPS, vgs):
+        assert m in g, (s, g)
+    prob = DeviceProblem.from_workload(w)
+    cidx = _sample(w.n_cells, 100, 7)
+    out = {}
+    for prec in ('fp64', 'fp64b', 'mixed'):
+        import torch
+        kap = torch.zeros(len(steps), dtype=torch.float64, device=prob.device)
+        e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=prec,
+                                  kappa=kap)
+        out[prec] = (e.cpu().numpy()[:, cidx], v.cpu().numpy()[:, cidx],
+                     info.cpu().numpy(), kap.cpu().numpy())
+        del e, v
+    exact, pinv_err = [], []
+    cx, cy = w.cell_x[cidx], w.cell_y[cidx]
+    for t in range(len(steps)):
+        act = ~np.isnan(vals[t])
+        xs, ys, z = w.stn_x[act], w.stn_y[act], vals[t][act]
+        rr, dr = O.dists_2d(xs, ys, xs, ys), O.dists_2d(cx, cy, xs, ys)
+        ex, vx, _ = O.krige_exact('OK', vgs[t], rr, dr, z, want_var=True,
+                                  its=1)
+        exact.append((ex, vx))
+        if HOURLY_STEPS[t][2] < 1e8:     # pinv is within 1e-9 there
+            pinv_err.append((np.nan, np.nan))
+            continue
+        K, R = O.krig_system('OK', vgs[t], rr, dr, None, None, 0.0)
+        ep, vp, _ = O.krige_contract(np.linalg.pinv(K), R, z, dr, True)
+        fe = 1e-3 * np.abs(ex).mean()
+        fv = 1e-3 * np.abs(vx).mean()
+        pinv_err.append((_rel(ep, ex, fe).max(), _rel(vp, vx, fv).max()))
+    return out, exact, pinv_err
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp64b', 'mixed'])
+def test_hourly_condition_sweep_vs_exact(hourly_case, precision):
+    """Every step the path leaves unflagged is within 1e-6 of the exact
+    solution (1e-9 for the FP64 paths up to condition estimate 1e5); the
+    steps at condition numbers 6e10 and 9e11, where the reference's pinv is
+    itself 1e-5 to 1e-4 away from the exact solution, are flagged."""
+    out, exact, pinv_err = hourly_case
+    e, v, info, kap = out[precision]
+    rows = []
+    for t, (step, model, kx) in enumerate(HOURLY_STEPS):
+        ex, vx = exact[t]
+        fe = 1e-3 * np.abs(ex).mean()
+        fv = 1e-3 * np.abs(vx).mean()
+        re = _rel(e[t], ex, fe).max()
+        rv = _rel(v[t], vx, fv).max()
+        ill = bool(info[t] & _lib.ST_ILLCOND)
+        failed = bool(info[t] & _lib.ST_FAILED)
+        rows.append((step, model, kx, kap[t], ill, failed, re, rv,
+                     *pinv_err[t]))
+    print(f'\nhourly {precision}: step model kappa kappa_est flagged failed '
+          'est_err var_err | pinv est_err var_err')
+    for r in rows:
+        print('  %5d %s %8.1e %8.1e %-5s %-5s %8.1e %8.1e | %8.1e %8.1e' % r)
+    for (step, model, kx, ke, ill, failed, re, rv, _, _) in rows:
+        assert not failed, step
+        if not ill:
+            assert re <= 1e-6 and rv <= 1e-6, (step, re, rv)
+            if precision != 'mixed' and ke <= 1e5:
+                assert re <= 1e-9 and rv <= 1e-9, (step, re, rv)
+        if kx >= 6e10:
+            assert ill, (step, kx)
+    n_ill = sum(r[4] for r in rows)
+    print(f'  flagged {n_ill} of {len(rows)}')
+
+
+# ---- stress (cfg4): close station pairs, Gaussian model ----------------------
+
+def test_stress_subset_flags(gpu):
+    """Stress steps against the exact solution: each path's unflagged
+    solved steps are within 1e-6; the rest is reported (flagged, or
+    failed when the Cholesky of a numerically singular matrix fails)."""
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.daily()
+    sx, sy = configs.stress_stations(w)
+    vals, vgs = configs.stress_steps(w, 0, 12)
+    prob = DeviceProblem(sx, sy, w.cell_x, w.cell_y)
+    cidx = _sample(w.n_cells, 100, 11)
+    cx, cy = w.cell_x[cidx], w.cell_y[cidx]
+    rr = O.dists_2d(sx, sy, sx, sy)
+    dr = O.dists_2d(cx, cy, sx, sy)
+    res = {}
+    for prec in ('fp64', 'mixed'):
+        e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=prec)
+        res[prec] = (e.cpu().numpy()[:, cidx], v.cpu().numpy()[:, cidx],
+                     info.cpu().numpy())
+    n_cmp = {p: 0 for p in res}
+    for t in range(len(vgs)):
+        G = np.empty(rr.shape)
+        O.fill_vg_var_arr(rr, G, 0, 1, vgs[t], 0.0)
+        lam_min = np.linalg.eigvalsh(1.0 - G)[0]
+        if lam_min <= 1e-14:        # numerically singular: no exact answer
+            continue
+        ex, vx, _ = O.krige_exact('OK', vgs[t], rr, dr, vals[t],
+                                  want_var=True, its=1)
+        fe, fv = 1e-3 * np.abs(ex).mean(), 1e-3 * np.abs(vx).mean()
+        for p, (e, v, info) in res.items():
+            st = info[t]
+            re = _rel(e[t], ex, fe).max()
+            rv = _rel(v[t], vx, fv).max()
+            print(f'stress step {t} {p}: lambda_min {lam_min:.1e} flags {st} '
+                  f'est {re:.1e} var {rv:.1e}')
+            if not st & (_lib.ST_ILLCOND | _lib.ST_FAILED):
+                n_cmp[p] += 1
+                assert re <= 1e-6 and rv <= 1e-6, (p, t, re, rv)
+    print('unflagged stress steps compared:', n_cmp)
+
+
+# ---- the stall guard -----------------------------------------------------------
+
+def test_simulate_after_pair_sort_refused(gpu):
+    """A context that has sorted station pairs refuses to simulate fields
+    (that order stalled once with no cause found); nothing is launched.
+    Uses a context of its own so the shared one is unaffected."""
+    import ctypes
+
+    import torch
+
+    from spinterps_amd._lib import Context, check, lib
+    from spinterps_amd.interp.steps import DeviceProblem, vg_device
+    ctx = Context(0)
+    try:
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        rng = np.random.default_rng(1)
+        x = rng.uniform(0, 1e4, 40)
+        y = rng.uniform(0, 1e4, 40)
+        dev = torch.device('cuda', 0)
+        xd = torch.as_tensor(x, device=dev)
+        yd = torch.as_tensor(y, device=dev)
+        M = 40 * 39 // 2
+        pi = torch.empty(M, dtype=torch.int32, device=dev)
+        pj = torch.empty(M, dtype=torch.int32, device=dev)
+        pd = torch.empty(M, dtype=torch.float64, device=dev)
+        check(lib().spk_vgfit_pairs_dev(ctx.h, xd.data_ptr(), yd.data_ptr(),
+                                        40, pi.data_ptr(), pj.data_ptr(),
+                                        pd.data_ptr()))
+        torch.cuda.synchronize()
+        assert np.all(np.diff(pd.cpu().numpy()) >= 0)
+        prob = DeviceProblem(x, y, x[:1], y[:1])
+        mask = torch.zeros((2, 40), dtype=torch.float64, device=dev)
+        vg = vg_device(['1.0 Exp(3000.0)'] * 2, dev)
+        mu = torch.zeros(2, dtype=torch.float64, device=dev)
+        out = torch.empty((2, 40), dtype=torch.float64, device=dev)
+        info = torch.zeros(2, dtype=torch.int32, device=dev)
+        pst = prob.c_struct()
+        with pytest.raises(AssertionError, match='pair sort'):
+            check(lib().spk_simulate_fields_dev(
+                ctx.h, ctypes.byref(pst), 2, mask.data_ptr(), vg.data_ptr(),
+                mu.data_ptr(), 5, out.data_ptr(), info.data_ptr()))
+    finally:
+        ctx.close()

@@ -298,3 +298,101 @@ def test_interp_all_small_runs():
     assert f['OK'].dtype == np.float32 and f['OK'].shape == (3, 64)
     assert np.isfinite(f64['OK']).all() and np.isfinite(f64['EST_VARS_OK']).all()
     assert not prb
+
+
+# ---- exact solution of the kriging system (ill-conditioned steps) --------
+
+def _bordered_solve_mp(K, rhs, dps=40):
+    """The reference's bordered system K x = rhs solved with 40 significant
+    digits (mpmath inverse by LU): an algorithm independent of krige_exact's
+    covariance-form shift and of float64 rounding."""
+    import mpmath
+    with mpmath.workdps(dps):
+        A = mpmath.matrix(K.tolist())
+        B = mpmath.matrix(rhs.tolist())
+        X = mpmath.inverse(A) * B
+        return np.array([[float(X[i, j]) for j in range(rhs.shape[1])]
+                         for i in range(K.shape[0])])
+
+
+def _ill_step(n=60, ncell=12, seed=5):
+    """A Gaussian-model OK step with a long range and four stations 2-5 m
+    from another one: condition number of C about 1e11."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1e4, n)
+    y = rng.uniform(0, 1e4, n)
+    x[:4] = x[4:8] + rng.uniform(2, 5, 4)
+    y[:4] = y[4:8]
+    cx = rng.uniform(0, 1e4, ncell)
+    cy = rng.uniform(0, 1e4, ncell)
+    z = rng.lognormal(0, 1, n)
+    vg = '0.01 Nug(0.0) + 1.0 Gau(5000.0)'
+    return x, y, cx, cy, z, vg
+
+
+def test_krige_exact_matches_pinv_when_well_conditioned():
+    rng = np.random.default_rng(2)
+    n = 60
+    x, y = rng.uniform(0, 1e4, n), rng.uniform(0, 1e4, n)
+    cx, cy = rng.uniform(0, 1e4, 30), rng.uniform(0, 1e4, 30)
+    z = rng.gamma(0.8, 5.0, n)
+    rr, dr = O.dists_2d(x, y, x, y), O.dists_2d(cx, cy, x, y)
+    for vg in ('0.2 Nug(0.0) + 1.5 Sph(3000.0)', '1.0 Exp(2500.0)',
+               '0.5 Lin(4000.0) + 0.1 Nug(0.0)', '2.0 Pow(1.2)'):
+        K, R = O.krig_system('OK', vg, rr, dr, None, None, 0.0)
+        e_p, v_p, ok = O.krige_contract(np.linalg.pinv(K), R, z, dr, True)
+        assert ok.all()
+        e_x, v_x, lsum = O.krige_exact('OK', vg, rr, dr, z, want_var=True)
+        np.testing.assert_allclose(e_x, e_p, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(v_x, v_p, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(lsum, 1.0, rtol=0, atol=1e-13)
+
+
+def test_krige_exact_against_long_double_bordered_solve():
+    """On an ill-conditioned step (kappa(C) about 1e11) the float64 pinv
+    loses digits; krige_exact agrees with a 40-digit solve of the bordered
+    system [[G, 1], [1^T, 0]] to 5e-9 where pinv does not (7e-6)."""
+    x, y, cx, cy, z, vg = _ill_step()
+    rr, dr = O.dists_2d(x, y, x, y), O.dists_2d(cx, cy, x, y)
+    K, R = O.krig_system('OK', vg, rr, dr, None, None, 0.0)
+    n = z.size
+    ev = np.linalg.eigvalsh(1.01 - K[:n, :n])
+    assert ev[-1] / ev[0] > 1e10
+    L = _bordered_solve_mp(K, R.T).T                 # [cells x (n+1)]
+    e_ref = L[:, :n] @ z
+    v_ref = (L * R).sum(1) + L[:, n]
+    e_x, v_x, lsum = O.krige_exact('OK', vg, rr, dr, z, want_var=True)
+    fe = 1e-3 * np.abs(e_ref).mean()
+    fv = 1e-3 * np.abs(v_ref).mean()
+    err_e = np.abs(e_x - e_ref) / np.maximum(np.abs(e_ref), fe)
+    err_v = np.abs(v_x - v_ref) / np.maximum(np.abs(v_ref), fv)
+    assert err_e.max() < 5e-9 and err_v.max() < 5e-9
+    e_p, v_p, _ = O.krige_contract(np.linalg.pinv(K), R, z, dr, True)
+    err_p = np.abs(e_p - e_ref) / np.maximum(np.abs(e_ref), fe)
+    assert err_p.max() > 10 * err_e.max()       # pinv is the less exact one
+
+
+def test_krige_exact_edk_reproduces_drift_and_is_shift_invariant():
+    """EDK: sum(lambda s) = s_cell exactly (drift reproduction) and the
+    estimate does not depend on the covariance shift c0 (a nugget-only
+    component adds a constant to every entry of G and changes c0)."""
+    rng = np.random.default_rng(4)
+    n = 50
+    x, y = rng.uniform(0, 1e4, n), rng.uniform(0, 1e4, n)
+    cx, cy = rng.uniform(0, 1e4, 25), rng.uniform(0, 1e4, 25)
+    s = (0.3 * x + 0.1 * y)[:, None] / 1e3
+    sc = (0.3 * cx + 0.1 * cy)[None, :] / 1e3
+    z = 2.0 + 0.5 * s[:, 0] + rng.normal(0, 0.1, n)
+    rr, dr = O.dists_2d(x, y, x, y), O.dists_2d(cx, cy, x, y)
+    vg = '1.0 Sph(4000.0)'
+    e1, _, l1 = O.krige_exact('EDK', vg, rr, dr, z, s, sc)
+    e2, _, _ = O.krige_exact('EDK', '5.0 Nug(0.0) + ' + vg, rr, dr, z, s, sc)
+    np.testing.assert_allclose(l1, 1.0, atol=1e-13)
+    np.testing.assert_allclose(e1, e2, rtol=1e-11, atol=1e-12)
+    # a field exactly linear in the drift is reproduced exactly
+    e3, _, _ = O.krige_exact('EDK', vg, rr, dr, 2.0 + 0.5 * s[:, 0], s, sc)
+    np.testing.assert_allclose(e3, 2.0 + 0.5 * sc[0], rtol=1e-11)
+    K, R = O.krig_system('EDK', vg, rr, dr, s, sc, 0.0)
+    e_p, _, ok = O.krige_contract(np.linalg.pinv(K), R, z, dr, False)
+    assert ok.all()
+    np.testing.assert_allclose(e1, e_p, rtol=1e-10, atol=1e-12)
