@@ -6,7 +6,12 @@ Workload (hourly): 2000 stations over 500 km x 500 km, a 1000 x 1000 grid at
 500 m, per step a random 200-2000 active stations and its own variogram
 '<f v> Nug(0.0) + <(1-f) v> {Sph|Exp|Gau}(5-80 km)'; lognormal values.
 One bench step = one batch of --batch time steps over the full grid through
-the batched mixed path, inputs resident in HBM.  Synthetic seeded data.
+the batched mixed path.  Stations and cells are resident in HBM; the timed
+region copies each step's values and variogram record to the device and
+both fields back as float32 (the reference's output type).  Every timed
+step is checked against the batched FP64 path on sampled cells, and a
+spread of them against the exact solution of the reference's system.
+Synthetic seeded data.
 
 Prints ONE JSON line (rank 0, N = 1).  Progress goes to stderr.
 """
@@ -50,7 +55,13 @@ def main():
     ap.add_argument('--plain-steps', type=int, default=16)
     ap.add_argument('--grid', type=int, default=1000)
     ap.add_argument('--cpu-cells', type=int, default=200)
-    ap.add_argument('--cpu-steps', type=int, default=6)
+    ap.add_argument('--gate-cells', type=int, default=4096,
+                    help='sampled cells of every timed step compared with '
+                         'the batched FP64 path')
+    ap.add_argument('--gate-oracle-steps', type=int, default=12,
+                    help='timed steps (spread) compared with the exact '
+                         'oracle, besides the 4 unflagged steps of largest '
+                         'condition estimate')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--vg-fields', type=int, default=512,
                     help='simulated fields for the variogram-fit leg '
@@ -84,61 +95,90 @@ def main():
               'during compute).  Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
+    import multiprocessing as mp
+
     import numpy as np
+    # CPU baseline workers, forked before anything touches the GPU: one
+    # worker per core of this process's share (at most 16), each with one
+    # BLAS thread, as the reference runs its chunks (interp/main.py:84-98)
+    n_cores = min(16, len(os.sched_getaffinity(0)))
+    pool = None if args.no_cpu else mp.get_context('fork').Pool(n_cores)
     import torch
 
     from spinterps_amd import _lib, configs
-    from spinterps_amd.interp.steps import (DeviceProblem, interp_steps,
-                                            prof_enable, prof_get, prof_work,
-                                            vg_device)
+    from spinterps_amd.interp.steps import (DeviceProblem, clip_cast,
+                                            interp_steps, prof_enable,
+                                            prof_get, prof_work, vg_device)
 
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     dev = torch.device('cuda', 0)
     B, K, W, P = args.batch, args.steps, args.warmup, args.plain_steps
     w = configs.hourly(n_side=args.grid, cell=5e5 / args.grid)
-    C = w.n_cells
-    log(f'hourly workload: N={w.n_stns} C={C} batch={B} steps={K} '
+    C, N = w.n_cells, w.n_stns
+    log(f'hourly workload: N={N} C={C} batch={B} steps={K} '
         f'warmup={W} plain_steps={P}')
     n_tot = (W + K) * B
     t = time.time()
     vals, vgs = configs.hourly_steps(w, 0, n_tot + P)
     log(f'generated {n_tot + P} steps in {time.time() - t:.1f}s')
     prob = DeviceProblem.from_workload(w, device=0)
-    vals_d = torch.as_tensor(vals, device=dev)
-    vg_d = vg_device(vgs, dev)
+    vrec = _lib.VG_DTYPE.itemsize
     n_act = (~np.isnan(vals)).sum(1)
-    # the timed steps run as one call, as a user interpolates a series:
-    # batch b+1's systems are factorised while batch b's cells contract
+    # per-step inputs on the host, pinned: the values [T x N] and the parsed
+    # variogram records; each run copies its steps to the device and the
+    # fields back as the reference's float32 (interp/steps.py:815-831)
+    vals_h = torch.from_numpy(vals).pin_memory()
+    vg_h = vg_device(vgs, torch.device('cpu')).pin_memory()
     rows = max(K, W) * B
+    vals_d = torch.empty((rows, N), dtype=torch.float64, device=dev)
+    vg_dd = torch.empty(rows * vrec, dtype=torch.uint8, device=dev)
     est = torch.empty((rows, C), dtype=torch.float64, device=dev)
     var = torch.empty((rows, C), dtype=torch.float64, device=dev)
-    vrec = _lib.VG_DTYPE.itemsize
+    e32 = torch.empty((rows, C), dtype=torch.float32, device=dev)
+    v32 = torch.empty((rows, C), dtype=torch.float32, device=dev)
+    e32_h = torch.empty((rows, C), dtype=torch.float32, pin_memory=True)
+    v32_h = torch.empty((rows, C), dtype=torch.float32, pin_memory=True)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
 
-    def steps(s0, s1):
-        return (vals_d[s0:s1], vg_d[s0 * vrec:s1 * vrec])
+    def run(s0, s1, kap=None):
+        """Steps s0..s1 as one call, as a user interpolates a series (batch
+        b+1's systems are factorised while batch b's cells contract):
+        H2D of the per-step inputs, OK estimate + variance, float32 cast,
+        D2H of both fields."""
+        m = s1 - s0
+        vals_d[:m].copy_(vals_h[s0:s1], non_blocking=True)
+        vg_dd[:m * vrec].copy_(vg_h[s0 * vrec:s1 * vrec], non_blocking=True)
+        ev0.record()
+        _, _, info = interp_steps(prob, vals_d[:m], None, 'OK',
+                                  precision='mixed', est=est[:m], var=var[:m],
+                                  vg_dev=vg_dd[:m * vrec], kappa=kap)
+        ev1.record()
+        clip_cast(est[:m], out=e32[:m])
+        clip_cast(var[:m], out=v32[:m])
+        e32_h[:m].copy_(e32[:m], non_blocking=True)
+        v32_h[:m].copy_(v32[:m], non_blocking=True)
+        return info
 
     # ---- warmup + timed region (mixed) --------------------------------
     if W:
-        v, g = steps(0, W * B)
-        interp_steps(prob, v, None, 'OK', precision='mixed',
-                     est=est[:W * B], var=var[:W * B], vg_dev=g)
+        run(0, W * B)
         torch.cuda.synchronize()
         log(f'warmup: {W} x {B} steps done')
+    TK = K * B
+    kap_t = torch.zeros(TK, dtype=torch.float64, device=dev)
     prof_enable(0, True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    v, g = steps(W * B, (W + K) * B)
-    kap_t = torch.zeros(K * B, dtype=torch.float64, device=dev)
-    _, _, info = interp_steps(prob, v, None, 'OK', precision='mixed',
-                              est=est[:K * B], var=var[:K * B], vg_dev=g,
-                              kappa=kap_t)
-    infos = [info]
+    info_d = run(W * B, n_tot, kap_t)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / K
     value = B * C / ((t1 - t0) / K)
-    log(f'mixed: {ms_per_step:.1f} ms per batch of {B} steps, '
-        f'{value:.3e} cell-estimates/s')
+    dev_ms = ev0.elapsed_time(ev1)
+    log(f'mixed: {ms_per_step:.1f} ms per batch of {B} steps incl. H2D of '
+        f'the inputs and D2H of the float32 fields, {value:.3e} '
+        f'cell-estimates/s; device only {dev_ms / K:.1f} ms per batch')
     kern = {}
     for nm in ('nearest', 'kgen', 'gemm_tri_f32', 'gemm_tri_f64',
                'slice_cols', 'gemm_tri_i8', 'var_reduce', 'potrf', 'trtri',
@@ -148,10 +188,14 @@ def main():
     macs32 = prof_work('gemm_tri_f32_macs')
     ops8x = prof_work('gemm_tri_i8_ops')
     prof_enable(0, False)
-    info_t = torch.cat(infos).cpu().numpy()
-    timed_n = n_act[W * B:(W + K) * B]
-    solved_t = (info_t & (_lib.ST_NOSTNS | _lib.ST_MEANFILL | _lib.ST_FAILED |
-                          _lib.ST_ONESTN)) == 0
+    info_t = info_d.cpu().numpy()
+    kap_np = kap_t.cpu().numpy()
+    timed_n = n_act[W * B:n_tot]
+    special_t = (info_t & (_lib.ST_NOSTNS | _lib.ST_MEANFILL |
+                           _lib.ST_ONESTN)) != 0
+    failed_t = (info_t & _lib.ST_FAILED) != 0
+    ill_t = (info_t & _lib.ST_ILLCOND) != 0
+    solved_t = ~special_t & ~failed_t
     f64_steps = solved_t & ((info_t & _lib.ST_FP64VAR) != 0)
     i8_steps = solved_t & ~f64_steps & ((info_t & _lib.ST_INT8VAR) != 0)
     f32_steps = solved_t & ~f64_steps & ~i8_steps
@@ -172,7 +216,6 @@ def main():
     # the plane products executed; the algorithmic count is 15 plane
     # products of the triangle per step up to condition estimate 1e7, 21
     # above, and the FP64-equivalent rate that of the same L^-1 delta
-    kap_np = kap_t.cpu().numpy()
     prods = np.where(kap_np <= 1e7, 15.0, 21.0)
     ops8 = float(np.sum((prods * nsq)[i8_steps]) * C)
     ach8 = ops8x / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
@@ -181,19 +224,100 @@ def main():
     # kgen: algorithmic HBM bytes = delta written (4 or 8 B) + cell coords
     kb = float(np.sum(timed_n[f32_steps]) * C * 4 +
                np.sum(timed_n[f64_steps | i8_steps]) * C * 8 +
-               16 * C * K * B)
+               16 * C * TK)
     mskg, ckg = kern['kgen']
     kg_gbs = kb / (mskg * 1e-3) / 1e9 if mskg > 0 else 0.0
     log('kernel totals (ms, launches): ' +
         ', '.join(f'{k}={v[0]:.1f}/{v[1]}' for k, v in kern.items()))
 
-    # ---- plain FP64 path on P steps + error of mixed vs FP64 ------------
+    # ---- accuracy gate on the timed steps ------------------------------
+    # (b) every timed step against the batched FP64 path (FP64 contraction,
+    #     the same dual-form estimate) on sampled cells
+    rng = np.random.default_rng(5)
+    gidx = np.sort(rng.choice(C, args.gate_cells, replace=False))
+    gidx_d = torch.as_tensor(gidx, device=dev)
+    probs = DeviceProblem(w.stn_x, w.stn_y, w.cell_x[gidx], w.cell_y[gidx],
+                          origin=prob.origin)
+    vals_d[:TK].copy_(vals_h[W * B:n_tot])
+    vg_dd[:TK * vrec].copy_(vg_h[W * B * vrec:n_tot * vrec])
+    eb, vb, infob = interp_steps(probs, vals_d[:TK], None, 'OK',
+                                 precision='fp64b', vg_dev=vg_dd[:TK * vrec])
+    em_s = est[:TK].index_select(1, gidx_d).cpu().numpy()
+    vm_s = var[:TK].index_select(1, gidx_d).cpu().numpy()
+    eb, vb = eb.cpu().numpy(), vb.cpu().numpy()
+
+    def rel(a, b):
+        f = 1e-3 * np.abs(b).mean(axis=-1, keepdims=True)
+        return (np.abs(a - b) / np.maximum(np.abs(b), f)).max(axis=-1)
+
+    with np.errstate(invalid='ignore', divide='ignore'):
+        re_b, rv_b = rel(em_s, eb), rel(vm_s, vb)
+    ok_b = solved_t & ~ill_t
+    gate_b = {'steps_compared': int(ok_b.sum()), 'cells': int(len(gidx)),
+              'est_max': float(re_b[ok_b].max()) if ok_b.any() else None,
+              'var_max': float(rv_b[ok_b].max()) if ok_b.any() else None}
+    log(f'gate (mixed vs batched FP64, all timed steps): {gate_b}')
+    # (a) a spread of timed steps, and the unflagged ones of largest
+    #     condition estimate, against the exact solution of the reference's
+    #     system (oracle.krige_exact) on the first 100 sampled cells; flagged
+    #     steps of largest condition estimate are reported with the
+    #     reference's own pinv error beside the library's
+    from oracle import spk_oracle as O
+    oc = gidx[:100]
+    cand = np.flatnonzero(solved_t)
+    pick = list(cand[np.linspace(0, len(cand) - 1, args.gate_oracle_steps)
+                     .astype(int)]) if len(cand) else []
+    unfl = cand[~ill_t[cand]]
+    pick += list(unfl[np.argsort(kap_np[unfl])[-4:]])
+    flg = cand[ill_t[cand]]
+    pick_ill = list(flg[np.argsort(kap_np[flg])[:3]])
+    gate_a, ill_rep = [], []
+    for t in sorted(set(pick)) + pick_ill:
+        st = W * B + t
+        act = ~np.isnan(vals[st])
+        xs, ys, z = w.stn_x[act], w.stn_y[act], vals[st][act]
+        rr = O.dists_2d(xs, ys, xs, ys)
+        dr = O.dists_2d(w.cell_x[oc], w.cell_y[oc], xs, ys)
+        ex, vx, _ = O.krige_exact('OK', vgs[st], rr, dr, z, want_var=True,
+                                  its=1)
+        ge = est[t].index_select(0, gidx_d[:100]).cpu().numpy()
+        gv = var[t].index_select(0, gidx_d[:100]).cpu().numpy()
+        row = {'step': int(st), 'n': int(act.sum()),
+               'kappa': float(kap_np[t]), 'flags': int(info_t[t]),
+               'est_err': float(rel(ge, ex)), 'var_err': float(rel(gv, vx))}
+        if ill_t[t]:
+            K_, R_ = O.krig_system('OK', vgs[st], rr, dr, None, None, 0.0)
+            ep, vp, _ = O.krige_contract(np.linalg.pinv(K_), R_, z, dr, True)
+            row.update({'pinv_est_err': float(rel(ep, ex)),
+                        'pinv_var_err': float(rel(vp, vx))})
+            ill_rep.append(row)
+        else:
+            gate_a.append(row)
+    ea = max((r['est_err'] for r in gate_a), default=0.0)
+    va = max((r['var_err'] for r in gate_a), default=0.0)
+    log(f'gate (mixed vs exact, {len(gate_a)} timed steps): est {ea:.2e} '
+        f'var {va:.2e}; flagged: {ill_rep}')
+    gate_pass = (ea <= 1e-6 and va <= 1e-6 and
+                 (gate_b['est_max'] or 0.0) <= 1e-6 and
+                 (gate_b['var_max'] or 0.0) <= 1e-6)
+    gate = {'passed': bool(gate_pass), 'bound': 1e-6,
+            'floors': '1e-3 x the step\'s mean |estimate| and mean variance',
+            'timed_steps': int(TK), 'flagged_ill_conditioned': int(ill_t.sum()),
+            'failed_nearest_fill': int(failed_t.sum()),
+            'special_steps': int(special_t.sum()),
+            'vs_batched_fp64_all_timed_steps': gate_b,
+            'vs_exact_oracle': {'steps': len(gate_a), 'cells': int(len(oc)),
+                                'est_max': ea, 'var_max': va,
+                                'per_step': gate_a},
+            'flagged_steps_vs_exact': ill_rep}
+
+    # ---- plain FP64 path on P steps (the ratio's denominator) ----------
     s0 = n_tot
-    pv = vals_d[s0:s0 + P]
-    vrec = _lib.VG_DTYPE.itemsize
-    pg = vg_d[s0 * vrec:(s0 + P) * vrec]
-    e64 = torch.empty((P, C), dtype=torch.float64, device=dev)
-    v64 = torch.empty((P, C), dtype=torch.float64, device=dev)
+    vals_d[:P].copy_(vals_h[s0:s0 + P])
+    vg_dd[:P * vrec].copy_(vg_h[s0 * vrec:(s0 + P) * vrec])
+    pv, pg = vals_d[:P], vg_dd[:P * vrec]
+    e64 = est[:P]
+    v64 = var[:P]
     interp_steps(prob, pv[:1], None, 'OK', precision='fp64', est=e64[:1],
                  var=v64[:1], vg_dev=pg[:vrec])   # warm the plain kernels
     prof_enable(0, True)
@@ -201,6 +325,10 @@ def main():
     tp0 = time.perf_counter()
     _, _, pinfo = interp_steps(prob, pv, None, 'OK', precision='fp64',
                                est=e64, var=v64, vg_dev=pg)
+    clip_cast(e64, out=e32[:P])
+    clip_cast(v64, out=v32[:P])
+    e32_h[:P].copy_(e32[:P], non_blocking=True)
+    v32_h[:P].copy_(v32[:P], non_blocking=True)
     torch.cuda.synchronize()
     tp1 = time.perf_counter()
     msp, cp = prof_get('gemm_plain_f64')
@@ -211,113 +339,64 @@ def main():
     pn = n_act[s0:s0 + P].astype(np.float64)
     flp = float(np.sum(2 * (pn + 1) ** 2 * C))
     achp = flp / (msp * 1e-3) / 1e12 if msp > 0 else 0.0
-    em = torch.empty((P, C), dtype=torch.float64, device=dev)
-    vm = torch.empty((P, C), dtype=torch.float64, device=dev)
-    kap = torch.zeros(P, dtype=torch.float64, device=dev)
-    _, _, minfo = interp_steps(prob, pv, None, 'OK', precision='mixed',
-                               est=em, var=vm, vg_dev=pg, kappa=kap)
     pi = pinfo.cpu().numpy()
-    mi = minfo.cpu().numpy()
-    kp = kap.cpu().numpy()
-    # errors are stated on steps with condition estimate <= 1e6; above it
-    # two FP64 formulations (the plain path's explicit inverse, like the
-    # reference's pinv, and the batched path's dual form) differ from each
-    # other by more than 1e-6, so those steps are reported apart
-    solved = (pi & (_lib.ST_ILLCOND | _lib.ST_FAILED)) == 0
-    comparable = solved & (kp <= 1e6)
-    high = solved & (kp > 1e6)
-    per_step = []
-    ee_all, ev_all = [], []
-    eh_all, vh_all = [], []
+    plain_err = []
     for s in range(P):
-        a, b = em[s], e64[s]
-        floor = 1e-3 * b.abs().mean()
-        re = ((a - b).abs() / torch.maximum(b.abs(), floor)).cpu().numpy()
-        vfloor = 1e-3 * v64[s].abs().mean()
-        rv = ((vm[s] - v64[s]).abs() /
-              torch.maximum(v64[s].abs(), vfloor)).cpu().numpy()
-        per_step.append({'kappa': float(kp[s]), 'fp64_var': bool(mi[s] & 32),
-                         'n': int(n_act[s0 + s]),
-                         'est_max': float(re.max()),
-                         'var_max': float(rv.max())})
-        if comparable[s]:
-            ee_all.append(re)
-            ev_all.append(rv)
-        elif high[s]:
-            eh_all.append(re.max())
-            vh_all.append(rv.max())
-    err = {'est_floor': '1e-3 x mean |estimate| of the step',
-           'var_floor': '1e-3 x mean variance of the step',
-           'steps_compared': int(comparable.sum()),
-           'steps_excluded': int((~comparable).sum()),
-           'high_kappa': {'steps': int(high.sum()),
-                          'est_max': float(max(eh_all, default=0.0)),
-                          'var_max': float(max(vh_all, default=0.0))},
-           'per_step': per_step}
-    if ee_all:
-        ee = np.concatenate(ee_all)
-        ev = np.concatenate(ev_all)
-        err.update({'est_max': float(ee.max()),
-                    'est_median': float(np.median(ee)),
-                    'var_max': float(ev.max()),
-                    'var_median': float(np.median(ev))})
-    log(f'mixed vs fp64: {err}')
+        if pi[s] & (_lib.ST_NOSTNS | _lib.ST_MEANFILL | _lib.ST_ONESTN |
+                    _lib.ST_FAILED):
+            continue
+        st = s0 + s
+        act = ~np.isnan(vals[st])
+        xs, ys, z = w.stn_x[act], w.stn_y[act], vals[st][act]
+        rr = O.dists_2d(xs, ys, xs, ys)
+        dr = O.dists_2d(w.cell_x[oc], w.cell_y[oc], xs, ys)
+        ex, vx, _ = O.krige_exact('OK', vgs[st], rr, dr, z, want_var=True,
+                                  its=1)
+        ge = e64[s].index_select(0, gidx_d[:100]).cpu().numpy()
+        gv = v64[s].index_select(0, gidx_d[:100]).cpu().numpy()
+        plain_err.append({'step': int(st), 'flags': int(pi[s]),
+                          'est_err': float(rel(ge, ex)),
+                          'var_err': float(rel(gv, vx))})
+    pe = [r for r in plain_err if not r['flags'] & _lib.ST_ILLCOND]
+    plain_gate = {'steps': len(pe),
+                  'flagged': len(plain_err) - len(pe),
+                  'est_max': max((r['est_err'] for r in pe), default=None),
+                  'var_max': max((r['var_err'] for r in pe), default=None)}
+    log(f'plain vs exact: {plain_gate}')
 
     # ---- CPU baseline (reference loop structure, bounded sample) -------
-    # The oracle runs on sampled cells of the plain steps: its time is the
-    # CPU baseline and its values are the FP64 reference for the errors.
+    # The oracle's restatement of interp/steps.py:350-434 (pinv per step,
+    # np.matmul per cell) on timed steps: one step per worker, one worker
+    # per core, all at once; then one step alone for the 1-core figure.
     cpu = None
-    err_oracle = None
-    if not args.no_cpu:
-        from threadpoolctl import threadpool_limits
-        from oracle import spk_oracle as O
+    if pool is not None:
         rng = np.random.default_rng(7)
         cidx = rng.choice(C, args.cpu_cells, replace=False)
-        tot_setup, tot_cell, used = 0.0, 0.0, 0
-        errs = {'mixed': [[], []], 'plain': [[], []]}
-        with threadpool_limits(1):
-            for s in range(min(args.cpu_steps, P)):
-                st = n_tot + s
-                ts, tc, ecpu, vcpu = O.time_ok_step_loop(
-                    w.stn_x, w.stn_y, w.cell_x[cidx], w.cell_y[cidx],
-                    vals[st], vgs[st])
-                tot_setup += ts
-                tot_cell += tc
-                used += 1
-                if comparable[s]:
-                    floor = 1e-3 * np.abs(ecpu).mean()
-                    for nm, (eg, vg_) in (('mixed', (em, vm)),
-                                          ('plain', (e64, v64))):
-                        ge = eg[s].cpu().numpy()[cidx]
-                        gv = vg_[s].cpu().numpy()[cidx]
-                        errs[nm][0].append(np.abs(ge - ecpu) /
-                                           np.maximum(np.abs(ecpu), floor))
-                        errs[nm][1].append(np.abs(gv - vcpu) / np.abs(vcpu))
-                log(f'cpu sample step {s}: setup {ts:.2f}s, '
-                    f'{tc * 1e6:.0f} us/cell')
-        per_step_t = tot_setup / used + C * (tot_cell / used)
-        cpu = {'value': C / per_step_t, 'unit': 'cell-estimates/s',
-               'cores': 1, 'kind': 'port',
-               'sample': (f'{used} hourly steps x {args.cpu_cells} cells, '
-                          'numpy pinv per step + np.matmul per cell '
-                          '(interp/steps.py:350-434), one BLAS thread; '
-                          'rate = C / (setup + C * per-cell time)')}
-        err_oracle = {}
-        for nm, (le, lv) in errs.items():
-            if le:
-                e_ = np.concatenate(le)
-                v_ = np.concatenate(lv)
-                err_oracle[nm] = {'est_max': float(e_.max()),
-                                  'est_median': float(np.median(e_)),
-                                  'var_max': float(v_.max()),
-                                  'var_median': float(np.median(v_))}
-        err_oracle['cells_per_step'] = int(args.cpu_cells)
-        err_oracle['steps'] = int(sum(comparable[:used]))
+        steps_c = [W * B + int(t) for t in
+                   np.linspace(0, TK - 1, n_cores).astype(int)]
+        jobs = [(w.stn_x, w.stn_y, w.cell_x[cidx], w.cell_y[cidx], vals[st],
+                 vgs[st]) for st in steps_c]
+        tc0 = time.perf_counter()
+        res = pool.map(_cpu_step, jobs)
+        tc1 = time.perf_counter()
+        per_step = [ts + C * tc for ts, tc in res]
+        all_rate = n_cores * C / float(np.mean(per_step))
+        ts1, tc1_ = pool.map(_cpu_step, jobs[:1])[0]
+        one_rate = C / (ts1 + C * tc1_)
+        pool.close()
+        cpu = {'value': all_rate, 'unit': 'cell-estimates/s',
+               'cores': n_cores, 'kind': 'port',
+               'one_core_value': one_rate,
+               'sample': (f'{n_cores} timed hourly steps x '
+                          f'{args.cpu_cells} cells, one per worker process '
+                          f'with one BLAS thread, all {n_cores} at once '
+                          f'({tc1 - tc0:.1f} s); numpy pinv per step + '
+                          'np.matmul per cell (interp/steps.py:350-434); '
+                          'rate = cores x C / mean(setup + C x per-cell '
+                          'time); one_core_value: the first step alone')}
         log(f'cpu baseline: {cpu}')
-        log(f'error vs oracle: {err_oracle}')
 
     if args.calib:
-        from spinterps_amd.interp.steps import clip_cast
         x = torch.zeros(1 << 28, dtype=torch.float64, device=dev)
         for _ in range(2):
             clip_cast(x)
@@ -370,11 +449,16 @@ def main():
                                'active stations, OK estimate + variance',
                    'time_steps_per_bench_step': B, 'n_cells': C,
                    'n_stations': w.n_stns},
+        'timed_region': 'H2D of the steps\' values and variogram '
+                        'records, OK estimate + variance, float32 cast, '
+                        'D2H of both fields (stations and cells resident)',
+        'device_only': {'value': TK * C / (dev_ms * 1e-3),
+                        'ms_per_step': dev_ms / K},
         'plain_fp64': {'value': plain_rate, 'steps': P,
                        'ratio_mixed_over_plain': value / plain_rate,
-                       'gemm_tflops': achp},
-        'error_mixed_vs_fp64': err,
-        'error_vs_oracle': err_oracle,
+                       'gemm_tflops': achp, 'vs_exact': plain_gate,
+                       'per_step': plain_err},
+        'accuracy_gate': gate,
         'roofline': dominant,
         'rooflines': {
             'gemm_tri_f32': {'bound': 'mfma', 'achieved': ach32,
@@ -406,6 +490,17 @@ def main():
     if e2e is not None:
         out['end_to_end'] = e2e
     print(json.dumps(out), flush=True)
+
+
+def _cpu_step(job):
+    """One hourly step of the CPU baseline in a worker process, with one
+    BLAS thread: (setup seconds, seconds per cell)."""
+    from threadpoolctl import threadpool_limits
+    from oracle import spk_oracle as O
+    sx, sy, cx, cy, z, vg = job
+    with threadpool_limits(1):
+        ts, tc, _, _ = O.time_ok_step_loop(sx, sy, cx, cy, z, vg)
+    return ts, tc
 
 
 def e2e_leg(args, log):

@@ -58,10 +58,10 @@ struct Mfma<float> {
 // the plain path's error against an exact solution stays below 450 times
 // this deviation, so unflagged steps are within 5e-7.
 constexpr double kSumTol = 1e-9;
-// Relative change of a cell's estimate by the last refinement step above
-// which the step is flagged SPK_ST_ILLCOND.  On hourly steps the change
-// is within a factor 3 of the dual form's error against an exact solution
-// (long-double residuals), so unflagged steps stay well inside 1e-6.
+// Relative change of a cell's estimate by the second refinement step above
+// which the step is flagged SPK_ST_ILLCOND (the change is of the size of
+// the error left by the first step; the refined estimate is at least as
+// good), so unflagged steps stay well inside 1e-6.
 constexpr double kRefineTol = 1e-7;
 
 // np.isclose(x, 1.0) with rtol 1e-5, atol 1e-8 (interp/steps.py:418)
@@ -1112,9 +1112,9 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   const int64_t oc = perm ? (int64_t)perm[c] : c;
   double *eo = est + (int64_t)sl * ldo;
   const double *z = zact + (int64_t)sl * ldz;
-  // Residual check: the last refinement step changed this cell's estimate
-  // by de.  It is of the size of the error left before it, and the refined
-  // result is at least as good, so a change above kRefineTol of the cell's
+  // Residual check: the second refinement step changed this cell's
+  // estimate by de, of the size of the error the first one left, and the
+  // result is at least as good; a change above kRefineTol of the cell's
   // scale (its estimate, at least 1e-3 of the step's mean |value|) marks
   // the step ill-conditioned: no FP64 solve of it is reproducible to 1e-6.
   // (The variance is far less sensitive: in the dual form its error stays

@@ -500,12 +500,13 @@ __global__ __launch_bounds__(256) void k_trtri(const double *__restrict__ L,
 //   EDK: F = [1 | S], U = Li F, W = Li^T U, B = U^T U, h = B^-1 U^T (Li z),
 //        alpha = C^-1 z - W h  ->  est = k.alpha + h.[1; s_cell]
 // alpha and h solve the bordered system [C F; F^T 0] [alpha; h] = [z; 0].
-// One step of iterative refinement follows, with FP64 residuals against
+// Two steps of iterative refinement follow, with FP64 residuals against
 // the covariances the Cholesky factorised (recomputed from the
 // coordinates): r1 = z - C alpha - F h, r2 = -F^T alpha, rw = 1 - C w;
 // dh = B^-1 (F^T C^-1 r1 - r2), dalpha = C^-1 r1 - W dh, dw = C^-1 rw.
-// The corrections are added, and kept: k.dalpha + f.dh is the change of a
-// cell's estimate, which the cell kernel uses as its error indicator.
+// The corrections are added; the second step's are kept: k.dalpha + f.dh
+// is the last change of a cell's estimate, of the size of the error left,
+// which the cell kernel uses as its error indicator.
 // Vectors are stored per step in V[s][0:ld] (alpha), V[s][ld:2 ld] (w),
 // V[s][2 ld:3 ld] (dalpha) and V[s][3 ld:4 ld] (dw); dh in the StepSc.
 __device__ inline double wave_sum(double v) {
@@ -733,113 +734,122 @@ __global__ __launch_bounds__(256) void k_dual(
   }
   __syncthreads();
 
-  // ---- one step of iterative refinement with FP64 residuals ----------------
+  // ---- iterative refinement with FP64 residuals ------------------------------
   // r1 = z - C alpha - F h, rw = 1 - C w, with C the covariances the
   // Cholesky factorised: c0 - gamma (OK, EDK) or the covariance sum (SK),
   // recomputed from the re-centred coordinates with the same arithmetic.
+  // Two steps: the first removes the error of the explicit-inverse solve,
+  // the second's correction is of the size of the error left, which the
+  // cell kernel turns into each cell's error indicator.
   {
     __shared__ double stx[256], sty[256], sal[256], sw[256];
     __shared__ spk_vg vg;
+    __shared__ double hcur[kMaxP];
+    __shared__ double gF[kMaxP], r2s[kMaxP], dhs[kMaxP];
     if (tid == 0) vg = vgs[s];
+    if (tid < kMaxP) hcur[tid] = tid < p ? hcoef[tid] : 0.0;
     const bool sk = method == SPK_M_SK;
     const double c0 = sc[s].c0;
-    for (int i0 = 0; i0 < n; i0 += 256) {
-      const int i = i0 + tid;
-      double xi = 0.0, yi = 0.0;
-      if (i < n) {
-        xi = sx[my[i]];
-        yi = sy[my[i]];
-      }
-      double ca = 0.0, cw = 0.0;
-      for (int j0 = 0; j0 < n; j0 += 256) {
-        __syncthreads();
-        if (j0 + tid < n) {
-          const int g = my[j0 + tid];
-          stx[tid] = sx[g];
-          sty[tid] = sy[g];
-          sal[tid] = alpha[j0 + tid];
-          sw[tid] = w[j0 + tid];
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int i0 = 0; i0 < n; i0 += 256) {
+        const int i = i0 + tid;
+        double xi = 0.0, yi = 0.0;
+        if (i < n) {
+          xi = sx[my[i]];
+          yi = sy[my[i]];
         }
-        __syncthreads();
-        const int cnt = min(256, n - j0);
-        if (i < n)
-          for (int j = 0; j < cnt; ++j) {
-            const double d = dist2(xi, yi, stx[j], sty[j]);
-            const double e = vg_entry(vg, d, sk ? 1 : 0, min_vg_val);
-            const double c = sk ? e : c0 - e;
-            ca += c * sal[j];
-            cw += c * sw[j];
+        double ca = 0.0, cw = 0.0;
+        for (int j0 = 0; j0 < n; j0 += 256) {
+          __syncthreads();
+          if (j0 + tid < n) {
+            const int g = my[j0 + tid];
+            stx[tid] = sx[g];
+            sty[tid] = sy[g];
+            sal[tid] = alpha[j0 + tid];
+            sw[tid] = w[j0 + tid];
+          }
+          __syncthreads();
+          const int cnt = min(256, n - j0);
+          if (i < n)
+            for (int j = 0; j < cnt; ++j) {
+              const double d = dist2(xi, yi, stx[j], sty[j]);
+              const double e = vg_entry(vg, d, sk ? 1 : 0, min_vg_val);
+              const double c = sk ? e : c0 - e;
+              ca += c * sal[j];
+              cw += c * sw[j];
+            }
+        }
+        if (i < n) {
+          double r1 = z[i] - ca;
+          if (!sk) {
+            r1 -= hcur[0];
+            for (int k = 1; k < p; ++k)
+              r1 -= stn_drift[(int64_t)my[i] * m + (k - 1)] * hcur[k];
+          }
+          R2[i] = r1;
+          R2[i + ldt] = 1.0 - cw;
+        }
+      }
+      __syncthreads();
+      tri_lower_mv(L, ld, n, R2, Y2, 2);  // Li [r1 rw]
+      __syncthreads();
+      tri_lower_tmv(L, ld, n, Y2, Z2, 2);  // C^-1 [r1 rw]
+      __syncthreads();
+      // gF[a] = U_F[:, a] . (Li r1) = F^T C^-1 r1, r2[a] = -F[:, a] . alpha
+      if (!sk) {
+        for (int a = tid >> 6; a < 2 * p; a += 4) {
+          double acc = 0.0;
+          const int col = a < p ? a : a - p;
+          for (int i = (tid & 63); i < n; i += 64) {
+            if (a < p) {
+              acc += U[i + (int64_t)col * ldt] * Y2[i];
+            } else {
+              const double f =
+                  col == 0 ? 1.0 : stn_drift[(int64_t)my[i] * m + (col - 1)];
+              acc -= f * alpha[i];
+            }
+          }
+          acc = wave_sum(acc);
+          if ((tid & 63) == 0) {
+            if (a < p)
+              gF[col] = acc;
+            else
+              r2s[col] = acc;
+          }
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        for (int a = 0; a < kMaxP; ++a) dhs[a] = 0.0;
+        if (!sk)
+          for (int a = 0; a < p; ++a) {
+            double v = 0.0;
+            for (int b = 0; b < p; ++b) v += Bi[a * p + b] * (gF[b] - r2s[b]);
+            dhs[a] = v;
           }
       }
-      if (i < n) {
-        double r1 = z[i] - ca;
-        if (!sk) {
-          r1 -= hcoef[0];
-          for (int k = 1; k < p; ++k)
-            r1 -= stn_drift[(int64_t)my[i] * m + (k - 1)] * hcoef[k];
-        }
-        R2[i] = r1;
-        R2[i + ldt] = 1.0 - cw;
+      __syncthreads();
+      for (int i = tid; i < n; i += 256) {
+        double da = Z2[i];
+        if (!sk)
+          for (int k = 0; k < p; ++k) da -= X[i + (int64_t)k * ldt] * dhs[k];
+        const double dwi = Z2[i + ldt];
+        alpha[i] += da;
+        w[i] += dwi;
+        dalpha[i] = da;
+        dw[i] = dwi;
       }
-    }
-    __syncthreads();
-    tri_lower_mv(L, ld, n, R2, Y2, 2);  // Li [r1 rw]
-    __syncthreads();
-    tri_lower_tmv(L, ld, n, Y2, Z2, 2);  // C^-1 [r1 rw]
-    __syncthreads();
-    // gF[a] = U_F[:, a] . (Li r1) = F^T C^-1 r1, r2[a] = -F[:, a] . alpha
-    __shared__ double gF[kMaxP], r2s[kMaxP], dhs[kMaxP];
-    if (!sk) {
-      for (int a = tid >> 6; a < 2 * p; a += 4) {
-        double acc = 0.0;
-        const int col = a < p ? a : a - p;
-        for (int i = (tid & 63); i < n; i += 64) {
-          if (a < p) {
-            acc += U[i + (int64_t)col * ldt] * Y2[i];
-          } else {
-            const double f =
-                col == 0 ? 1.0 : stn_drift[(int64_t)my[i] * m + (col - 1)];
-            acc -= f * alpha[i];
-          }
-        }
-        acc = wave_sum(acc);
-        if ((tid & 63) == 0) {
-          if (a < p)
-            gF[col] = acc;
-          else
-            r2s[col] = acc;
-        }
-      }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      for (int a = 0; a < kMaxP; ++a) dhs[a] = 0.0;
-      if (!sk)
-        for (int a = 0; a < p; ++a) {
-          double v = 0.0;
-          for (int b = 0; b < p; ++b) v += Bi[a * p + b] * (gF[b] - r2s[b]);
-          dhs[a] = v;
-        }
-    }
-    __syncthreads();
-    for (int i = tid; i < n; i += 256) {
-      double da = Z2[i];
-      if (!sk)
-        for (int k = 0; k < p; ++k) da -= X[i + (int64_t)k * ldt] * dhs[k];
-      const double dwi = Z2[i + ldt];
-      alpha[i] += da;
-      w[i] += dwi;
-      dalpha[i] = da;
-      dw[i] = dwi;
+      if (tid < p) hcur[tid] += dhs[tid];
+      __syncthreads();
     }
     if (tid == 0) {
       StepSc &o = sc[s];
       o.b = G[0];  // 1^T C^-1 1
       for (int a = 0; a < p; ++a) {
-        o.h[a] = hcoef[a] + dhs[a];
+        o.h[a] = hcur[a];
         o.dh[a] = dhs[a];
       }
-      o.a0 = sk ? 0.0 : hcoef[0] + dhs[0];
+      o.a0 = sk ? 0.0 : hcur[0];
     }
   }
 }

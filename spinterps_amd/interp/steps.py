@@ -184,9 +184,13 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     return est, (var if wv else None), info
 
 
-def clip_cast(f64, min_var_cut=None, max_var_cut=None):
-    """interp/steps.py:466-476 clip + float32 cast, on the device."""
-    out = torch.empty(f64.shape, dtype=torch.float32, device=f64.device)
+def clip_cast(f64, min_var_cut=None, max_var_cut=None, out=None):
+    """interp/steps.py:466-476 clip + float32 cast, on the device (into
+    ``out``, a float32 device tensor of f64's shape, when given)."""
+    if out is None:
+        out = torch.empty(f64.shape, dtype=torch.float32, device=f64.device)
+    assert out.dtype == torch.float32 and out.shape == f64.shape
+    assert out.is_contiguous() and f64.is_contiguous()
     ctx = _ctx_for(f64.device)
     check(lib().spk_clip_cast_dev(
         ctx.h, f64.data_ptr(), out.data_ptr(), f64.numel(),

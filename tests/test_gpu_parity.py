@@ -329,9 +329,10 @@ def test_mixed_precision_policy_500m_grid(gpu, mode):
     contraction runs in FP32 below the FP32 limit (condition estimate 1e5,
     1e3 for Gaussian models) and by exact int8 plane products above it.
     'fp32' (int8 switched off): FP64 above the limit.  'int8_limit'
-    (caller's int8 limit 1e10): FP64 above that.  Against the plain FP64 path every step with condition
-    estimate <= 1e6 stays within 1e-6 (relative, floor 1e-3 x the step's
-    mean)."""
+    (caller's int8 limit 1e10): FP64 above that.  Against the batched FP64
+    path (the same FP64 estimate, the variance contraction in FP64) every
+    step the mixed path does not flag ill-conditioned stays within 1e-6
+    (relative, floor 1e-3 x the step's mean)."""
     import torch
     from spinterps_amd import _lib
     from spinterps_amd.interp.steps import DeviceProblem, interp_steps
@@ -343,16 +344,16 @@ def test_mixed_precision_policy_500m_grid(gpu, mode):
     k8 = {'int8': 0.0, 'fp32': -1.0, 'int8_limit': 1e10}[mode]
     e, v, info = interp_steps(prob, vals, vgs, 'OK', precision='mixed',
                               kappa=kap, int8var_kappa=k8)
-    e64, v64, i64 = interp_steps(prob, vals, vgs, 'OK', precision='fp64')
+    e64, v64, i64 = interp_steps(prob, vals, vgs, 'OK', precision='fp64b')
     e, v, inf = e.cpu().numpy(), v.cpu().numpy(), info.cpu().numpy()
     e64, v64, i64 = e64.cpu().numpy(), v64.cpu().numpy(), i64.cpu().numpy()
     k = kap.cpu().numpy()
     gau = np.array(['Gau' in g for g in vgs])
     fp64 = (inf & _lib.ST_FP64VAR) != 0
     int8 = ((inf & _lib.ST_INT8VAR) != 0) & ~fp64
-    solved = (i64 & _SOLVED_MASK) == 0
     runs = (inf & (1 | 2 | 4 | 8)) == 0      # steps whose systems were solved
-    assert solved.sum() > 80
+    solved = runs & ((inf & _lib.ST_ILLCOND) == 0)
+    assert runs.sum() > 80 and solved.sum() > 70
     above = runs & np.where(gau, k > 1e3, k > 1e5)
     if mode == 'int8':
         assert int8[above].all() and not int8[runs & ~above].any()
@@ -367,7 +368,7 @@ def test_mixed_precision_policy_500m_grid(gpu, mode):
     else:
         assert fp64[runs & (k > 1e10)].all() and (runs & (k > 1e10)).any()
         assert int8[above & (k <= 1e10)].all()
-    cmp = solved & (k <= 1e6)
+    cmp = solved
     floor_e = 1e-3 * np.abs(e64).mean(axis=1, keepdims=True)
     floor_v = 1e-3 * np.abs(v64).mean(axis=1, keepdims=True)
     with np.errstate(invalid='ignore'):   # steps without variance are NaN
@@ -489,7 +490,9 @@ def test_int8_plane_contraction_matches_fp64(gpu):
     _, v64, i64 = interp_steps(prob, vals, vgs, 'OK', precision='fp64b')
     k = kap.cpu().numpy()
     inf8, i64 = inf8.cpu().numpy(), i64.cpu().numpy()
-    solved = (i64 & _SOLVED_MASK) == 0
+    # steps whose systems were solved (flagged ones too: the contraction's
+    # precision is what is compared, not the conditioning)
+    solved = (i64 & (1 | 2 | 4 | 8)) == 0
     int8 = ((inf8 & _lib.ST_INT8VAR) != 0) & ((inf8 & _lib.ST_FP64VAR) == 0)
     sel = solved & int8
     assert sel.sum() >= 20, (k, inf8)
