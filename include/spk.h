@@ -217,6 +217,19 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          const double *vals, const spk_vg *vgs, double *est,
                          double *var, int32_t *info, double *kappa);
 
+/* No counterpart in the reference.  spk_interp_steps_dev for SPK_M_IDW and
+ * SPK_M_NNB that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and results; its scratch (re-centred
+ * coordinates, per-step compaction) is a static device array of the
+ * library, about 23 MB per device, held until the process ends.  At most
+ * 8192 stations and 2^20 cells (SPK_ESHAPE beyond).  Calls that overlap on
+ * one device share that array and race.  kappa may be NULL. */
+int spk_idw_steps_noalloc_dev(spk_ctx *ctx, const spk_problem *prob,
+                              const spk_opts *opts, int64_t n_steps,
+                              const double *vals, double *est, int32_t *info,
+                              double *kappa);
+
 /* interp/steps.py:466-476 + the float32 cast of steps.py:815-831:
  * out32[i] = (float)clip(in64[i]) on device; NaN passes through.
  * has_min/has_max select the cuts (None in the reference = 0). */

@@ -100,6 +100,10 @@ SIGNATURES = {
                                      POINTER(SpkOpts), c_int64, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p]),
+    'spk_idw_steps_noalloc_dev': (c_int, [c_void_p, POINTER(SpkProblem),
+                                          POINTER(SpkOpts), c_int64,
+                                          c_void_p, c_void_p, c_void_p,
+                                          c_void_p]),
     'spk_clip_cast_dev': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                   c_double, c_int, c_double]),
     'spk_last_timings': (c_int, [c_void_p, _PD, c_int, POINTER(c_int)]),

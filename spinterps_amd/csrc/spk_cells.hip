@@ -1319,6 +1319,12 @@ __global__ __launch_bounds__(256) void k_special(
   if (vo) vo[c] = 0.0;
 }
 
+// n ints set to v (a kernel, not hipMemsetAsync, so that a caller may
+// capture it in a graph)
+__global__ void k_fill_int(int *__restrict__ p, int n, int v) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
 __global__ void k_status_out(const StepSc *__restrict__ sc, int S,
                              int32_t *__restrict__ info,
                              double *__restrict__ kappa) {
@@ -1426,6 +1432,52 @@ static int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 }  // namespace spk
 
 using namespace spk;
+
+// IDW / nearest neighbour over T steps, S steps per launch
+// (interp/steps.py:285-313, and its special steps :280-283): the kernels
+// of spk_interp_steps_dev and spk_idw_steps_noalloc_dev.  sx.. cy are the
+// re-centred coordinates; idx [S x N], zact [S x ldz], sc [S] and bad [1]
+// are the caller's scratch.  The weight-sum flag is cleared by a fill
+// kernel, and read back at the end (the reference asserts, steps.py:305).
+static int idw_nnb_steps(spk_ctx *ctx, const spk_opts *opts, int64_t T,
+                         int64_t N, int64_t C, const double *vals,
+                         const double *sx, const double *sy,
+                         const double *cx, const double *cy, int32_t *idx,
+                         double *zact, int64_t ldz, StepSc *sc, int64_t S,
+                         int *bad, double *est, int32_t *info,
+                         double *kappa) {
+  hipStream_t sm = ctx->stream;
+  const int method = opts->method;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, sm, bad, 1, 0);
+  for (int64_t t0 = 0; t0 < T; t0 += S) {
+    const int Sb = (int)std::min<int64_t>(S, T - t0);
+    double *eb = est + t0 * C;
+    hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm, vals + t0 * N,
+                       N, idx, zact, ldz, sc, (const spk_vg *)nullptr,
+                       method, opts->min_var_thr, opts->min_vg_val,
+                       (const int32_t *)nullptr);
+    {
+      ProfScope ps(ctx, "idw_nnb");
+      hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
+                         dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
+                         cy, C, method, opts->idw_exp, eb, bad);
+    }
+    hipLaunchKernelGGL(k_special, dim3((unsigned)ceil_div(C, 256), Sb),
+                       dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
+                       cy, C, eb, (double *)nullptr);
+    hipLaunchKernelGGL(k_status_out, dim3((unsigned)ceil_div(Sb, 256)),
+                       dim3(256), 0, sm, sc, Sb, info + t0,
+                       kappa ? kappa + t0 : nullptr);
+    SPK_TRY(check_launch("idw/nnb"));
+  }
+  int hbad = 0;
+  SPK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, sm));
+  SPK_HIP(hipStreamSynchronize(sm));
+  if (hbad)
+    return set_error(SPK_EARG, "IDW weight sum below 1e-14 "
+                     "(interp/steps.py:305)");
+  return SPK_OK;
+}
 
 extern "C" {
 
@@ -1562,46 +1614,19 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int64_t strideM = ld * ld;
   const double thr = opts->min_var_thr;
   const double mvg = opts->min_vg_val;
-  int *bad;
-  SPK_TRY(ws_get(ctx, "bad", 64, (void **)&bad));
-  SPK_HIP(hipMemsetAsync(bad, 0, sizeof(int), sm));
-
   // ---- IDW / nearest neighbour: no systems ---------------------------------
   if (!krig) {
     const int64_t S = std::min<int64_t>(T, 512);
     int32_t *idx;
     double *zact;
     StepSc *sc;
+    int *bad;
+    SPK_TRY(ws_get(ctx, "bad", 64, (void **)&bad));
     SPK_TRY(ws_get(ctx, "idx0", S * N * sizeof(int32_t), (void **)&idx));
     SPK_TRY(ws_get(ctx, "zact0", S * ldz * sizeof(double), (void **)&zact));
     SPK_TRY(ws_get(ctx, "sc0", S * sizeof(StepSc), (void **)&sc));
-    for (int64_t t0 = 0; t0 < T; t0 += S) {
-      const int Sb = (int)std::min<int64_t>(S, T - t0);
-      double *eb = est + t0 * C;
-      hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sm, vals + t0 * N,
-                         N, idx, zact, ldz, sc, (const spk_vg *)nullptr,
-                         method, thr, mvg, (const int32_t *)nullptr);
-      {
-        ProfScope ps(ctx, "idw_nnb");
-        hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
-                           dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
-                           cy, C, method, opts->idw_exp, eb, bad);
-      }
-      hipLaunchKernelGGL(k_special, dim3((unsigned)ceil_div(C, 256), Sb),
-                         dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
-                         cy, C, eb, (double *)nullptr);
-      hipLaunchKernelGGL(k_status_out, dim3((unsigned)ceil_div(Sb, 256)),
-                         dim3(256), 0, sm, sc, Sb, info + t0,
-                         kappa ? kappa + t0 : nullptr);
-      SPK_TRY(check_launch("idw/nnb"));
-    }
-    int hbad = 0;
-    SPK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, sm));
-    SPK_HIP(hipStreamSynchronize(sm));
-    if (hbad)
-      return set_error(SPK_EARG, "IDW weight sum below 1e-14 "
-                       "(interp/steps.py:305)");
-    return SPK_OK;
+    return idw_nnb_steps(ctx, opts, T, N, C, vals, sx, sy, cx, cy, idx, zact,
+                         ldz, sc, S, bad, est, info, kappa);
   }
 
   // ---- kriging --------------------------------------------------------------
@@ -2033,6 +2058,63 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_HIP(hipStreamWaitEvent(sm, ev_fact[0], 0));
   }
   return SPK_OK;
+}
+
+// spk_interp_steps_dev for IDW / nearest neighbour with no allocation:
+// every buffer beyond the inputs and outputs comes from the library's
+// file-scope scratch array, whose address on this device was resolved by
+// spk_ctx_create.  Same kernels, same results; at most kNaMaxStns
+// stations and kNaMaxCells cells.  Calls that overlap on one device share
+// that array and race.
+int spk_idw_steps_noalloc_dev(spk_ctx *ctx, const spk_problem *prob,
+                              const spk_opts *opts, int64_t T,
+                              const double *vals, double *est, int32_t *info,
+                              double *kappa) {
+  if (!ctx || !prob || !opts) return set_error(SPK_EARG, "null argument");
+  if (T < 0) return set_error(SPK_ESHAPE, "negative step count");
+  const int64_t N = prob->n_stns, C = prob->n_cells;
+  if (N < 1 || C < 1) return set_error(SPK_ESHAPE, "need stations and cells");
+  if (N > kNaMaxStns || C > kNaMaxCells)
+    return set_error(SPK_ESHAPE,
+                     "%lld stations and %lld cells exceed the scratch of the "
+                     "allocation-free entry point (at most %lld stations and "
+                     "%lld cells): use spk_interp_steps_dev",
+                     (long long)N, (long long)C, (long long)kNaMaxStns,
+                     (long long)kNaMaxCells);
+  if (opts->method != SPK_M_IDW && opts->method != SPK_M_NNB)
+    return set_error(SPK_EARG, "the allocation-free entry point runs IDW "
+                     "and nearest neighbour only (method %d)", opts->method);
+  if (!vals || !est || !info || !prob->stn_x || !prob->stn_y ||
+      !prob->cell_x || !prob->cell_y)
+    return set_error(SPK_EARG, "null device pointer");
+  if (T == 0) return SPK_OK;
+  SPK_HIP(hipSetDevice(ctx->device));
+  unsigned char *base = nullptr;
+  SPK_TRY(noalloc_scratch(ctx->device, &base));
+  double *sx = reinterpret_cast<double *>(base + kNaOffStn);
+  double *sy = sx + N;
+  double *cx = reinterpret_cast<double *>(base + kNaOffCell);
+  double *cy = cx + C;
+  int *bad = reinterpret_cast<int *>(base + kNaOffBad);
+  int32_t *idx = reinterpret_cast<int32_t *>(base + kNaOffIdx);
+  double *zact = reinterpret_cast<double *>(base + kNaOffZ);
+  StepSc *sc = reinterpret_cast<StepSc *>(base + kNaOffSc);
+  const int64_t ldz = round_up(N, 64);  // <= kNaMaxStns
+  hipStream_t sm = ctx->stream;
+  const unsigned gs = (unsigned)std::min<int64_t>(ceil_div(N, 256), 4096);
+  const unsigned gc = (unsigned)std::min<int64_t>(ceil_div(C, 256), 65536);
+  hipLaunchKernelGGL(k_recenter, dim3(gs), dim3(256), 0, sm, prob->stn_x, sx,
+                     N, prob->origin_x);
+  hipLaunchKernelGGL(k_recenter, dim3(gs), dim3(256), 0, sm, prob->stn_y, sy,
+                     N, prob->origin_y);
+  hipLaunchKernelGGL(k_recenter, dim3(gc), dim3(256), 0, sm, prob->cell_x, cx,
+                     C, prob->origin_x);
+  hipLaunchKernelGGL(k_recenter, dim3(gc), dim3(256), 0, sm, prob->cell_y, cy,
+                     C, prob->origin_y);
+  SPK_TRY(check_launch("k_recenter"));
+  return idw_nnb_steps(ctx, opts, T, N, C, vals, sx, sy, cx, cy, idx, zact,
+                       ldz, sc, std::min<int64_t>(T, kNaSteps), bad, est,
+                       info, kappa);
 }
 
 }  // extern "C"

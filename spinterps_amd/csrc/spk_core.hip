@@ -4,9 +4,50 @@
 #include <cstring>
 #include <string>
 
-#include "spk_internal.h"
+#include <mutex>
+
+#include "spk_krige.h"
 
 namespace spk {
+
+// Scratch memory of the allocation-free entry point
+// spk_idw_steps_noalloc_dev (layout in spk_krige.h, about 23 MB).  Each
+// device the library runs on holds its own copy until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory:
+// only the address resolved below (per device, with that device current)
+// is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[kMaxDevices] = {};
+static int g_scratch_rc[kMaxDevices] = {};
+
+// Once per device, with the device current.
+static void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device] || g_scratch_rc[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratch_rc[device] = e != hipSuccess ? (int)e : -1;
+    return;
+  }
+  g_scratch[device] = static_cast<unsigned char *>(p);
+}
+
+int noalloc_scratch(int device, unsigned char **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices || !g_scratch[device])
+    return set_error(SPK_EHIP,
+                     "scratch of the allocation-free entry point not "
+                     "resolved on device %d (hip error %d)", device,
+                     (device >= 0 && device < kMaxDevices)
+                         ? g_scratch_rc[device] : 0);
+  *out = g_scratch[device];
+  return SPK_OK;
+}
 
 static thread_local std::string g_err;
 
@@ -332,6 +373,9 @@ int spk_ctx_create(int device, spk_ctx **out) {
   if (device < 0 || device >= n)
     return set_error(SPK_EARG, "device %d not present (%d devices)", device, n);
   SPK_HIP(hipSetDevice(device));
+  // the allocation-free entry point's scratch address on this device
+  // (a failure only disables that entry point)
+  resolve_scratch(device);
   spk_ctx *c = new spk_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

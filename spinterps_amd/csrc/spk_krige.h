@@ -27,6 +27,26 @@ struct StepSc {
   double dh[kMaxP];  // last refinement correction of h (dual form)
 };
 
+// ---- scratch of the allocation-free IDW / nearest-neighbour entry point ----
+// spk_idw_steps_noalloc_dev takes every buffer beyond its inputs and outputs
+// from one file-scope device array (spk_core.hip), laid out as below: the
+// re-centred station and cell coordinates, the weight-sum flag, and the
+// per-step compaction of kNaSteps steps at a time.
+constexpr int64_t kNaMaxStns = 8192;
+constexpr int64_t kNaMaxCells = (int64_t)1 << 20;
+constexpr int64_t kNaSteps = 64;
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaOffStn = 0;
+constexpr size_t kNaOffCell = kNaOffStn + na_align(2 * kNaMaxStns * 8);
+constexpr size_t kNaOffBad = kNaOffCell + na_align(2 * kNaMaxCells * 8);
+constexpr size_t kNaOffIdx = kNaOffBad + 256;
+constexpr size_t kNaOffZ = kNaOffIdx + na_align(kNaSteps * kNaMaxStns * 4);
+constexpr size_t kNaOffSc = kNaOffZ + na_align(kNaSteps * kNaMaxStns * 8);
+constexpr size_t kNaBytes = kNaOffSc + na_align(kNaSteps * sizeof(StepSc));
+// Device address of that array on `device`, resolved once per device by
+// spk_ctx_create; SPK_EHIP if the resolution failed.
+int noalloc_scratch(int device, unsigned char **out);
+
 __global__ void k_compact(const double *vals, int64_t N, int32_t *idx,
                           double *zact, int64_t ldz, StepSc *sc,
                           const spk_vg *vgs, int method, double thr,
