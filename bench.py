@@ -253,9 +253,15 @@ def main():
     with np.errstate(invalid='ignore', divide='ignore'):
         re_b, rv_b = rel(em_s, eb), rel(vm_s, vb)
     ok_b = solved_t & ~ill_t
+    fl_b = solved_t & ill_t
     gate_b = {'steps_compared': int(ok_b.sum()), 'cells': int(len(gidx)),
               'est_max': float(re_b[ok_b].max()) if ok_b.any() else None,
-              'var_max': float(rv_b[ok_b].max()) if ok_b.any() else None}
+              'var_max': float(rv_b[ok_b].max()) if ok_b.any() else None,
+              'flagged_steps': int(fl_b.sum()),
+              'flagged_est_max': float(re_b[fl_b].max()) if fl_b.any()
+              else None,
+              'flagged_var_max': float(rv_b[fl_b].max()) if fl_b.any()
+              else None}
     log(f'gate (mixed vs batched FP64, all timed steps): {gate_b}')
     # (a) a spread of timed steps, and the unflagged ones of largest
     #     condition estimate, against the exact solution of the reference's
@@ -270,7 +276,8 @@ def main():
     unfl = cand[~ill_t[cand]]
     pick += list(unfl[np.argsort(kap_np[unfl])[-4:]])
     flg = cand[ill_t[cand]]
-    pick_ill = list(flg[np.argsort(kap_np[flg])[:3]])
+    srt = flg[np.argsort(kap_np[flg])]
+    pick_ill = sorted(set(list(srt[:2]) + list(srt[-2:])))
     gate_a, ill_rep = [], []
     for t in sorted(set(pick)) + pick_ill:
         st = W * B + t

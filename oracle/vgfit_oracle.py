@@ -10,7 +10,10 @@ Pinning status: pairs, clip, bins and the estimators are pinned by closed
 forms on tiny inputs (tests/test_vgfit_oracle.py).  The differential
 evolution is SciPy's own routine, called as vgs.py:758-764 calls it; it is
 stochastic (unseeded in the reference), so a fit is pinned by recovering
-known parameters from noise-free variograms, not by exact values.
+known parameters from noise-free variograms, not by exact values.  The
+model choice and the output strings (select_models, vg_strings) are pinned
+on hand-built objective values whose AIC outcomes are worked out by hand
+from vgs.py:771-846.
 """
 from __future__ import annotations
 
@@ -173,6 +176,52 @@ def bounds_for(o, h0, g0, var, fil_nug_vg, range_ub=1e9):
     return b
 
 
+def select_models(cands, solve, tot_pts, fit_thresh=0.01, n_best=1):
+    """The model choice of vgs.py:771-812 over the candidates in order.
+    solve(o) -> (fun, x) with x = [range_0, sill_0, range_1, sill_1, ...].
+    A candidate is accepted when its AIC = tot_pts ln(fun) + 2 len(x) is
+    below min_AIC (1 - fit_thresh) (as written, also for a negative AIC)
+    and its ranges ascend after sorting; out-of-order ranges are sorted,
+    with names and sills, and the sorted permutation is skipped when it
+    comes up later.  The last n_best accepted are kept, oldest first.
+    Returns (best [(names, ranges, sills, fun, aic)], skipped, funs)."""
+    min_obj = np.inf
+    best = []
+    skip = []
+    funs = {}
+    for o in cands:
+        if o in skip:
+            continue
+        fun, xs = solve(o)
+        xs = np.asarray(xs, np.float64)
+        funs[o] = (fun, xs.copy())
+        rngs = xs[0::2].copy()
+        sills = xs[1::2].copy()
+        aic = tot_pts * np.log(fun) + 2 * xs.shape[0]
+        c1 = aic < min_obj * (1.0 - fit_thresh)
+        names = list(o)
+        if not np.all(np.ediff1d(rngs) >= 0):
+            srt = np.argsort(rngs)
+            rngs, sills = rngs[srt], sills[srt]
+            skip.append(tuple(np.array(o)[srt]))
+            names = list(np.array(o)[srt])
+        c2 = np.all(np.ediff1d(rngs) >= 0)
+        if c1 and c2:
+            min_obj = aic
+            best.append((names, rngs, sills, fun, aic))
+        if len(best) > n_best:
+            del best[0]
+    return best, skip, funs
+
+
+def vg_strings(best):
+    """vgs.py:826-846: '%0.5f %s(%0.5f)' % (sill, name, range) joined by
+    ' + ', one string per kept model."""
+    return [' + '.join('%0.5f %s(%0.5f)' % (s, nm, r)
+                       for nm, r, s in zip(nms, rr, ss))
+            for nms, rr, ss, _, _ in best]
+
+
 def fit(x, y, z, mdr=0.5, nk=50, perm_r_list=(1,), fit_vgs=('Sph', 'Exp'),
         fil_nug_vg=None, fit_thresh=0.01, n_best=1, opt_iters=10000,
         evg='robust', use_wts=False, seed=None, range_ub=1e9):
@@ -184,40 +233,20 @@ def fit(x, y, z, mdr=0.5, nk=50, perm_r_list=(1,), fit_vgs=('Sph', 'Exp'),
     bins = empirical(x, y, z, mdr, nk, evg)
     h, g, npairs = bins[:, 0], bins[:, 1], bins[:, 2]
     wts = npairs if use_wts else None
-    min_obj = np.inf
-    best = []
-    skip = []
-    funs = {}
     rng = np.random.default_rng(seed)
-    for o in candidate_sets(fit_vgs, perm_r_list, fil_nug_vg):
-        if o in skip:
-            continue
+
+    def solve(o):
         bnds = bounds_for(o, h[0], g[0], var, fil_nug_vg, range_ub)
         res = differential_evolution(objective, bnds, args=(o, h, g, wts),
                                      maxiter=opt_iters,
                                      popsize=len(bnds) * 50, polish=False,
                                      seed=rng)
-        funs[o] = (res.fun, res.x.copy())
-        rngs = res.x[0::2].copy()
-        sills = res.x[1::2].copy()
-        aic = tot_pts * np.log(res.fun) + 2 * res.x.shape[0]
-        c1 = aic < min_obj * (1.0 - fit_thresh)
-        names = list(o)
-        if not np.all(np.ediff1d(rngs) >= 0):
-            srt = np.argsort(rngs)
-            rngs, sills = rngs[srt], sills[srt]
-            skip.append(tuple(np.array(o)[srt]))
-            names = list(np.array(o)[srt])
-        c2 = np.all(np.ediff1d(rngs) >= 0)
-        if c1 and c2:
-            min_obj = aic
-            best.append((names, rngs, sills, res.fun, aic))
-        if len(best) > n_best:
-            del best[0]
-    strs = [' + '.join('%0.5f %s(%0.5f)' % (s, nm, r)
-                       for nm, r, s in zip(nms, rr, ss))
-            for nms, rr, ss, _, _ in best]
-    return strs, dict(bins=bins, best=best, var=var, funs=funs)
+        return res.fun, res.x
+
+    best, _, funs = select_models(
+        candidate_sets(fit_vgs, perm_r_list, fil_nug_vg), solve, tot_pts,
+        fit_thresh, n_best)
+    return vg_strings(best), dict(bins=bins, best=best, var=var, funs=funs)
 
 
 def fit_step(x, y, z, min_vld_stns=10, drop_zeros=False, **kw):

@@ -396,3 +396,132 @@ def test_krige_exact_edk_reproduces_drift_and_is_shift_invariant():
     e_p, _, ok = O.krige_contract(np.linalg.pinv(K), R, z, dr, False)
     assert ok.all()
     np.testing.assert_allclose(e1, e_p, rtol=1e-10, atol=1e-12)
+
+
+# ---- class API (pyx:251-719) ------------------------------------------------
+
+def test_ok_class_two_stations_closed_form():
+    """OrdinaryKriging with two stations: Gamma = [[0, g12], [g12, 0]] (zero
+    diagonal), so for a point with semivariances (g1, g2) the system solves
+    by hand: l1 = (1 - (g1 - g2) / g12) / 2, l2 = 1 - l1,
+    mu = g1 - g12 l2; est_vars = max(0, l1 g1 + l2 g2 + mu)."""
+    xi, yi = np.array([0.0, 3000.0]), np.array([0.0, 0.0])
+    zi = np.array([2.0, 5.0])
+    xk, yk = np.array([1000.0, 2500.0, -400.0]), np.array([0.0, 700.0, 300.0])
+    model = '0.3 Nug(0.0) + 1.2 Sph(5000.0)'
+    out = O.ordinary_kriging(xi, yi, zi, xk, yk, model)
+
+    def gam(d):
+        return 0.3 + 1.2 * (1.5 * d / 5000 - 0.5 * (d / 5000) ** 3)
+    g12 = gam(3000.0)
+    for c in range(3):
+        d1 = math.hypot(xk[c] - xi[0], yk[c] - yi[0])
+        d2 = math.hypot(xk[c] - xi[1], yk[c] - yi[1])
+        g1, g2 = gam(d1), gam(d2)
+        l1 = (1 - (g1 - g2) / g12) / 2
+        l2 = 1 - l1
+        mu = g1 - g12 * l2
+        assert math.isclose(out['zk'][c], l1 * 2.0 + l2 * 5.0, rel_tol=1e-12)
+        assert math.isclose(out['mus'][c], mu, rel_tol=1e-11, abs_tol=1e-13)
+        assert math.isclose(out['est_vars'][c],
+                            max(0.0, l1 * g1 + l2 * g2 + mu), rel_tol=1e-11)
+
+
+def test_ok_class_exact_at_stations_and_weights_sum_to_one():
+    """Zero-diagonal Gamma and the rhs skipping d == 0 make lambda = e_j,
+    mu = 0 the solution at station j even with a nugget: the value is
+    reproduced and est_vars is 0 there; elsewhere sum(lambda) = 1."""
+    sx, sy, cx, cy, rr, dr, z, vg = _system_case(
+        vg='0.25 Nug(0.0) + 1 Exp(4000.0)')
+    xk = np.concatenate([sx[:3], cx])
+    yk = np.concatenate([sy[:3], cy])
+    out = O.ordinary_kriging(sx, sy, z, xk, yk, '0.25 Nug(0.0) + 1 Exp(4000.0)')
+    np.testing.assert_allclose(out['lambdas'][:3], np.eye(z.size)[:3],
+                               atol=1e-10)
+    np.testing.assert_allclose(out['zk'][:3], z[:3], rtol=1e-10)
+    np.testing.assert_allclose(out['est_vars'][:3], 0.0, atol=1e-10)
+    np.testing.assert_allclose(out['lambdas'].sum(1), 1.0, atol=1e-12)
+    assert (out['est_vars'][3:] > 0.25).all()   # nugget + spatial part
+
+
+def test_sk_class_init_order_quirk():
+    """SimpleKriging fills in_covars from self.covar before the sills are
+    summed (pyx:395-399, 413, 439), so K = -Gamma with a zero diagonal
+    (Gamma from fill_vg_var_arr's semivariances); the weights are then the
+    minimum-norm least-squares solution of K lambda = rhs (lstsq)."""
+    sx, sy, cx, cy, rr, dr, z, vg = _system_case(vg='1.5 Sph(6000.0)')
+    out = O.simple_kriging(sx, sy, z, cx, cy, '1.5 Sph(6000.0)')
+    G = np.empty(rr.shape)
+    O.fill_vg_var_arr(rr, G, 0, 1, '1.5 Sph(6000.0)', -np.inf)
+    np.fill_diagonal(G, 0.0)
+    np.testing.assert_array_equal(out['in_covars'], -G)
+    lam = np.linalg.lstsq(-G, out['rhss'].T, rcond=None)[0].T
+    np.testing.assert_allclose(out['lambdas'], lam, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(out['zk'], lam @ z, rtol=1e-9)
+    cov = np.empty(dr.shape)
+    O.fill_vg_var_arr(dr, cov, 1, 0, '1.5 Sph(6000.0)', -np.inf)
+    np.testing.assert_allclose(out['rhss'], cov, rtol=1e-15)
+
+
+def test_edk_md_class_reproduces_drifts():
+    """ExternalDriftKriging_MD: sum(lambda) = 1 and sum(lambda s_k) = s_k at
+    every point for each drift, a field linear in the drifts is reproduced
+    exactly, and lambda = e_j at station j."""
+    sx, sy, cx, cy, rr, dr, z, vg = _system_case(n=14)
+    si = np.stack([(0.3 * sx + 0.1 * sy) / 1e3, np.sin(sx / 2e3)])
+    sk = np.stack([(0.3 * cx + 0.1 * cy) / 1e3, np.sin(cx / 2e3)])
+    zl = 1.0 + 2.0 * si[0] - 0.5 * si[1]
+    out = O.external_drift_kriging_md(sx, sy, zl, si, cx, cy, sk, vg)
+    np.testing.assert_allclose(out['lambdas'].sum(1), 1.0, atol=1e-10)
+    np.testing.assert_allclose(out['lambdas'] @ si.T, sk.T, rtol=1e-9,
+                               atol=1e-10)
+    np.testing.assert_allclose(out['zk'], 1.0 + 2.0 * sk[0] - 0.5 * sk[1],
+                               rtol=1e-9)
+    at = O.external_drift_kriging_md(sx, sy, z, si, sx[:2], sy[:2],
+                                     si[:, :2], vg)
+    np.testing.assert_allclose(at['lambdas'], np.eye(sx.size)[:2],
+                               atol=1e-9)
+    assert at['mus_arr'].shape == (3, 2)
+
+
+# ---- step-driver helpers -------------------------------------------------------
+
+def test_neb_groups_partition_cells_by_identical_rows():
+    rng = np.random.default_rng(0)
+    idx = rng.integers(0, 3, (40, 2))          # few distinct rows
+    groups = O.neb_groups(idx)
+    allc = np.concatenate(groups)
+    assert sorted(allc.tolist()) == list(range(40))
+    uniq, inv = np.unique(idx, axis=0, return_inverse=True)
+    assert len(groups) == len(uniq)
+    for g in groups:
+        assert (idx[g] == idx[g[0]]).all()
+        assert len(set(inv[g].tolist())) == 1
+    firsts = [g[0] for g in groups]
+    assert firsts == sorted(firsts)             # order of first appearance
+    assert all((np.diff(g) > 0).all() for g in groups)
+
+
+def test_mod_min_max_clips_and_keeps_nan():
+    f = np.array([[-3.0, 0.5, np.nan], [7.0, 2.0, -0.0]])
+    g = f.copy()
+    O.mod_min_max(g, 0.0, 5.0)
+    np.testing.assert_array_equal(g, np.where(np.isnan(f), np.nan,
+                                              np.clip(f, 0.0, 5.0)))
+    g = f.copy()
+    O.mod_min_max(g, None, None)
+    np.testing.assert_array_equal(g, f)
+    g = f.copy()
+    O.mod_min_max(g, None, 1.0)
+    np.testing.assert_array_equal(g, np.where(np.isnan(f), np.nan,
+                                              np.minimum(f, 1.0)))
+
+
+def test_krige_solve_matches_pinv_when_well_conditioned():
+    sx, sy, cx, cy, rr, dr, z, vg = _system_case()
+    K, R = O.krig_system('OK', vg, rr, dr, None, None, 0.0)
+    e1, v1, ok1 = O.krige_contract(np.linalg.pinv(K), R, z, dr, True)
+    e2, v2, ok2 = O.krige_solve(K, R, z, dr, True)
+    assert ok1.all() and ok2.all()
+    np.testing.assert_allclose(e2, e1, rtol=1e-10)
+    np.testing.assert_allclose(v2, v1, rtol=1e-10)

@@ -65,3 +65,67 @@ def test_de_recovers_noise_free_parameters(name, r, s):
                                  popsize=100, polish=False, seed=3)
     assert res.fun < 1e-6 * np.sum(g ** 2)
     assert abs(res.x[0] / r - 1) < 2e-2 and abs(res.x[1] / s - 1) < 2e-2
+
+
+# ---- model choice and output strings (vgs.py:771-846), hand-built values ----
+
+def _fake(table):
+    calls = []
+
+    def solve(o):
+        calls.append(o)
+        return table[o]
+    return solve, calls
+
+
+def test_aic_choice_threshold_and_n_best():
+    """tot_pts 100: Sph fun 2.0 -> AIC 100 ln 2 + 4 = 73.31 (first: always
+    taken); Exp fun 1.99 -> 72.81, not below 0.99 x 73.31 = 72.58, so
+    rejected; Gau fun 1.9 -> 68.19, taken."""
+    cands = [('Sph',), ('Exp',), ('Gau',)]
+    table = {('Sph',): (2.0, [1000.0, 1.0]), ('Exp',): (1.99, [900.0, 1.1]),
+             ('Gau',): (1.9, [800.0, 1.2])}
+    solve, calls = _fake(table)
+    best, skip, _ = V.select_models(cands, solve, 100, 0.01, n_best=1)
+    assert calls == cands and skip == []
+    assert [b[0] for b in best] == [['Gau']]
+    assert math.isclose(best[0][4], 100 * math.log(1.9) + 4, rel_tol=1e-15)
+    # n_best = 2 keeps the last two accepted, oldest first (Exp rejected)
+    best2, _, _ = V.select_models(cands, _fake(table)[0], 100, 0.01, 2)
+    assert [b[0] for b in best2] == [['Sph'], ['Gau']]
+    assert V.vg_strings(best2) == ['1.00000 Sph(1000.00000)',
+                                   '1.20000 Gau(800.00000)']
+
+
+def test_aic_rule_with_negative_aic_as_written():
+    """With fun < 1 the AIC is negative and min_AIC (1 - 0.01) lies above
+    min_AIC: the reference's rule then takes a candidate that is slightly
+    worse.  Sph fun 0.5 -> AIC -65.31; Exp fun 0.51 -> -63.34, below
+    0.99 x -65.31 = -64.66?  No: -63.34 > -64.66, rejected; Gau fun
+    0.502 -> -64.91 < -64.66, taken although its fun is larger."""
+    cands = [('Sph',), ('Exp',), ('Gau',)]
+    table = {('Sph',): (0.5, [1.0, 1.0]), ('Exp',): (0.51, [1.0, 1.0]),
+             ('Gau',): (0.502, [1.0, 1.0])}
+    best, _, _ = V.select_models(cands, _fake(table)[0], 100, 0.01)
+    assert [b[0] for b in best] == [['Gau']]
+
+
+def test_out_of_order_ranges_are_sorted_and_skipped():
+    """A nested candidate whose ranges descend is sorted (names and sills
+    with it), accepted in sorted order, and its sorted permutation is not
+    fitted when it comes up later."""
+    cands = [('Nug', 'Sph'), ('Nug', 'Exp'), ('Sph', 'Nug')]
+    table = {('Nug', 'Sph'): (1.0, [500.0, 0.1, 200.0, 2.0]),
+             ('Nug', 'Exp'): (0.999, [500.0, 0.1, 900.0, 2.0]),
+             ('Sph', 'Nug'): (0.1, [1.0, 1.0, 2.0, 2.0])}
+    solve, calls = _fake(table)
+    best, skip, _ = V.select_models(cands, solve, 50, 0.01, n_best=2)
+    assert skip == [('Sph', 'Nug')]
+    assert ('Sph', 'Nug') not in calls
+    assert best[0][0] == ['Sph', 'Nug']
+    np.testing.assert_array_equal(best[0][1], [200.0, 500.0])
+    np.testing.assert_array_equal(best[0][2], [2.0, 0.1])
+    # AIC 50 ln 1 + 8 = 8; Exp nested: 50 ln 0.999 + 8 = 7.95 > 0.99 x 8
+    assert len(best) == 1
+    assert V.vg_strings(best) == [
+        '2.00000 Sph(200.00000) + 0.10000 Nug(500.00000)']
