@@ -182,7 +182,8 @@ def main():
     kern = {}
     for nm in ('nearest', 'kgen', 'gemm_tri_f32', 'gemm_tri_f64',
                'slice_cols', 'gemm_tri_i8', 'var_reduce', 'potrf', 'trtri',
-               'dual', 'slice_rows', 'assemble', 'special', 'cast_tri'):
+               'dual', 'slice_rows', 'assemble', 'special', 'cast_tri',
+               'pinv', 'pinv_contract'):
         ms, cnt = prof_get(nm)
         kern[nm] = (ms, cnt)
     macs32 = prof_work('gemm_tri_f32_macs')
@@ -247,8 +248,12 @@ def main():
     eb, vb = eb.cpu().numpy(), vb.cpu().numpy()
 
     def rel(a, b):
+        # relative error, floor 1e-3 x the mean |reference|; equal values
+        # count as 0 (steps whose variances are all 0)
         f = 1e-3 * np.abs(b).mean(axis=-1, keepdims=True)
-        return (np.abs(a - b) / np.maximum(np.abs(b), f)).max(axis=-1)
+        d = np.abs(a - b)
+        r = d / np.maximum(np.abs(b), f)
+        return np.where(d == 0, 0.0, r).max(axis=-1)
 
     with np.errstate(invalid='ignore', divide='ignore'):
         re_b, rv_b = rel(em_s, eb), rel(vm_s, vb)
@@ -261,7 +266,12 @@ def main():
               'flagged_est_max': float(re_b[fl_b].max()) if fl_b.any()
               else None,
               'flagged_var_max': float(rv_b[fl_b].max()) if fl_b.any()
-              else None}
+              else None,
+              'nan_variance_steps_mixed': int(np.isnan(vm_s[solved_t])
+                                              .any(axis=1).sum()),
+              'nan_variance_steps_fp64b': int(np.isnan(vb[solved_t])
+                                              .any(axis=1).sum()),
+              'pinv_steps': int(((info_t & _lib.ST_PINV) != 0).sum())}
     log(f'gate (mixed vs batched FP64, all timed steps): {gate_b}')
     # (a) a spread of timed steps, and the unflagged ones of largest
     #     condition estimate, against the exact solution of the reference's

@@ -146,7 +146,9 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
 /* Per-step status written to info[t] (bit flags). */
 #define SPK_ST_NOSTNS    1   /* no station: outputs untouched (NaN)        */
 #define SPK_ST_MEANFILL  2   /* interp flag off or nugget-only: mean fill  */
-#define SPK_ST_FAILED    4   /* factorisation failed: nearest-station fill */
+#define SPK_ST_FAILED    4   /* no solution: nearest-station fill (the
+                                Cholesky of the covariance block and the
+                                pseudo-inverse fallback both failed)      */
 #define SPK_ST_ONESTN    8   /* one station: its value everywhere          */
 #define SPK_ST_ILLCOND  16   /* results are set by rounding at the 1e-6
                                 level (as the reference's pinv's are):
@@ -170,6 +172,15 @@ enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
                                 spk_opts.fp64var_kappa): the variance
                                 contraction used exact int8 plane
                                 products (unless FP64VAR is also set)    */
+#define SPK_ST_PINV    128   /* the Cholesky of the covariance block failed
+                                (indefinite or numerically singular): the
+                                step was solved as the reference solves
+                                every step, lambda = pinv(K) rhs per cell
+                                with K the bordered system, pinv from a
+                                symmetric eigendecomposition with numpy's
+                                cutoff 1e-15 x the largest |eigenvalue|
+                                (interp/steps.py:350-371, 403-435); also
+                                SPK_ST_ILLCOND                           */
 
 typedef struct spk_problem {
   int64_t n_stns;               /* N                                      */

@@ -27,7 +27,7 @@ P_FP64, P_MIXED, P_FP64B = range(3)
 PRECISIONS = {'fp64': P_FP64, 'mixed': P_MIXED, 'fp64b': P_FP64B}
 
 ST_NOSTNS, ST_MEANFILL, ST_FAILED, ST_ONESTN = 1, 2, 4, 8
-ST_ILLCOND, ST_FP64VAR, ST_INT8VAR = 16, 32, 64
+ST_ILLCOND, ST_FP64VAR, ST_INT8VAR, ST_PINV = 16, 32, 64, 128
 
 
 class SpkVg(ctypes.Structure):

@@ -17,7 +17,12 @@ HERE = Path(__file__).resolve().parent
 SRC = HERE / 'csrc'
 OUT = HERE / 'libspk.so'
 SOURCES = ['spk_core.hip', 'spk_factor.hip', 'spk_cells.hip', 'spk_neb.hip',
-           'spk_vgfit.hip', 'spk_mlp.hip', 'spk_class.hip', 'spk_int8.hip']
+           'spk_vgfit.hip', 'spk_mlp.hip', 'spk_class.hip', 'spk_int8.hip',
+           'spk_pinv.hip']
+# rocSOLVER (dsyevd) and rocBLAS (dgemm) for the pseudo-inverse fallback of
+# steps whose covariance block does not factorise
+LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver',
+        '-lrocblas']
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
@@ -64,7 +69,7 @@ def build(verbose=False, jobs=None):
         raise RuntimeError('hipcc failed:\n' + '\n'.join(errs))
     tmp = OUT.with_suffix('.so.tmp')
     cmd = [hipcc, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', str(tmp)]
-    cmd += [str(o) for o in objs]
+    cmd += [str(o) for o in objs] + LIBS
     if verbose:
         print(' '.join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -85,9 +85,12 @@ __global__ __launch_bounds__(256) void k_rhs_plain(
     const double *__restrict__ cy, int64_t Cs, int64_t C,
     const double *__restrict__ cell_drift, int method, int m, int covar_flag,
     double min_vg_val, double *__restrict__ R, int64_t ldr,
-    int32_t *__restrict__ nnb) {
+    int32_t *__restrict__ nnb, const int *__restrict__ ccount) {
   if (sc->status) return;
   const int n = sc->n;
+  // ccount (device, optional): the cells of this launch, at most Cs
+  if (ccount) Cs = min(Cs, (int64_t)*ccount);
+  if (blockIdx.x * (int64_t)256 >= Cs) return;
   __shared__ double stx[256], sty[256];
   __shared__ spk_vg vg;
   const int tid = threadIdx.x;
@@ -108,7 +111,9 @@ __global__ __launch_bounds__(256) void k_rhs_plain(
     if (valid) {
       for (int j = 0; j < cnt; ++j) {
         const double d = dist2(x, y, stx[j], sty[j]);
-        if (d < best) {
+        // ties go to the lower station index (np.argmin), whatever order
+        // the step's stations are compacted in
+        if (d < best || (d == best && idx[base + j] < idx[bi])) {
           best = d;
           bi = base + j;
         }
@@ -157,8 +162,12 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
     double *__restrict__ est, double *__restrict__ var, int64_t ldo,
     int sel, double *__restrict__ qpart, int nrb,
     const uint8_t *__restrict__ nzb, int64_t nzk,
-    unsigned long long *__restrict__ nexec) {
+    unsigned long long *__restrict__ nexec,
+    const int *__restrict__ ccount, const int32_t *__restrict__ cmap) {
   using MF = Mfma<T>;
+  // ccount (device, optional): the cells of this launch, at most Cs;
+  // cmap (PLAIN, optional): output index of each cell
+  if (ccount) Cs = min(Cs, (int64_t)*ccount);
   using V = typename MF::V;
   constexpr int BM = 64, BN = 128;
   // PLAIN: grid (cell tiles, steps), each block walks all row blocks.
@@ -411,15 +420,16 @@ __global__ __launch_bounds__(256, PLAIN ? 1 : 3) void k_gemm(
       // exactly, so their sum's distance from one is rounding of the
       // explicit inverse
       if (method != SPK_M_SK && fabs(acc_l[tj] - 1.0) > kSumTol) ill = true;
+      const int64_t oc = cmap ? (int64_t)cmap[col] : col;
       if (skm) {
         // SK with the step mean m as known mean: m + lambda^T (z - m)
-        eo[col] = acc_e[tj] + st.mean * (1.0 - acc_l[tj]);
+        eo[oc] = acc_e[tj] + st.mean * (1.0 - acc_l[tj]);
       } else if (isclose1(acc_l[tj])) {
-        eo[col] = acc_e[tj];
-        if (want_var) vo[col] = acc_v[tj] + acc_m[tj];
+        eo[oc] = acc_e[tj];
+        if (want_var) vo[oc] = acc_v[tj] + acc_m[tj];
       } else {
-        eo[col] = z[nnb[(int64_t)sl * ldn + col]];
-        if (want_var) vo[col] = 0.0;
+        eo[oc] = z[nnb[(int64_t)sl * ldn + col]];
+        if (want_var) vo[oc] = 0.0;
       }
     } else {
       // this row block's share of |L^-1 delta|^2
@@ -1651,7 +1661,22 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     int8_t *L8;  // int8 planes of L^-1 rows (mixed path)
     int *ea;     // their row exponents
     StepSc *sc;
+    // steps of the batch whose Cholesky failed: solved by the
+    // pseudo-inverse fallback (spk_pinv.hip) and the plain contraction
+    std::vector<int> pinv;
   } set[2] = {};
+  // pinned host copy of a batch's step records, read after its Cholesky
+  {
+    const size_t need = (size_t)S * sizeof(StepSc);
+    if (ctx->h_sc_bytes < need) {
+      if (ctx->h_sc) SPK_HIP(hipHostFree(ctx->h_sc));
+      ctx->h_sc = nullptr;
+      ctx->h_sc_bytes = 0;
+      SPK_HIP(hipHostMalloc(&ctx->h_sc, need));
+      ctx->h_sc_bytes = need;
+    }
+  }
+  StepSc *h_sc = static_cast<StepSc *>(ctx->h_sc);
   // int8 plane layout: rows padded to R (a multiple of the 128-row tile),
   // k to KS (a multiple of the 64-deep MFMA)
   const int64_t R8 = round_up(N, 128), KS8 = round_up(N, 64);
@@ -1853,6 +1878,14 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          q.sc, gb, method == SPK_M_SK ? 0 : 1);
     }
     SPK_TRY(check_launch("k_potrf"));
+    // which steps failed the Cholesky (the host waits for this batch's
+    // factorisation while the previous batch's cells contract)
+    SPK_HIP(hipMemcpyAsync(h_sc, q.sc, (size_t)Sb * sizeof(StepSc),
+                           hipMemcpyDeviceToHost, sf));
+    SPK_HIP(hipStreamSynchronize(sf));
+    q.pinv.clear();
+    for (int s2 = 0; s2 < Sb; ++s2)
+      if (h_sc[s2].status == SPK_ST_FAILED) q.pinv.push_back(s2);
     {
       ProfScope ps(ctx, "trtri", sf);
       hipLaunchKernelGGL(k_trtri, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
@@ -1886,6 +1919,16 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       }
     }
     SPK_TRY(check_launch("factorise"));
+    // pseudo-inverses of the failed steps' bordered systems, written where
+    // the plain contraction reads K^-1: the M slab (plain path) or the
+    // step's L^-1 slab, which a failed step leaves unused (batched path)
+    for (int s2 : q.pinv) {
+      ProfScope ps(ctx, "pinv", sf);
+      SPK_TRY(pinv_step(ctx, sf, sx, sy, q.idx + (int64_t)s2 * N, q.sc + s2,
+                        h_sc[s2].n, gb + s2, method, m, prob->stn_drift, mvg,
+                        q.A + s2 * strideM,
+                        (batched ? q.Li : q.M) + s2 * strideM, ld));
+    }
     return SPK_OK;
   };
 
@@ -1897,6 +1940,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     double *vbo = want_var ? var + t0 * C : nullptr;
     if (!batched) {
       for (int s = 0; s < Sb; ++s) {
+        if (std::find(q.pinv.begin(), q.pinv.end(), s) != q.pinv.end())
+          hipLaunchKernelGGL(k_mark_pinv, dim3(1), dim3(64), 0, sm, q.sc + s);
         for (int64_t cb = 0; cb < C; cb += Cs) {
           const int64_t cn = std::min<int64_t>(Cs, C - cb);
           {
@@ -1905,7 +1950,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                 k_rhs_plain, dim3((unsigned)ceil_div(cn, 256)), dim3(256), 0,
                 sm, sx, sy, q.idx + s * N, q.sc + s, gb + s, cx + cb, cy + cb,
                 cn, C, prob->cell_drift ? prob->cell_drift + cb : nullptr,
-                method, m, method == SPK_M_SK ? 1 : 0, mvg, R, ldk, nnb);
+                method, m, method == SPK_M_SK ? 1 : 0, mvg, R, ldk, nnb,
+                (const int *)nullptr);
           }
           SPK_TRY(check_launch("k_rhs_plain"));
           {
@@ -1920,7 +1966,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                eb + s * C + cb,
                                vbo ? vbo + s * C + cb : nullptr, C, 0,
                                (double *)nullptr, 0, (const uint8_t *)nullptr,
-                               (int64_t)0, (unsigned long long *)nullptr);
+                               (int64_t)0, (unsigned long long *)nullptr,
+                               (const int *)nullptr, (const int32_t *)nullptr);
           }
           SPK_TRY(check_launch("k_gemm<double,plain>"));
         }
@@ -2000,7 +2047,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              (const int32_t *)nullptr, (int64_t)0, abuf, ldk,
                              cn, eb + cb, vbo + cb, C, 1, qpart, nrb,
                              (const uint8_t *)nzb, nzk,
-                             prof_of(ctx).on ? prof_of(ctx).work : nullptr);
+                             prof_of(ctx).on ? prof_of(ctx).work : nullptr,
+                             (const int *)nullptr, (const int32_t *)nullptr);
         }
         {
           ProfScope ps(ctx, "gemm_tri_f64");
@@ -2020,6 +2068,72 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         }
         SPK_TRY(check_launch("k_var_reduce"));
       }
+      // steps solved by the pseudo-inverse fallback: estimates and the
+      // reference's per-cell sum test from the dual vectors of pinv(K)
+      // (O(n) per cell); the plain contraction then gives the OK variance
+      // (and the reference's estimate arithmetic) of the cells that pass
+      if (!q.pinv.empty()) {
+        const int64_t Csp = std::min<int64_t>(
+            C, std::max<int64_t>(
+                   256, (int64_t)(rhs_budget / ((N + p + 64) * 8))));
+        const int64_t ldkp = round_up(Csp, 128);
+        const int nstr = (int)ceil_div(C, Csp);
+        const int want_list = (method == SPK_M_OK && want_var) ? 1 : 0;
+        double *bwp, *gxy = nullptr, *Rp = nullptr;
+        int32_t *plist, *nnbp = nullptr;
+        int *pcnt;
+        SPK_TRY(ws_get(ctx, "pinv_bw", 2 * ld * 8, (void **)&bwp));
+        SPK_TRY(ws_get(ctx, "pinv_list", C * 4, (void **)&plist));
+        SPK_TRY(ws_get(ctx, "pinv_cnt", (1 + nstr) * 4, (void **)&pcnt));
+        if (want_list) {
+          SPK_TRY(ws_get(ctx, "pinv_gxy", 2 * C * 8, (void **)&gxy));
+          SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldkp * 8, (void **)&Rp));
+          SPK_TRY(ws_get(ctx, "nnb", ldkp * 4, (void **)&nnbp));
+        }
+        for (int s : q.pinv) {
+          ProfScope ps(ctx, "pinv_contract");
+          const double *Ms = q.Li + s * strideM;
+          hipLaunchKernelGGL(k_mark_pinv, dim3(1), dim3(64), 0, sm, q.sc + s);
+          hipLaunchKernelGGL(k_pinv_vec, dim3((unsigned)ceil_div(N + p, 256)),
+                             dim3(256), 0, sm, Ms, ld, p, q.sc + s,
+                             q.zact + s * ldz, bwp);
+          hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, sm, pcnt, 1, 0);
+          hipLaunchKernelGGL(k_pinv_cells, dim3((unsigned)ceil_div(C, 256)),
+                             dim3(256), 0, sm, sx, sy, q.idx + s * N, q.sc + s,
+                             gb + s, cx, cy, C, C, (int64_t)0,
+                             prob->cell_drift, method, m, skm, mvg, bwp, ld,
+                             q.zact + s * ldz, eb + s * C,
+                             vbo ? vbo + s * C : nullptr, want_list, plist,
+                             pcnt);
+          SPK_TRY(check_launch("k_pinv_cells"));
+          if (!want_list) continue;
+          hipLaunchKernelGGL(
+              k_pinv_gather,
+              dim3((unsigned)std::min<int64_t>(ceil_div(C, 256), 4096)),
+              dim3(256), 0, sm, plist, pcnt, cx, cy, gxy, gxy + C, Csp, nstr,
+              pcnt + 1);
+          for (int k2 = 0; k2 < nstr; ++k2) {
+            const int64_t off = (int64_t)k2 * Csp;
+            const int64_t cn = std::min<int64_t>(Csp, C - off);
+            hipLaunchKernelGGL(
+                k_rhs_plain, dim3((unsigned)ceil_div(cn, 256)), dim3(256), 0,
+                sm, sx, sy, q.idx + s * N, q.sc + s, gb + s, gxy + off,
+                gxy + C + off, cn, C, (const double *)nullptr, method, m, 0,
+                mvg, Rp, ldkp, nnbp, (const int *)(pcnt + 1 + k2));
+            hipLaunchKernelGGL((k_gemm<double, true, 16>),
+                               dim3((unsigned)ceil_div(cn, 128), 1), dim3(256),
+                               0, sm, Ms, ld, (int64_t)0, Rp, ldkp, (int64_t)0,
+                               q.sc + s, p, method, 1, skm, q.zact + s * ldz,
+                               ldz, nnbp, ldkp, (const double *)nullptr,
+                               (int64_t)0, cn, eb + s * C, vbo + s * C, C, 0,
+                               (double *)nullptr, 0, (const uint8_t *)nullptr,
+                               (int64_t)0, (unsigned long long *)nullptr,
+                               (const int *)(pcnt + 1 + k2),
+                               (const int32_t *)(plist + off));
+          }
+          SPK_TRY(check_launch("pinv contraction"));
+        }
+      }
     }
     {
       ProfScope ps(ctx, "special");
@@ -2037,10 +2151,16 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
 
   const int64_t nb = ceil_div(T, S);
   auto bsize = [&](int64_t b) { return (int)std::min<int64_t>(S, T - b * S); };
+  // Batch b's contraction is queued before batch b + 1's factorisation:
+  // the host reads each batch's Cholesky outcome (for the pseudo-inverse
+  // fallback) while the previous batch's cells contract.
   SPK_TRY(factorise(0, bsize(0), set[0]));
   if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[0], sf));
   for (int64_t b = 0; b < nb; ++b) {
     const int k = (int)(b & 1);
+    if (sf != sm) SPK_HIP(hipStreamWaitEvent(sm, ev_fact[k], 0));
+    SPK_TRY(contract(b * S, bsize(b), set[nsets > 1 ? k : 0]));
+    if (sf != sm) SPK_HIP(hipEventRecord(ev_done[k], sm));
     if (b + 1 < nb) {
       const int k2 = k ^ 1;
       // set k2 was last read by the contraction of batch b - 1
@@ -2048,9 +2168,6 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       SPK_TRY(factorise((b + 1) * S, bsize(b + 1), set[nsets > 1 ? k2 : 0]));
       if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[k2], sf));
     }
-    if (sf != sm) SPK_HIP(hipStreamWaitEvent(sm, ev_fact[k], 0));
-    SPK_TRY(contract(b * S, bsize(b), set[nsets > 1 ? k : 0]));
-    if (sf != sm) SPK_HIP(hipEventRecord(ev_done[k], sm));
   }
   // nothing may remain queued on the factor stream after return
   if (sf != sm) {

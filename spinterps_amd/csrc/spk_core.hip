@@ -402,6 +402,8 @@ int spk_ctx_trim(spk_ctx *ctx) {
 int spk_ctx_destroy(spk_ctx *ctx) {
   if (!ctx) return SPK_OK;
   int rc = spk_ctx_trim(ctx);
+  pinv_release(ctx);
+  if (ctx->h_sc) hipHostFree(ctx->h_sc);
   for (auto ev : ctx->events) hipEventDestroy(ev);
   for (auto ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

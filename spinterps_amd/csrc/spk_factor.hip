@@ -94,6 +94,7 @@ __global__ __launch_bounds__(256) void k_compact(
     o.status = st;
     o.flags = 0;
     o.planes = 0;
+    o.pinv = 0;
     o.kappa = 0.0;
     o.c0 = 0.0;
     o.b = 0.0;

@@ -57,6 +57,12 @@ struct spk_ctx {
   // a context that has sorted station pairs (that order stalled with no
   // cause found; simulations before any pair sort are unaffected)
   bool pairs_sorted = false;
+  // pseudo-inverse fallback of failed steps: a rocBLAS handle (rocSOLVER
+  // eigendecompositions and the reassembly GEMM) and pinned host copies
+  // of the step records, read after each batch's Cholesky
+  void *blas = nullptr;
+  void *h_sc = nullptr;
+  size_t h_sc_bytes = 0;
 };
 
 namespace spk {

@@ -16,7 +16,8 @@ struct StepSc {
   int n;          // active stations
   int status;     // SPK_ST_* that skip the step's regular kernels (0 = run)
   int flags;      // informational SPK_ST_* bits (ill-conditioned, FP64 var)
-  int planes;     // int8 planes of the variance contraction (0, 4 or 5)
+  int planes;     // int8 planes of the variance contraction (0, 5 or 6)
+  int pinv;       // 1: a failed step's pseudo-inverse is ready (spk_pinv)
   double kappa;   // max diag(C^-1) * max diag(C), a condition estimate
   double mean;    // mean of the active values (interp/steps.py:276)
   double c0;      // covariance shift of the OK/EDK system
@@ -26,6 +27,35 @@ struct StepSc {
   double h[kMaxP];
   double dh[kMaxP];  // last refinement correction of h (dual form)
 };
+
+// ---- pseudo-inverse fallback of failed steps (spk_pinv.hip) ----------------
+// Builds step s's bordered system in A_s (ld), eigendecomposes it and
+// writes pinv(K) to M_s (ld), all on stream st; records sc_s->pinv and
+// sc_s->kappa on the device.  n_s: the step's active stations.
+int pinv_step(spk_ctx *ctx, hipStream_t st, const double *sx,
+              const double *sy, const int32_t *idx_s, StepSc *sc_s,
+              int n_s, const spk_vg *vg_s, int method, int m,
+              const double *stn_drift, double min_vg_val, double *A_s,
+              double *M_s, int64_t ld);
+// The step leaves SPK_ST_FAILED for the plain contraction when its
+// pseudo-inverse is ready (SPK_ST_PINV | SPK_ST_ILLCOND).
+__global__ void k_mark_pinv(StepSc *sc);
+__global__ void k_pinv_vec(const double *M, int64_t ld, int p,
+                           const StepSc *sc, const double *z, double *bw);
+__global__ void k_pinv_cells(const double *sx, const double *sy,
+                             const int32_t *idx, const StepSc *sc,
+                             const spk_vg *vgp, const double *cx,
+                             const double *cy, int64_t Cs, int64_t C,
+                             int64_t cb, const double *cell_drift, int method,
+                             int m, int skm, double min_vg_val,
+                             const double *bw, int64_t ld, const double *z,
+                             double *est, double *var, int want_list,
+                             int32_t *list, int *cnt);
+__global__ void k_pinv_gather(const int32_t *list, const int *cnt,
+                              const double *cx, const double *cy, double *gx,
+                              double *gy, int64_t Csp, int nstripes,
+                              int *scnt);
+void pinv_release(spk_ctx *ctx);
 
 // ---- scratch of the allocation-free IDW / nearest-neighbour entry point ----
 // spk_idw_steps_noalloc_dev takes every buffer beyond its inputs and outputs

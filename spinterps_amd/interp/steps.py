@@ -252,7 +252,8 @@ def interpolate(prob: DeviceProblem, vals, vgs, methods, *,
             continue
         e, var, info = interp_steps(
             prob, v, vgs, typ, precision=precision,
-            want_var=est_vars_flag and typ == 'OK', idw_exp=idw_exp,
+            want_var=est_vars_flag and typ == 'OK',
+            idw_exp=idw_exp if typ == 'IDW' else 2.0,
             min_vg_val=min_vg_val, min_var_thr=min_var_thr, vg_dev=vg_dev)
         f64[lab] = e
         fields[lab] = clip_cast(e, min_var_cut, max_var_cut)

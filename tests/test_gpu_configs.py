@@ -295,3 +295,87 @@ def test_simulate_after_pair_sort_refused(gpu):
                 mu.data_ptr(), 5, out.data_ptr(), info.data_ptr()))
     finally:
         ctx.close()
+
+
+# ---- pseudo-inverse fallback (steps whose Cholesky fails) -------------------
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp64b', 'mixed'])
+def test_pinv_fallback_indefinite_linear_model(gpu, precision):
+    """A bounded linear model over long distances gives an indefinite
+    covariance block (Cholesky fails) while the reference's bordered system
+    stays well conditioned (condition ~3e5): the reference's pinv is then
+    its inverse, and the fallback matches the exact solution like any step
+    at that condition number (1e-8: two float64 pseudo-inverses of a
+    matrix with condition 3e5 differ by ~1e-9; the per-cell sum test and
+    the variance formula included)."""
+    import torch
+
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.daily(n_stns=300, n_side=40, cell=12500.0)
+    vals, _ = configs.daily_steps(w, 0, 4)
+    vgs = ['1.0 Lin(150000.0)', '0.1 Nug(0.0) + 0.9 Lin(60000.0)',
+           '1.0 Sph(30000.0)', '2.0 Lin(120000.0)']
+    _, refp, prblm = O.interp_all(w.stn_x, w.stn_y, w.cell_x, w.cell_y,
+                                  vals, vgs, [('OK', 'OK', None)])
+    assert not prblm
+    # the exact solution (the system is well conditioned: pinv = inverse)
+    ref = {'OK': np.empty_like(refp['OK']),
+           'EST_VARS_OK': np.empty_like(refp['OK'])}
+    for t in range(4):
+        act = ~np.isnan(vals[t])
+        xs, ys = w.stn_x[act], w.stn_y[act]
+        ref['OK'][t], ref['EST_VARS_OK'][t], _ = O.krige_exact(
+            'OK', vgs[t], O.dists_2d(xs, ys, xs, ys),
+            O.dists_2d(w.cell_x, w.cell_y, xs, ys), vals[t][act],
+            want_var=True)
+    prob = DeviceProblem.from_workload(w)
+    kap = torch.zeros(4, dtype=torch.float64, device=gpu)
+    e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=precision,
+                              kappa=kap)
+    e, v, info = e.cpu().numpy(), v.cpu().numpy(), info.cpu().numpy()
+    fe = 1e-3 * np.abs(ref['OK']).mean(axis=1, keepdims=True)
+    fv = 1e-3 * np.abs(ref['EST_VARS_OK']).mean(axis=1, keepdims=True)
+    re = _rel(e, ref['OK'], fe).max(axis=1)
+    rv = _rel(v, ref['EST_VARS_OK'], fv).max(axis=1)
+    rp = _rel(refp['EST_VARS_OK'], ref['EST_VARS_OK'], fv).max(axis=1)
+    print(f'lin {precision}: flags {info.tolist()} kappa '
+          f'{kap.cpu().numpy()} est {re} var {rv}; the oracle pinv var {rp}')
+    assert not (info & _lib.ST_FAILED).any()
+    # the two long-range linear models do not factorise
+    assert (info[[0, 3]] & _lib.ST_PINV).all()
+    assert not (info[[0, 3]] & _lib.ST_ILLCOND).any()   # full rank
+    assert not info[2] & _lib.ST_PINV
+    assert re.max() <= 1e-8
+    assert rv.max() <= (1e-8 if precision != 'mixed' else 1e-6)
+
+
+def test_pinv_fallback_stress_steps(gpu):
+    """Stress steps whose covariance block is numerically singular: no step
+    is left to the nearest-station fill; every cell gets the reference's
+    contraction with pinv(K).  There the outcome depends on rounding (the
+    eigenvalues run continuously through numpy's cutoff), so the
+    difference from the oracle's pinv is reported, and bounded only by the
+    fields' own scale."""
+    from spinterps_amd.interp.steps import DeviceProblem, interp_steps
+    w = configs.daily()
+    sx, sy = configs.stress_stations(w)
+    vals, vgs = configs.stress_steps(w, 0, 12)
+    prob = DeviceProblem(sx, sy, w.cell_x, w.cell_y)
+    e, v, info = interp_steps(prob, vals, vgs, 'OK', precision='fp64')
+    info = info.cpu().numpy()
+    assert not (info & _lib.ST_FAILED).any()
+    pin = np.flatnonzero(info & _lib.ST_PINV)
+    assert len(pin) >= 3
+    cidx = _sample(w.n_cells, 100, 13)
+    rr = O.dists_2d(sx, sy, sx, sy)
+    dr = O.dists_2d(w.cell_x[cidx], w.cell_y[cidx], sx, sy)
+    e = e.cpu().numpy()[:, cidx]
+    for t in pin[:3]:
+        K, R = O.krig_system('OK', vgs[t], rr, dr, None, None, 0.0)
+        ep, vp, ok = O.krige_contract(np.linalg.pinv(K), R, vals[t], dr, True)
+        d = _rel(e[t], ep, 1e-3 * np.abs(ep).mean())
+        print(f'stress step {t} (pinv): vs the oracle pinv median '
+              f'{np.median(d):.1e} max {d.max():.1e}; oracle cells on the '
+              f'nearest station {int((~ok).sum())}')
+        assert np.isfinite(e[t]).all()
+        assert np.abs(e[t]).max() <= 100 * np.abs(vals[t]).max()
