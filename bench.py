@@ -197,9 +197,12 @@ def main():
     failed_t = (info_t & _lib.ST_FAILED) != 0
     ill_t = (info_t & _lib.ST_ILLCOND) != 0
     solved_t = ~special_t & ~failed_t
-    f64_steps = solved_t & ((info_t & _lib.ST_FP64VAR) != 0)
-    i8_steps = solved_t & ~f64_steps & ((info_t & _lib.ST_INT8VAR) != 0)
-    f32_steps = solved_t & ~f64_steps & ~i8_steps
+    pinv_t = (info_t & _lib.ST_PINV) != 0
+    # steps the batched contractions ran (not the pseudo-inverse fallback)
+    batched_t = solved_t & ~pinv_t
+    f64_steps = batched_t & ((info_t & _lib.ST_FP64VAR) != 0)
+    i8_steps = batched_t & ~f64_steps & ((info_t & _lib.ST_INT8VAR) != 0)
+    f32_steps = batched_t & ~f64_steps & ~i8_steps
     # algorithmic flops of W = L^-1 delta: n^2 per cell (n(n+1)/2 MACs)
     nsq = timed_n.astype(np.float64) ** 2
     fl32 = float(np.sum(nsq[f32_steps]) * C)
@@ -398,19 +401,24 @@ def main():
         tc1 = time.perf_counter()
         per_step = [ts + C * tc for ts, tc in res]
         all_rate = n_cores * C / float(np.mean(per_step))
-        ts1, tc1_ = pool.map(_cpu_step, jobs[:1])[0]
-        one_rate = C / (ts1 + C * tc1_)
+        # the 1-core figure: the same steps one after another on one worker
+        # (the first four, to bound the time)
+        one = [ts + C * tc for ts, tc in map(_cpu_step, jobs[:4])]
+        one_rate = C / float(np.mean(one))
         pool.close()
         cpu = {'value': all_rate, 'unit': 'cell-estimates/s',
                'cores': n_cores, 'kind': 'port',
                'one_core_value': one_rate,
+               'one_core_vs_parallel_same_steps': float(
+                   np.mean(per_step[:4]) / np.mean(one)),
                'sample': (f'{n_cores} timed hourly steps x '
                           f'{args.cpu_cells} cells, one per worker process '
                           f'with one BLAS thread, all {n_cores} at once '
                           f'({tc1 - tc0:.1f} s); numpy pinv per step + '
                           'np.matmul per cell (interp/steps.py:350-434); '
                           'rate = cores x C / mean(setup + C x per-cell '
-                          'time); one_core_value: the first step alone')}
+                          'time); one_core_value: the first 4 steps alone, '
+                          'one after another in this process')}
         log(f'cpu baseline: {cpu}')
 
     if args.calib:

@@ -58,6 +58,12 @@ struct Mfma<float> {
 // the plain path's error against an exact solution stays below 450 times
 // this deviation, so unflagged steps are within 5e-7.
 constexpr double kSumTol = 1e-9;
+// Condition estimate above which a batched step whose Cholesky succeeded is
+// solved by the pseudo-inverse of its bordered system instead: there the
+// Cholesky solve is rounding noise (the estimate can be off by orders of
+// magnitude), while pinv drops the eigenvalues under 1e-15 of the largest
+// as the reference does.
+constexpr double kPinvKappa = 1e14;
 // Relative change of a cell's estimate by the second refinement step above
 // which the step is flagged SPK_ST_ILLCOND (the change is of the size of
 // the error left by the first step; the refined estimate is at least as
@@ -1906,6 +1912,21 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                            mvg, fp64var_thr, fp64var_thr_gau, int8var_thr,
                            (const double *)sx, (const double *)sy);
       }
+      // Steps whose Cholesky succeeded with a condition estimate above
+      // kPinvKappa take the pseudo-inverse route too: their solve is
+      // rounding noise, where the reference's pinv truncates the
+      // eigenvalues under its cutoff (and sends most cells to the nearest
+      // station).  Marked failed before the slicing kernels, which skip them.
+      SPK_TRY(check_launch("k_dual"));
+      SPK_HIP(hipMemcpyAsync(h_sc, q.sc, (size_t)Sb * sizeof(StepSc),
+                             hipMemcpyDeviceToHost, sf));
+      SPK_HIP(hipStreamSynchronize(sf));
+      for (int s2 = 0; s2 < Sb; ++s2)
+        if (h_sc[s2].status == 0 && !(h_sc[s2].kappa <= kPinvKappa)) {
+          q.pinv.push_back(s2);
+          hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, sf,
+                             &q.sc[s2].status, 1, SPK_ST_FAILED);
+        }
       if (q.L8) {
         ProfScope ps(ctx, "slice_rows", sf);
         hipLaunchKernelGGL(k_slice_rows, dim3((unsigned)(R8 / 64), Sb),

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(256) void k_pinv_scale(
 __global__ void k_mark_pinv(StepSc *__restrict__ sc) {
   if (threadIdx.x == 0 && sc->status == SPK_ST_FAILED && sc->pinv > 0) {
     sc->status = 0;
-    sc->flags |= SPK_ST_PINV;
+    // the batched variance contraction did not run for the step
+    sc->flags = (sc->flags & ~(SPK_ST_FP64VAR | SPK_ST_INT8VAR)) | SPK_ST_PINV;
     if (sc->pinv == 2 || !(sc->kappa <= 1e12)) sc->flags |= SPK_ST_ILLCOND;
   }
 }
