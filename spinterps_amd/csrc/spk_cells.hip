@@ -16,8 +16,8 @@
 //                locally deflated covariances delta for the variance
 //  k_var_reduce  variance from the FP64 terms and the partial norms
 //  (the int8 plane-product contraction is in spk_int8.hip)
-//  k_idw_nnb     interp/steps.py:285-313 (IDW with per-cell max
-//                normalisation, nearest neighbour)
+//  k_idw_nnb     interp/steps.py:285-313 (nearest neighbour; IDW runs as
+//                a masked FP64 MFMA contraction, k_idw_mfma in spk_idw.hip)
 //  k_special     one-station, mean-fill and failed-system steps
 //                (interp/steps.py:280-283, 325-331, 372-384)
 #include <algorithm>
@@ -1474,9 +1474,19 @@ static int idw_nnb_steps(spk_ctx *ctx, const spk_opts *opts, int64_t T,
                        (const int32_t *)nullptr);
     {
       ProfScope ps(ctx, "idw_nnb");
-      hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
-                         dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,
-                         cy, C, method, opts->idw_exp, eb, bad);
+      if (method == SPK_M_IDW)
+        // num = (Z . M) W^T and den = M W^T on FP64 MFMA, W = d^-p
+        hipLaunchKernelGGL(k_idw_mfma,
+                           dim3((unsigned)ceil_div(C, 128),
+                                (unsigned)ceil_div(Sb, 32)),
+                           dim3(256), 0, sm, sx, sy, N, vals + t0 * N, Sb,
+                           (const StepSc *)sc, (const int32_t *)idx,
+                           (const double *)zact, ldz, cx, cy, C,
+                           opts->idw_exp, eb, bad);
+      else
+        hipLaunchKernelGGL(k_idw_nnb, dim3((unsigned)ceil_div(C, 256), Sb),
+                           dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc,
+                           cx, cy, C, method, opts->idw_exp, eb, bad);
     }
     hipLaunchKernelGGL(k_special, dim3((unsigned)ceil_div(C, 256), Sb),
                        dim3(256), 0, sm, sx, sy, idx, N, zact, ldz, sc, cx,

@@ -28,6 +28,14 @@ struct StepSc {
   double dh[kMaxP];  // last refinement correction of h (dual form)
 };
 
+// ---- IDW as a masked contraction on FP64 MFMA (spk_idw.hip) ----------------
+// grid (ceil(C / 128), ceil(S / 32)), 256 threads.
+__global__ void k_idw_mfma(const double *sx, const double *sy, int64_t N,
+                           const double *vals, int S, const StepSc *sc,
+                           const int32_t *idx, const double *zact,
+                           int64_t ldz, const double *cx, const double *cy,
+                           int64_t C, double p, double *est, int *bad);
+
 // ---- pseudo-inverse fallback of failed steps (spk_pinv.hip) ----------------
 // Builds step s's bordered system in A_s (ld), eigendecomposes it and
 // writes pinv(K) to M_s (ld), all on stream st; records sc_s->pinv and
