@@ -228,6 +228,43 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          const double *vals, const spk_vg *vgs, double *est,
                          double *var, int32_t *info, double *kappa);
 
+/* ---- neighbourhood interpolation ------------------------------------
+ * Each cell uses only n_nebs of the step's active stations: its nearest
+ * (SPK_NEB_NRST, interp/grps.py:147-166) or n_nebs taken rank by rank over
+ * n_pies angular sectors (SPK_NEB_PIE, interp/grps.py:168-247).  Stations
+ * that no cell selects with every station present are treated as missing
+ * (interp/steps.py:592-608).  Cells with the same station set form a
+ * group (interp/grps.py:103-139) whose flags, mean fill, nearest-station
+ * and kriging system are those of interp/steps.py:732-831 over the set.
+ * Replaces the per-group loop of interp/steps.py:732-831 with its
+ * _get_interp calls (interp/steps.py:245-401). */
+#define SPK_NEB_NRST 1
+#define SPK_NEB_PIE  2
+typedef struct spk_neb {
+  int32_t sel;                  /* SPK_NEB_NRST or SPK_NEB_PIE            */
+  int32_t n_nebs;               /* stations per cell, 1..64               */
+  int32_t n_pies;               /* SPK_NEB_PIE: sectors, 1..n_nebs        */
+  int32_t pad_;
+} spk_neb;
+
+/* spk_interp_steps_dev's arguments and outputs, FP64 throughout
+ * (opts->precision is not used).  info[t]: SPK_ST_NOSTNS for a step
+ * without stations, else the OR over the step's groups of SPK_ST_ONESTN
+ * (a one-station set), SPK_ST_MEANFILL (a group filled with its mean),
+ * SPK_ST_PINV (a group solved by the pseudo-inverse of its bordered
+ * system: its Cholesky failed or its condition estimate exceeded 1e14),
+ * SPK_ST_FAILED (that eigendecomposition failed too: nearest station) and
+ * SPK_ST_ILLCOND (condition estimate above 1e12, or eigenvalues dropped).
+ * kappa[t]: the largest condition estimate of the step's groups, or NULL.
+ * SPK_ESHAPE for pie steps with 1 < active stations < n_nebs (the
+ * reference's row assignment fails there); SPK_EINDEX for a sector index
+ * out of range (pyx:868-870). */
+int spk_interp_neb_steps_dev(spk_ctx *ctx, const spk_problem *prob,
+                             const spk_opts *opts, const spk_neb *neb,
+                             int64_t n_steps, const double *vals,
+                             const spk_vg *vgs, double *est, double *var,
+                             int32_t *info, double *kappa);
+
 /* No counterpart in the reference.  spk_interp_steps_dev for SPK_M_IDW and
  * SPK_M_NNB that allocates nothing, for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

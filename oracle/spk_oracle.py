@@ -627,7 +627,7 @@ def interp_all(stn_x, stn_y, cell_x, cell_y, data, vgs, methods, *,
                stn_drfts=None, cell_drfts=None, min_var_thr=-np.inf,
                min_var_cut=None, max_var_cut=None, min_vg_val=0.0,
                neb_sel_mthd='all', n_nebs=None, n_pies=None,
-               est_vars_flag=True):
+               est_vars_flag=True, stn_keep=None):
     """The numeric core of interp/steps.py:478-835 without files.
 
     data [T x N] with NaN for missing stations, vgs a length-T list of
@@ -650,6 +650,24 @@ def interp_all(stn_x, stn_y, cell_x, cell_y, data, vgs, methods, *,
         f64['EST_VARS_OK'] = np.full((T, C), np.nan)
     prblm = []
     steps = np.arange(T)
+    if neb_sel_mthd != 'all':
+        # stations that no cell selects with every station present are
+        # dropped before the time grouping (interp/steps.py:592-608); a
+        # dropped station behaves as one without values.  stn_keep: the
+        # kept stations of a larger grid whose cells these are a sample of
+        if stn_keep is not None:
+            neb0 = np.where(np.asarray(stn_keep, dtype=bool))[0]
+        elif neb_sel_mthd == 'nrst':
+            neb0 = neb_idxs_nrst(cell_x, cell_y, stn_x, stn_y, n_nebs)
+        elif neb_sel_mthd == 'pie':
+            neb0 = neb_idxs_pie(cell_x, cell_y, stn_x, stn_y, n_nebs, n_pies)
+        else:
+            raise NotImplementedError(neb_sel_mthd)
+        drop = np.ones(N, dtype=bool)
+        drop[np.unique(neb0)] = False
+        if drop.any():
+            data = data.copy()
+            data[:, drop] = np.nan
     dst_ref_all = dists_2d(cell_x, cell_y, stn_x, stn_y)
     ref_ref_all = dists_2d(stn_x, stn_y, stn_x, stn_y)
     for stns, tmask in time_groups(data):

@@ -53,6 +53,14 @@ class SpkOpts(ctypes.Structure):
                 ('int8var_kappa', c_double)]
 
 
+class SpkNeb(ctypes.Structure):
+    _fields_ = [('sel', c_int32), ('n_nebs', c_int32), ('n_pies', c_int32),
+                ('pad_', c_int32)]
+
+
+NEB_SEL = {'nrst': 1, 'pie': 2}
+
+
 VG_DTYPE = np.dtype([('n', '<i4'), ('type', '<i4', (VG_MAX,)),
                      ('sill', '<f8', (VG_MAX,)), ('range', '<f8', (VG_MAX,)),
                      ('sill_sum', '<f8'), ('range_max_raw', '<f8')],
@@ -100,6 +108,11 @@ SIGNATURES = {
                                      POINTER(SpkOpts), c_int64, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p]),
+    'spk_interp_neb_steps_dev': (c_int, [c_void_p, POINTER(SpkProblem),
+                                         POINTER(SpkOpts), POINTER(SpkNeb),
+                                         c_int64, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p,
+                                         c_void_p]),
     'spk_idw_steps_noalloc_dev': (c_int, [c_void_p, POINTER(SpkProblem),
                                           POINTER(SpkOpts), c_int64,
                                           c_void_p, c_void_p, c_void_p,

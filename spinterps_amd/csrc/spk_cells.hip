@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "spk_krige.h"
+#include "spk_prof.h"
 
 namespace spk {
 
@@ -1364,22 +1365,7 @@ __global__ void k_clip_cast(const double *__restrict__ in,
 }
 
 // ---- kernel profiling (hipEvents around the heavy launches) ----------------
-struct Prof {
-  bool on = false;
-  struct Rec {
-    std::string name;
-    hipEvent_t a, b;
-  };
-  std::vector<Rec> pending;
-  std::vector<hipEvent_t> pool;
-  std::vector<std::pair<std::string, std::pair<double, int64_t>>> stats;
-  // device counters of executed work: [0] 16-row chunks multiplied by the
-  // FP32 triangular contraction, [1] 64-k chunks times plane products of
-  // the int8 contraction
-  unsigned long long *work = nullptr;
-};
-
-static Prof &prof_of(spk_ctx *ctx) {
+Prof &prof_of(spk_ctx *ctx) {
   static std::vector<std::pair<spk_ctx *, Prof *>> all;
   for (auto &kv : all)
     if (kv.first == ctx) return *kv.second;
@@ -1387,7 +1373,7 @@ static Prof &prof_of(spk_ctx *ctx) {
   return *all.back().second;
 }
 
-static hipEvent_t prof_event(Prof &pf) {
+hipEvent_t prof_event(Prof &pf) {
   if (!pf.pool.empty()) {
     hipEvent_t e = pf.pool.back();
     pf.pool.pop_back();
@@ -1397,28 +1383,6 @@ static hipEvent_t prof_event(Prof &pf) {
   hipEventCreate(&e);
   return e;
 }
-
-struct ProfScope {
-  spk_ctx *ctx;
-  Prof &pf;
-  const char *name;
-  hipStream_t st;
-  hipEvent_t a{}, b{};
-  ProfScope(spk_ctx *c, const char *nm, hipStream_t s = nullptr)
-      : ctx(c), pf(prof_of(c)), name(nm), st(s ? s : c->stream) {
-    if (pf.on) {
-      a = prof_event(pf);
-      hipEventRecord(a, st);
-    }
-  }
-  ~ProfScope() {
-    if (pf.on) {
-      b = prof_event(pf);
-      hipEventRecord(b, st);
-      pf.pending.push_back({name, a, b});
-    }
-  }
-};
 
 static int prof_collect(spk_ctx *ctx) {
   Prof &pf = prof_of(ctx);

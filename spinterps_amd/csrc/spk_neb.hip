@@ -7,45 +7,9 @@
 // distances are ordered by station index (the reference's np.argsort is an
 // unstable quicksort there, and such ties have measure zero for continuous
 // coordinates).
-#include "spk_internal.h"
+#include "spk_neb.h"
 
 namespace spk {
-
-__device__ inline bool lex_less(double a, int ia, double b, int ib) {
-  return (a < b) || (a == b && ia < ib);
-}
-
-// Wave-wide arg-min of (d, i) pairs; every lane returns the winner.
-__device__ inline void wave_argmin(double &d, int &i) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const double od = __shfl_xor(d, off, 64);
-    const int oi = __shfl_xor(i, off, 64);
-    if (lex_less(od, oi, d, i)) {
-      d = od;
-      i = oi;
-    }
-  }
-}
-
-// Angular sector of station (rx, ry) seen from (x, y), pyx:850-868.
-__device__ inline long long pie_of(double x, double y, double rx, double ry,
-                                   long long n_pies) {
-  const double two_pi = 2.0 * kPi;
-  const double xd = rx - x, yd = ry - y;
-  double ang;
-  if (!(xd != 0.0)) {
-    ang = 0.0;
-  } else {
-    ang = atan(yd / xd);
-    if (xd < 0 && yd > 0)
-      ang = kPi + ang;
-    else if (xd < 0 && yd < 0)
-      ang = kPi + ang;
-    else if (xd > 0 && yd < 0)
-      ang = two_pi + ang;
-  }
-  return (long long)(ang * (double)n_pies / two_pi);
-}
 
 // nrst: the k nearest stations, indices sorted ascending
 // (interp/grps.py:147-166).
@@ -97,76 +61,16 @@ __global__ __launch_bounds__(64) void k_pie(
   const int64_t c = blockIdx.x;
   if (c >= n_dst) return;
   extern __shared__ double smem[];
-  double *sd = smem;                          // [n_refs]
-  double *cur_d = sd + n_refs;                // [n_pies]
-  double *cand_d = cur_d + n_pies;            // [n_pies]
-  int *sp = (int *)(cand_d + n_pies);         // [n_refs]
-  int *cur_i = sp + n_refs;                   // [n_pies]
-  int *sel = cur_i + n_pies;                  // [k]
-  int *cand = sel + k;                        // [n_pies]
   const int lane = threadIdx.x;
-  const double x = dx[c], y = dy[c];
-  for (int j = lane; j < n_refs; j += 64) {
-    sd[j] = dist2(x, y, rx[j], ry[j]);
-    const long long p = pie_of(x, y, rx[j], ry[j], n_pies);
-    if (p < 0 || p >= n_pies) atomicOr(bad, 1);
-    sp[j] = (int)min(max(p, 0ll), (long long)n_pies - 1);
+  int *sel = nullptr;
+  const int nsel = pie_select(
+      lane, dx[c], dy[c], n_refs, k, n_pies,
+      [&](int j) { return make_double2(rx[j], ry[j]); }, (char *)smem, &sel,
+      bad);
+  if (nsel < k) {
+    atomicOr(bad, 2);
+    return;
   }
-  for (int p = lane; p < n_pies; p += 64) {
-    cur_d[p] = -INFINITY;
-    cur_i[p] = -1;
-  }
-  __syncthreads();
-  int nsel = 0;
-  while (nsel < k) {
-    int ncand = 0;
-    for (int p = 0; p < n_pies; ++p) {
-      const double pd = cur_d[p];
-      const int pi = cur_i[p];
-      double bd = INFINITY;
-      int bi = 0x7fffffff;
-      for (int j = lane; j < n_refs; j += 64) {
-        if (sp[j] != p) continue;
-        const double d = sd[j];
-        if (lex_less(pd, pi, d, j) && lex_less(d, j, bd, bi)) {
-          bd = d;
-          bi = j;
-        }
-      }
-      wave_argmin(bd, bi);
-      __syncthreads();
-      if (bi != 0x7fffffff) {
-        if (lane == 0) {
-          cand[ncand] = bi;
-          cand_d[ncand] = bd;
-          cur_d[p] = bd;
-          cur_i[p] = bi;
-        }
-        ++ncand;
-      }
-      __syncthreads();
-    }
-    if (ncand == 0) break;
-    if (lane == 0) {
-      // this rank's stations by (distance, index)
-      for (int a = 1; a < ncand; ++a) {
-        const int vi = cand[a];
-        const double vd = cand_d[a];
-        int b = a - 1;
-        while (b >= 0 && lex_less(vd, vi, cand_d[b], cand[b])) {
-          cand[b + 1] = cand[b];
-          cand_d[b + 1] = cand_d[b];
-          --b;
-        }
-        cand[b + 1] = vi;
-        cand_d[b + 1] = vd;
-      }
-      for (int a = 0; a < ncand && nsel + a < k; ++a) sel[nsel + a] = cand[a];
-    }
-    nsel += ncand;
-    __syncthreads();
-  }
-  if (nsel < k) atomicOr(bad, 2);
   for (int r = lane; r < k; r += 64) {
     const int v = sel[r];
     int rank = 0;
@@ -288,8 +192,10 @@ int spk_neb_idxs_pie(spk_ctx *ctx, const double *dst_xs, const double *dst_ys,
   if (n_nebs > n_refs)
     return set_error(SPK_ESHAPE, "n_nebs %lld > %lld stations",
                      (long long)n_nebs, (long long)n_refs);
-  const size_t lds =
-      (n_refs + 2 * n_pies) * 8 + (n_refs + 2 * n_pies + n_nebs) * 4;
+  if (n_refs > (1 << 20) || n_pies > (1 << 20))
+    return set_error(SPK_ESHAPE, "pie selection over %lld stations does not "
+                     "fit the LDS", (long long)n_refs);
+  const size_t lds = pie_lds_bytes((int)n_refs, (int)n_pies, (int)n_nebs);
   if (lds > 150 * 1024)
     return set_error(SPK_ESHAPE, "pie selection over %lld stations does not "
                      "fit the LDS", (long long)n_refs);

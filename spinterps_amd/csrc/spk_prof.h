@@ -1,0 +1,53 @@
+// Kernel timing by hipEvents around the heavy launches of a context
+// (enabled by spk_prof_enable; read by spk_prof_get).
+#pragma once
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "spk_internal.h"
+
+namespace spk {
+
+struct Prof {
+  bool on = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<std::string, std::pair<double, int64_t>>> stats;
+  // device counters of executed work: [0] 16-row chunks multiplied by the
+  // FP32 triangular contraction, [1] 64-k chunks times plane products of
+  // the int8 contraction
+  unsigned long long *work = nullptr;
+};
+
+Prof &prof_of(spk_ctx *ctx);
+hipEvent_t prof_event(Prof &pf);
+
+struct ProfScope {
+  spk_ctx *ctx;
+  Prof &pf;
+  const char *name;
+  hipStream_t st;
+  hipEvent_t a{}, b{};
+  ProfScope(spk_ctx *c, const char *nm, hipStream_t s = nullptr)
+      : ctx(c), pf(prof_of(c)), name(nm), st(s ? s : c->stream) {
+    if (pf.on) {
+      a = prof_event(pf);
+      (void)hipEventRecord(a, st);
+    }
+  }
+  ~ProfScope() {
+    if (pf.on) {
+      b = prof_event(pf);
+      (void)hipEventRecord(b, st);
+      pf.pending.push_back({name, a, b});
+    }
+  }
+};
+
+}  // namespace spk

@@ -15,8 +15,8 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .._lib import (METHODS, PRECISIONS, SpkOpts, SpkProblem, check, context,
-                    lib, vg_array)
+from .._lib import (METHODS, NEB_SEL, PRECISIONS, SpkNeb, SpkOpts, SpkProblem,
+                    check, context, lib, vg_array)
 
 
 def _dev(device):
@@ -184,6 +184,69 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     return est, (var if wv else None), info
 
 
+def interp_neb_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
+                     neb_sel_mthd='nrst', n_nebs=50, n_pies=None,
+                     want_var=True, idw_exp=2.0, min_vg_val=0.0,
+                     min_var_thr=-np.inf, est=None, var=None, vg_dev=None,
+                     kappa=None):
+    """interp_steps with the reference's neighbourhoods: every cell uses
+    only the n_nebs nearest active stations ('nrst') or n_nebs taken
+    sector by sector over n_pies sectors ('pie'); stations that no cell
+    selects with every station present count as missing
+    (interp/grps.py:141-288, interp/steps.py:592-608, 732-831).  FP64.
+
+    Returns (est, var or None, info) like interp_steps; info holds the OR
+    over each step's neighbourhood groups of the SPK_ST_* flags.
+    """
+    if neb_sel_mthd not in NEB_SEL:
+        raise NotImplementedError(neb_sel_mthd)
+    dev = prob.device
+    v = _t64(vals, dev)
+    assert v.ndim == 2 and v.shape[1] == prob.n_stns
+    T = v.shape[0]
+    sk_mean = method == 'SKM'
+    if sk_mean:
+        method = 'SK'
+    m = METHODS[method]
+    krig = method in ('OK', 'SK', 'EDK')
+    if krig and vg_dev is None:
+        assert vgs is not None and len(vgs) == T
+        if any(str(s) == 'nan' for s in vgs):
+            raise ValueError('NaN VG!')
+        vg_dev = vg_device(vgs, dev)
+    if method == 'EDK' and prob.n_drifts < 1:
+        raise AssertionError('EDK needs drifts')
+    C = prob.n_cells
+    if est is None:
+        est = torch.full((T, C), float('nan'), dtype=torch.float64,
+                         device=dev)
+    wv = bool(want_var) and method == 'OK'
+    if wv and var is None:
+        var = torch.full((T, C), float('nan'), dtype=torch.float64,
+                         device=dev)
+    info = torch.zeros(T, dtype=torch.int32, device=dev)
+    opts = SpkOpts()
+    opts.method = m
+    opts.precision = PRECISIONS['fp64']
+    opts.want_var = 1 if wv else 0
+    opts.idw_exp = float(idw_exp)
+    opts.min_vg_val = float(min_vg_val)
+    opts.min_var_thr = float(min_var_thr)
+    opts.sk_mean = 1 if sk_mean else 0
+    neb = SpkNeb()
+    neb.sel = NEB_SEL[neb_sel_mthd]
+    neb.n_nebs = int(n_nebs)
+    neb.n_pies = int(n_pies) if n_pies is not None else 0
+    ctx = _ctx_for(dev)
+    pst = prob.c_struct()
+    check(lib().spk_interp_neb_steps_dev(
+        ctx.h, ctypes.byref(pst), ctypes.byref(opts), ctypes.byref(neb), T,
+        v.data_ptr(), vg_dev.data_ptr() if krig else None, est.data_ptr(),
+        var.data_ptr() if wv else None, info.data_ptr(),
+        kappa.data_ptr() if kappa is not None else None))
+    return est, (var if wv else None), info
+
+
 def clip_cast(f64, min_var_cut=None, max_var_cut=None, out=None):
     """interp/steps.py:466-476 clip + float32 cast, on the device (into
     ``out``, a float32 device tensor of f64's shape, when given)."""
@@ -227,10 +290,13 @@ def prof_work(name, device=0):
 
 def interpolate(prob: DeviceProblem, vals, vgs, methods, *,
                 min_var_thr=-np.inf, min_var_cut=None, max_var_cut=None,
-                min_vg_val=0.0, est_vars_flag=True, precision='fp64'):
+                min_vg_val=0.0, est_vars_flag=True, precision='fp64',
+                neb_sel_mthd='all', n_nebs=None, n_pies=None):
     """Fields of all methods for all steps, as the reference's chunk
-    routine produces them (interp/steps.py:478-835) with the default
-    'all' neighbour selection.
+    routine produces them (interp/steps.py:478-835).  neb_sel_mthd 'all'
+    (the reference's default, interp/data.py:62), 'nrst' or 'pie' with
+    n_nebs (and n_pies) as the reference takes them; the neighbourhood
+    modes compute in FP64 whatever ``precision`` says.
 
     methods: list of (type, label, idw_exp) like the reference's
     interp_args.  Returns (fields, fields64, prblm): fields[label] is the
@@ -250,11 +316,17 @@ def interpolate(prob: DeviceProblem, vals, vgs, methods, *,
     for typ, lab, idw_exp in methods:
         if lab == 'EST_VARS_OK':
             continue
-        e, var, info = interp_steps(
-            prob, v, vgs, typ, precision=precision,
-            want_var=est_vars_flag and typ == 'OK',
-            idw_exp=idw_exp if typ == 'IDW' else 2.0,
-            min_vg_val=min_vg_val, min_var_thr=min_var_thr, vg_dev=vg_dev)
+        kw = dict(want_var=est_vars_flag and typ == 'OK',
+                  idw_exp=idw_exp if typ == 'IDW' else 2.0,
+                  min_vg_val=min_vg_val, min_var_thr=min_var_thr,
+                  vg_dev=vg_dev)
+        if neb_sel_mthd == 'all':
+            e, var, info = interp_steps(prob, v, vgs, typ,
+                                        precision=precision, **kw)
+        else:
+            e, var, info = interp_neb_steps(
+                prob, v, vgs, typ, neb_sel_mthd=neb_sel_mthd, n_nebs=n_nebs,
+                n_pies=n_pies, **kw)
         f64[lab] = e
         fields[lab] = clip_cast(e, min_var_cut, max_var_cut)
         if var is not None:

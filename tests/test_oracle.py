@@ -300,6 +300,110 @@ def test_interp_all_small_runs():
     assert not prb
 
 
+def _neb_case(seed=11):
+    """30 stations on [0, 10]^2 plus one far away (never among the 6
+    nearest of any cell: dropped), 49 cells, 4 steps with missing
+    stations; low values in the west so that some neighbourhoods fall
+    entirely below min_var_thr while their step does not."""
+    rng = np.random.default_rng(seed)
+    sx = np.r_[rng.uniform(0, 10, 30), 60.0]
+    sy = np.r_[rng.uniform(0, 10, 30), 60.0]
+    g = np.linspace(0.5, 9.5, 7)
+    cx, cy = [a.ravel() for a in np.meshgrid(g, g[::-1])]
+    T = 4
+    z = rng.gamma(2.0, 2.0, (T, sx.size)) + 0.1
+    z[:, sx < 3.5] = rng.uniform(0.0, 0.4, (T, int((sx < 3.5).sum())))
+    z[1, rng.choice(30, 8, replace=False)] = np.nan
+    z[2, :25] = np.nan          # 5 left + the far one: only it is dropped
+    z[3, rng.choice(30, 3, replace=False)] = np.nan
+    vgs = ['0.1 Nug(0.0) + 2.0 Sph(6.0)', '1.5 Exp(4.0)',
+           '0.05 Nug(0.0) + 1.0 Gau(5.0)', '2.0 Sph(8.0)']
+    return sx, sy, cx, cy, z, vgs
+
+
+def _brute_neb(sx, sy, cx, cy, z, vgs, k, thr):
+    """Per cell, without cell groups: the k nearest active stations that
+    survive the never-selected drop, then OK by numpy.linalg.solve of the
+    bordered system, IDW and nearest station over them."""
+    d_all = cdist(np.c_[cx, cy], np.c_[sx, sy])
+    keep = np.zeros(sx.size, dtype=bool)
+    for i in range(cx.size):
+        keep[np.lexsort((np.arange(sx.size), d_all[i]))[:k]] = True
+    T, C = z.shape[0], cx.size
+    out = {lab: np.full((T, C), np.nan) for lab in ('OK', 'V', 'IDW', 'NNB')}
+    for t in range(T):
+        act = np.where(~np.isnan(z[t]) & keep)[0]
+        for i in range(C):
+            s = act[np.lexsort((act, d_all[i, act]))[:k]]
+            zs, d = z[t, s], d_all[i, s]
+            out['NNB'][t, i] = zs[np.argmin(d)]
+            if s.size == 1:
+                out['OK'][t, i] = out['IDW'][t, i] = zs[0]
+                continue
+            if not np.any(zs >= thr):
+                out['OK'][t, i] = out['IDW'][t, i] = zs.mean()
+                out['V'][t, i] = 0.0
+                continue
+            w = 1.0 / (d / d.max()) ** 2
+            out['IDW'][t, i] = (w * zs).sum() / w.sum()
+            n = s.size
+            K = np.zeros((n + 1, n + 1))
+            K[:n, :n] = O.vg_sum(cdist(np.c_[sx[s], sy[s]],
+                                       np.c_[sx[s], sy[s]]), vgs[t])
+            K[n, :n] = K[:n, n] = 1.0
+            r = np.r_[O.vg_sum(d, vgs[t]), 1.0]
+            lam = np.linalg.solve(K, r)
+            out['OK'][t, i] = lam[:n] @ zs
+            out['V'][t, i] = lam @ r + lam[n]
+    return out
+
+
+def test_interp_all_nrst_matches_per_cell_brute_force():
+    """interp/steps.py:592-608 (never-selected stations dropped) and
+    :732-831 (per cell group: flags, mean fill, system) against a per-cell
+    restatement without groups."""
+    sx, sy, cx, cy, z, vgs = _neb_case()
+    methods = [('OK', 'OK', None), ('IDW', 'IDW', 2.0), ('NNB', 'NNB', None)]
+    f, f64, prb = O.interp_all(sx, sy, cx, cy, z, vgs, methods,
+                               min_var_thr=0.5, neb_sel_mthd='nrst',
+                               n_nebs=6)
+    exp = _brute_neb(sx, sy, cx, cy, z, vgs, 6, 0.5)
+    assert not prb
+    for lab, key in (('OK', 'OK'), ('IDW', 'IDW'), ('NNB', 'NNB'),
+                     ('EST_VARS_OK', 'V')):
+        np.testing.assert_allclose(f64[lab], exp[key], rtol=1e-10,
+                                   atol=1e-12, err_msg=lab)
+    # mean-filled neighbourhoods inside steps that are interpolated
+    assert (exp['V'] == 0.0).any() and (exp['V'] > 0.0).any()
+    # the drop matters: step 2 keeps five stations and would otherwise
+    # also use the far one
+    f2, f642, _ = O.interp_all(sx, sy[:], cx, cy, z[2:3], vgs[2:3],
+                               methods, min_var_thr=0.5,
+                               neb_sel_mthd='nrst', n_nebs=6)
+    np.testing.assert_array_equal(f642['OK'], f64['OK'][2:3])
+    zz = z[2:3].copy()
+    zz[:, -1] = np.nan
+    _, f64all, _ = O.interp_all(sx, sy, cx, cy, zz, vgs[2:3], methods,
+                                min_var_thr=0.5)
+    np.testing.assert_allclose(f642['OK'], f64all['OK'], rtol=1e-10)
+    _, f64far, _ = O.interp_all(sx, sy, cx, cy, z[2:3], vgs[2:3], methods,
+                                min_var_thr=0.5)
+    assert np.abs(f64far['OK'] - f642['OK']).max() > 1e-3
+
+
+def test_interp_all_nrst_with_every_station_equals_all():
+    """n_nebs at least the station count: one neighbourhood of every active
+    station per step, so the results are those of 'all' bit for bit."""
+    sx, sy, cx, cy, z, vgs = _neb_case(3)
+    sx, sy, z = sx[:30], sy[:30], z[:, :30]
+    methods = [('OK', 'OK', None), ('IDW', 'IDW', 2.0)]
+    _, a, _ = O.interp_all(sx, sy, cx, cy, z, vgs, methods, min_var_thr=0.5)
+    _, b, _ = O.interp_all(sx, sy, cx, cy, z, vgs, methods, min_var_thr=0.5,
+                           neb_sel_mthd='nrst', n_nebs=40)
+    for lab in a:
+        np.testing.assert_array_equal(a[lab], b[lab])
+
+
 # ---- exact solution of the kriging system (ill-conditioned steps) --------
 
 def _bordered_solve_mp(K, rhs, dps=40):
