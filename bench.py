@@ -756,7 +756,8 @@ def neb_leg(args, log):
     import torch
     from spinterps_amd import configs
     from spinterps_amd.interp.steps import (DeviceProblem, interp_neb_steps,
-                                            prof_enable, prof_get, vg_device)
+                                            prof_enable, prof_get, prof_work,
+                                            vg_device)
     w = configs.daily()
     T = min(args.neb_steps, w.n_steps)
     K = args.neb_nebs
@@ -783,6 +784,7 @@ def neb_leg(args, log):
     t1 = time.perf_counter()
     prof = {nm: prof_get(nm) for nm in ('neb_select', 'neb_group',
                                         'neb_krige', 'neb_pinv')}
+    chunks = prof_work('neb_chunks')
     prof_enable(0, False)
     info = info.cpu().numpy()
     rate = T * C / (t1 - t0)
@@ -801,6 +803,7 @@ def neb_leg(args, log):
                      '500x500 cells at 1 km, OK estimate + variance, nrst '
                      f'neighbourhoods of {K} stations, FP64',
            'kernel_ms': {k: v[0] for k, v in prof.items()},
+           'group_chunks_per_step': chunks / T,
            'steps_pinv': int(np.sum((info & 128) != 0)),
            'steps_ill_conditioned': int(np.sum((info & 16) != 0)),
            'roofline': {'bound': 'mfma', 'achieved': ach, 'peak': 78.6,

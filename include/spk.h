@@ -398,7 +398,9 @@ int spk_prof_get(spk_ctx *ctx, const char *name, double *total_ms,
  * multiply-adds the FP32 triangular contraction executed, and
  * "gemm_tri_i8_ops", the int8 operations of the plane products the int8
  * contraction executed (both skip the zero chunks of delta, so these are
- * below the algorithmic counts). */
+ * below the algorithmic counts); "neb_chunks" and "neb_pinv_chunks", the
+ * neighbourhood group chunks spk_interp_neb_steps_dev solved and those of
+ * them solved by the pseudo-inverse. */
 int spk_prof_work(spk_ctx *ctx, const char *name, double *value);
 
 #ifdef __cplusplus

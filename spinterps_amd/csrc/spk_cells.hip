@@ -1489,6 +1489,7 @@ int spk_prof_reset(spk_ctx *ctx) {
   SPK_TRY(prof_collect(ctx));
   Prof &pf = prof_of(ctx);
   pf.stats.clear();
+  pf.counts.clear();
   if (pf.work) {
     SPK_HIP(hipMemsetAsync(pf.work, 0, 64, ctx->stream));
     SPK_TRY(sync(ctx));
@@ -1510,6 +1511,11 @@ int spk_prof_work(spk_ctx *ctx, const char *name, double *value) {
     slot = 1;  // 2 x 64 k x 128 rows x 64 cells ops per chunk and product
     per = 2.0 * 64.0 * 128.0 * 64.0;
   } else {
+    for (auto &kv : pf.counts)
+      if (kv.first == nm) {
+        *value = kv.second;
+        return SPK_OK;
+      }
     return set_error(SPK_EARG, "unknown work counter %s", name);
   }
   if (!pf.work) return SPK_OK;

@@ -97,8 +97,11 @@ __host__ __device__ inline double dist2(double x1, double y1, double x2,
 }
 
 // One variogram component gamma(h; r, s), pyx:38-83 (same operation order).
-__host__ __device__ inline double vg_comp(int type, double h, double r,
-                                         double s) {
+// PH = false leaves out the Pow and Hol models (NaN for them), whose pow and
+// sin cost registers in kernels that never see them.
+template <bool PH>
+__host__ __device__ inline double vg_comp_t(int type, double h, double r,
+                                           double s) {
 #pragma clang fp contract(off)
   switch (type) {
     case SPK_VG_RNG:
@@ -119,31 +122,47 @@ __host__ __device__ inline double vg_comp(int type, double h, double r,
     case SPK_VG_GAU:
       return s * (1.0 - exp(-3.0 * ((h * h) / (r * r))));
     case SPK_VG_POW:
-      return s * pow(h, r);
+      if constexpr (PH) return s * pow(h, r);
+      return NAN;
     case SPK_VG_HOL: {
-      if (h == 0.0) return 0.0;
-      const double a = (kPi * h) / r;
-      return s * (1.0 - (sin(a) / a));
+      if constexpr (PH) {
+        if (h == 0.0) return 0.0;
+        const double a = (kPi * h) / r;
+        return s * (1.0 - (sin(a) / a));
+      }
+      return NAN;
     }
     default:
       return NAN;
   }
 }
 
+__host__ __device__ inline double vg_comp(int type, double h, double r,
+                                         double s) {
+  return vg_comp_t<true>(type, h, r, s);
+}
+
 // fill_vg_var_arr's per-entry value (pyx:162-216): components summed in
 // order from 0.0, covariance form s - gamma when covar_flag, then entries
 // <= min_vg_val set to 0.
-__host__ __device__ inline double vg_entry(const spk_vg &vg, double h,
-                                          int covar_flag, double min_vg_val) {
+template <bool PH>
+__host__ __device__ inline double vg_entry_t(const spk_vg &vg, double h,
+                                            int covar_flag,
+                                            double min_vg_val) {
 #pragma clang fp contract(off)
   double acc = 0.0;
   const double mult = covar_flag ? 1.0 : 0.0;
   const double sign = covar_flag ? -1.0 : 1.0;
   for (int k = 0; k < vg.n; ++k)
-    acc += (mult * vg.sill[k]) + (sign * vg_comp(vg.type[k], h, vg.range[k],
-                                                 vg.sill[k]));
+    acc += (mult * vg.sill[k]) +
+           (sign * vg_comp_t<PH>(vg.type[k], h, vg.range[k], vg.sill[k]));
   if (acc <= min_vg_val) acc = 0.0;
   return acc;
+}
+
+__host__ __device__ inline double vg_entry(const spk_vg &vg, double h,
+                                          int covar_flag, double min_vg_val) {
+  return vg_entry_t<true>(vg, h, covar_flag, min_vg_val);
 }
 
 // A variogram with per-component constants folded in (reciprocals instead

@@ -32,6 +32,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "spk_krige.h"
 #include "spk_neb.h"
@@ -42,8 +43,8 @@ namespace spk {
 constexpr int kNebMax = 64;     // stations per neighbourhood
 constexpr int kNebChunk = 64;   // cells per chunk of a group
 constexpr int kNebTile = 64;    // cells per selection tile
-constexpr int kNebCand = 2048;  // candidate stations staged per tile
 constexpr int kNebSlots = 8;    // candidates per lane held in registers
+constexpr int kNebCand = kNebSlots * 64;  // candidates staged per tile
 constexpr int kStepShift = 56;  // step index above the 56-bit set hash
 
 __device__ inline bool neb_close1(double x) {
@@ -123,6 +124,39 @@ __device__ inline double idw_set_cell(double x, double y, const double *xs,
   return wz / ws;
 }
 
+// Start of column s of a k x k lower triangle packed by columns.
+__device__ inline int cofs(int s, int k) { return s * k - (s * (s - 1)) / 2; }
+
+// Lane j's value of v in every lane (j uniform across the wavefront).
+__device__ inline double bcast(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, j);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline int bcast_i(int v, int j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+
+// sum_{l0 <= l < l1} a[l] b[l] with four partial sums, so that the LDS
+// loads of four terms are in flight together
+__device__ inline double dot_lds(const double *a, const double *b, int l0,
+                                 int l1) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int l = l0;
+  for (; l + 4 <= l1; l += 4) {
+    const double a0 = a[l], a1 = a[l + 1], a2 = a[l + 2], a3 = a[l + 3];
+    const double b0 = b[l], b1 = b[l + 1], b2 = b[l + 2], b3 = b[l + 3];
+    s0 += a0 * b0;
+    s1 += a1 * b1;
+    s2 += a2 * b2;
+    s3 += a3 * b3;
+  }
+  for (; l < l1; ++l) s0 += a[l] * b[l];
+  return (s0 + s1) + (s2 + s3);
+}
+
 // ---- set-up ----------------------------------------------------------------
 
 // A pseudo-step with every station active, for the selection that decides
@@ -163,8 +197,69 @@ __global__ void k_neb_mask(const double *__restrict__ vals, int64_t n,
     out[i] = keep[i % N] ? vals[i] : NAN;
 }
 
+// ---- Z order of the cells ----------------------------------------------------
+// One workgroup: the bounding box of the cells, bb = (x0, x1, y0, y1).
+__global__ __launch_bounds__(1024) void k_neb_bbox(
+    const double *__restrict__ cx, const double *__restrict__ cy, int64_t C,
+    double *__restrict__ bb) {
+  __shared__ double red[4][16];
+  double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+  for (int64_t c = threadIdx.x; c < C; c += 1024) {
+    v[0] = fmin(v[0], cx[c]);
+    v[1] = fmax(v[1], cx[c]);
+    v[2] = fmin(v[2], cy[c]);
+    v[3] = fmax(v[3], cy[c]);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v[0] = wave_min(v[0]);
+  v[1] = wave_max(v[1]);
+  v[2] = wave_min(v[2]);
+  v[3] = wave_max(v[3]);
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q) red[q][wave] = v[q];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    double r = red[q][0];
+    for (int w = 1; w < 16; ++w)
+      r = (q & 1) ? fmax(r, red[q][w]) : fmin(r, red[q][w]);
+    bb[q] = r;
+  }
+}
+
+__device__ inline unsigned spread16(unsigned v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// code[c]: coordinates quantised to 16 bits over the bounding box and
+// interleaved (Z order); id[c] = c.
+__global__ void k_neb_morton(const double *__restrict__ cx,
+                             const double *__restrict__ cy, int64_t C,
+                             const double *__restrict__ bb,
+                             unsigned *__restrict__ code,
+                             int32_t *__restrict__ id) {
+  const double x0 = bb[0], y0 = bb[2];
+  const double wx = bb[1] - x0, wy = bb[3] - y0;
+  const double sx = wx > 0.0 ? 65535.0 / wx : 0.0;
+  const double sy = wy > 0.0 ? 65535.0 / wy : 0.0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const double fx = fmin(fmax((cx[c] - x0) * sx, 0.0), 65535.0);
+    const double fy = fmin(fmax((cy[c] - y0) * sy, 0.0), 65535.0);
+    code[c] = spread16((unsigned)fx) | (spread16((unsigned)fy) << 1);
+    id[c] = (int32_t)c;
+  }
+}
+
 // ---- nrst selection ----------------------------------------------------------
-// grid (ceil(C / 64), S), 256 threads.  For the tile's bounding box
+// grid (ceil(C / 64), S), 256 threads.  Tile i holds cells perm[64 i ..
+// 64 i + 63], consecutive in Z order, so a tile is a compact patch.  For
+// the tile's bounding box
 // (centre c, half diagonal h) the block finds a T with at least k active
 // stations within T of c; the k nearest of any cell x of the tile then lie
 // within T + 2h of c (r_k(x) <= r_k(c) + h, and |s - c| <= |s - x| + h),
@@ -176,8 +271,8 @@ __global__ __launch_bounds__(256) void k_neb_nrst(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N, const StepSc *__restrict__ sc,
     const double *__restrict__ cx, const double *__restrict__ cy, int64_t C,
-    int K, int32_t *__restrict__ nb, unsigned long long *__restrict__ key,
-    int32_t *__restrict__ cell) {
+    const int32_t *__restrict__ perm, int K, int32_t *__restrict__ nb,
+    unsigned long long *__restrict__ key, int32_t *__restrict__ cell) {
   const int t = blockIdx.y;
   const int64_t c0 = (int64_t)blockIdx.x * kNebTile;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -193,16 +288,16 @@ __global__ __launch_bounds__(256) void k_neb_nrst(
   __shared__ double wred[4];
   if (n <= 0) {
     if (tid < kNebTile && c0 + tid < C) {
-      key[rowbase + c0 + tid] = tkey;
-      cell[rowbase + c0 + tid] = (int32_t)(c0 + tid);
+      const int64_t c = perm[c0 + tid];
+      key[rowbase + c] = tkey;
+      cell[rowbase + c] = (int32_t)c;
     }
     return;
   }
   // bounding box of the tile's cells
   if (wave == 0) {
-    const int64_t c = c0 + lane;
-    const bool in = c < C;
-    const double x = in ? cx[c] : cx[c0], y = in ? cy[c] : cy[c0];
+    const int64_t c = perm[c0 + lane < C ? c0 + lane : c0];
+    const double x = cx[c], y = cy[c];
     const double x0 = wave_min(x), x1 = wave_max(x);
     const double y0 = wave_min(y), y1 = wave_max(y);
     if (lane == 0) {
@@ -274,72 +369,108 @@ __global__ __launch_bounds__(256) void k_neb_nrst(
   }
   __syncthreads();
   const int m = base;
-  const bool staged = m <= kNebSlots * 64;  // and so within kNebCand
+  const bool staged = m <= kNebCand;
   // each wave: 16 cells of the tile
   for (int u = 0; u < kNebTile / 4; ++u) {
-    const int64_t c = c0 + wave * (kNebTile / 4) + u;
-    if (c >= C) break;
+    const int64_t cs = c0 + wave * (kNebTile / 4) + u;
+    if (cs >= C) break;
+    const int64_t c = perm[cs];
     const double x = cx[c], y = cy[c];
     int32_t *out = nb + (rowbase + c) * K;
     unsigned long long hsum = 0;
     if (staged) {
-      unsigned long long kv[kNebSlots];
-      unsigned long long kmin = ~0ull, kmax = 0;
+      // distances to the staged candidates, kNebSlots per lane
+      const int ns = (m + 63) >> 6;
+      double dv[kNebSlots];
+      double dmax = 0.0;
 #pragma unroll
       for (int s = 0; s < kNebSlots; ++s) {
         const int q = s * 64 + lane;
-        kv[s] = q < m ? dbits(dist2(x, y, cand_x[q], cand_y[q])) : ~0ull;
-        if (q < m) {
-          kmin = min(kmin, kv[s]);
-          kmax = max(kmax, kv[s]);
+        dv[s] = INFINITY;
+        if (s < ns && q < m) {
+          dv[s] = dist2(x, y, cand_x[q], cand_y[q]);
+          dmax = fmax(dmax, dv[s]);
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long a =
-            ((unsigned long long)__shfl_xor((unsigned)(kmin >> 32), off, 64)
-             << 32) |
-            __shfl_xor((unsigned)kmin, off, 64);
-        const unsigned long long b =
-            ((unsigned long long)__shfl_xor((unsigned)(kmax >> 32), off, 64)
-             << 32) |
-            __shfl_xor((unsigned)kmax, off, 64);
-        kmin = min(kmin, a);
-        kmax = max(kmax, b);
-      }
-      // smallest v with count(key <= v) >= k: the k-th smallest key
-      unsigned long long vlo = uniform_u64(kmin), vhi = uniform_u64(kmax);
-      while (vlo < vhi) {
-        const unsigned long long mid = vlo + ((vhi - vlo) >> 1);
+      dmax = wave_max(dmax);
+      // bisection on the distance until exactly k lie at or below it
+      // (count(<= lo) < k <= count(<= hi)); a near tie that stops it falls
+      // back to the exact order on (distance, position) below
+      double lo = -1.0, hi = dmax, thr = -1.0;
+      for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break;
         int cnt = 0;
 #pragma unroll
-        for (int s = 0; s < kNebSlots; ++s) cnt += __popcll(__ballot(kv[s] <= mid));
-        if (cnt >= k)
-          vhi = mid;
-        else
-          vlo = mid + 1;
-      }
-      const unsigned long long thr = vhi;
-      int nlt = 0;
-#pragma unroll
-      for (int s = 0; s < kNebSlots; ++s) nlt += __popcll(__ballot(kv[s] < thr));
-      const int need = k - nlt;  // ties at thr taken by position
-      int nsel = 0, neq = 0;
-#pragma unroll
-      for (int s = 0; s < kNebSlots; ++s) {
-        const int q = s * 64 + lane;
-        const bool lt = kv[s] < thr, eq = kv[s] == thr;
-        const unsigned long long beq = __ballot(eq);
-        const bool take =
-            lt || (eq && neq + __popcll(beq & lanes_below(lane)) < need);
-        const unsigned long long bt = __ballot(take);
-        if (take) {
-          const int r = nsel + __popcll(bt & lanes_below(lane));
-          const int v = cand_j[q];
-          out[r] = v;
-          hsum += set_mix((unsigned)v, (unsigned)r);
+        for (int s = 0; s < kNebSlots; ++s)
+          if (s < ns) cnt += __popcll(__ballot(dv[s] <= mid));
+        if (cnt == k) {
+          thr = mid;
+          break;
         }
-        nsel += __popcll(bt);
-        neq += __popcll(beq);
+        if (cnt > k)
+          hi = mid;
+        else
+          lo = mid;
+      }
+      int nsel = 0;
+      if (thr >= 0.0) {
+#pragma unroll
+        for (int s = 0; s < kNebSlots; ++s) {
+          if (s >= ns) break;
+          const bool take = dv[s] <= thr;
+          const unsigned long long bt = __ballot(take);
+          if (take) {
+            const int r = nsel + __popcll(bt & lanes_below(lane));
+            const int v = cand_j[s * 64 + lane];
+            out[r] = v;
+            hsum += set_mix((unsigned)v, (unsigned)r);
+          }
+          nsel += __popcll(bt);
+        }
+      } else {
+        // exact: the k-th smallest key (distance bits, monotone for
+        // non-negative doubles), ties at it taken by position
+        unsigned long long kv[kNebSlots];
+#pragma unroll
+        for (int s = 0; s < kNebSlots; ++s)
+          kv[s] = dv[s] < INFINITY ? dbits(dv[s]) : ~0ull;
+        unsigned long long vlo = 0, vhi = uniform_u64(dbits(dmax));
+        while (vlo < vhi) {
+          const unsigned long long mid = vlo + ((vhi - vlo) >> 1);
+          int cnt = 0;
+#pragma unroll
+          for (int s = 0; s < kNebSlots; ++s)
+            cnt += __popcll(__ballot(kv[s] <= mid));
+          if (cnt >= k)
+            vhi = mid;
+          else
+            vlo = mid + 1;
+        }
+        const unsigned long long tk = vhi;
+        int nlt = 0;
+#pragma unroll
+        for (int s = 0; s < kNebSlots; ++s)
+          nlt += __popcll(__ballot(kv[s] < tk));
+        const int need = k - nlt;  // ties at tk taken by position
+        int neq = 0;
+#pragma unroll
+        for (int s = 0; s < kNebSlots; ++s) {
+          const int q = s * 64 + lane;
+          const bool lt = kv[s] < tk, eq = kv[s] == tk;
+          const unsigned long long beq = __ballot(eq);
+          const bool take =
+              lt || (eq && neq + __popcll(beq & lanes_below(lane)) < need);
+          const unsigned long long bt = __ballot(take);
+          if (take) {
+            const int r = nsel + __popcll(bt & lanes_below(lane));
+            const int v = cand_j[q];
+            out[r] = v;
+            hsum += set_mix((unsigned)v, (unsigned)r);
+          }
+          nsel += __popcll(bt);
+          neq += __popcll(beq);
+        }
       }
     } else {
       // many candidates: repeated wave arg-min over the active stations
@@ -532,9 +663,11 @@ __host__ __device__ inline int neb_p(int method, int m) {
 }
 
 // Dynamic LDS of k_neb_krige in doubles for K stations and p border
-// columns: packed lower triangle, 8 vectors, the border F and M F.
+// columns: packed lower triangle, 4 vectors, the border F, the chunk's
+// cell coordinates and indices.
 __host__ __device__ inline int neb_lds_doubles(int K, int p) {
-  return K * (K + 1) / 2 + 8 * K + 2 * K * (p > 0 ? p : 1);
+  return K * (K + 1) / 2 + 4 * K + K * (p > 0 ? p : 1) + 2 * kNebChunk +
+         kNebChunk / 2;
 }
 
 // Chunk ch of the batch: the step t, the cells [p0, p1) and the set.
@@ -565,7 +698,7 @@ __device__ inline NebChunk neb_chunk(const NebArgs &a, int ch, int nch) {
 //   OK variance = c0 - |M c|^2 + (a - 1)(a - 2) / b with a = 1^T C^-1 c,
 //   b = 1^T C^-1 1, the reference's lambda^T gamma + 2 mu
 //   (interp/steps.py:416-434).
-template <int PT>
+template <int PT, bool PH, bool KRIG>
 __global__ __launch_bounds__(64) void k_neb_krige(NebArgs a) {
   constexpr int PA = PT > 0 ? PT : 1;
   extern __shared__ double lds[];
@@ -574,32 +707,39 @@ __global__ __launch_bounds__(64) void k_neb_krige(NebArgs a) {
   const int p = min(neb_p(a.method, a.m), PT);
   const int P = p > 0 ? p : 1;
   const int K = a.K;
-  double *A = lds;                      // packed lower triangle, row i at
-  double *xs = A + K * (K + 1) / 2;     //   i (i + 1) / 2
+  double *A = lds;                    // packed lower triangle
+  double *xs = A + K * (K + 1) / 2;   // the set's stations
   double *ys = xs + K;
   double *zs = ys + K;
-  double *al = zs + K;   // alpha
-  double *wv = al + K;   // M 1
-  double *rv = wv + K;   // C^-1 1 (SK sum test)
-  double *cc = rv + K;   // a cell's covariances
-  double *tv = cc + K;   // scratch vector
-  double *F = tv + K;    // [K x P] border columns
-  double *G = F + K * P; // [K x P] M F
-  const bool krig = a.method <= SPK_M_EDK;
+  double *tv = zs + K;                // 1 / L_jj
+  double *F = tv + K;                 // [K x P] border columns
+  double *cxl = F + K * P;            // the chunk's cells
+  double *cyl = cxl + kNebChunk;
+  int *cil = (int *)(cyl + kNebChunk);
   const bool sk = a.method == SPK_M_SK;
   for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
     const NebChunk q = neb_chunk(a, ch, nch);
     if (q.n <= 0) continue;
     const int k = q.k, t = q.t;
-    const int64_t np = q.p1 - q.p0;
-    const int32_t *list =
-        a.nb + ((int64_t)t * a.C + a.cells[q.p0]) * K;
+    const int np = (int)(q.p1 - q.p0);  // at most kNebChunk
     double *eo = a.est + (int64_t)t * a.C;
     double *vo = a.var ? a.var + (int64_t)t * a.C : nullptr;
     __syncthreads();  // the previous chunk's readers are done
+    // the chunk's cells, all loads in flight together
+    int cme = 0;
+    if (lane < np) {
+      cme = a.cells[q.p0 + lane];
+      cil[lane] = cme;
+      cxl[lane] = a.cx[cme];
+      cyl[lane] = a.cy[cme];
+    }
+    const int32_t *list =
+        a.nb + ((int64_t)t * a.C + bcast_i(cme, 0)) * K;
     double zl = 0.0;
     if (lane < k) {
-      const int pos = list[lane];
+      // positions are below n by construction; the clamp keeps a
+      // selection that met NaN coordinates inside the step's list
+      const int pos = min(max(list[lane], 0), q.n - 1);
       const int s = a.idx[(int64_t)t * a.N + pos];
       xs[lane] = a.sx[s];
       ys[lane] = a.sy[s];
@@ -617,161 +757,170 @@ __global__ __launch_bounds__(64) void k_neb_krige(NebArgs a) {
     // one station: its value everywhere, variance untouched (NaN)
     // (interp/steps.py:280-283)
     if (k == 1) {
-      for (int64_t i = lane; i < np; i += 64) {
-        const int c = a.cells[q.p0 + i];
-        eo[c] = zs[0];
-        if (vo && a.want_var) vo[c] = NAN;
+      if (lane < np) {
+        eo[cme] = zs[0];
+        if (vo && a.want_var) vo[cme] = NAN;
       }
       if (lane == 0) atomicOr(&a.sflags[t], SPK_ST_ONESTN);
       continue;
     }
-    if (a.method == SPK_M_NNB) {
-      for (int64_t i = lane; i < np; i += 64) {
-        const int c = a.cells[q.p0 + i];
-        const double x = a.cx[c], y = a.cy[c];
-        double bd = INFINITY;
-        int bj = 0;
-        for (int j = 0; j < k; ++j) {
-          const double d = dist2(x, y, xs[j], ys[j]);
-          if (d < bd) {
-            bd = d;
-            bj = j;
+    if constexpr (!KRIG) {
+      if (a.method == SPK_M_NNB) {
+        if (lane < np) {
+          const double x = cxl[lane], y = cyl[lane];
+          double bd = INFINITY;
+          int bj = 0;
+          for (int j = 0; j < k; ++j) {
+            const double d = dist2(x, y, xs[j], ys[j]);
+            if (d < bd) {
+              bd = d;
+              bj = j;
+            }
           }
+          eo[cme] = zs[bj];
         }
-        eo[c] = zs[bj];
+        continue;
       }
-      continue;
     }
     bool fill = !anyhi;
-    if (krig) {
+    if constexpr (KRIG) {
       const spk_vg &vg = a.vgs[t];
       fill = fill || (vg.sill_sum <= a.mvg) || (vg.range_max_raw <= a.mvg);
     }
     if (fill) {
       // interp/steps.py:760-765, 325-331: the group's mean, variance 0
-      for (int64_t i = lane; i < np; i += 64) {
-        const int c = a.cells[q.p0 + i];
-        eo[c] = mean;
-        if (vo && a.want_var) vo[c] = 0.0;
+      if (lane < np) {
+        eo[cme] = mean;
+        if (vo && a.want_var) vo[cme] = 0.0;
       }
       if (lane == 0) atomicOr(&a.sflags[t], SPK_ST_MEANFILL);
       continue;
     }
-    if (a.method == SPK_M_IDW) {
-      // interp/steps.py:293-313 with pyx:784-808 over the set
-      const double pw = a.idw_exp;
-      for (int64_t i = lane; i < np; i += 64) {
-        const int c = a.cells[q.p0 + i];
-        eo[c] = idw_set_cell(a.cx[c], a.cy[c], xs, ys, zs, k, pw, a.bad);
-      }
+    if constexpr (!KRIG) {
+      // IDW, interp/steps.py:293-313 with pyx:784-808 over the set
+      if (lane < np)
+        eo[cme] = idw_set_cell(cxl[lane], cyl[lane], xs, ys, zs, k,
+                               a.idw_exp, a.bad);
       continue;
-    }
+    } else {
     // ---- kriging ----
-    const spk_vg vg = a.vgs[t];
-    for (int e = lane; e < k * k; e += 64) {
-      const int i = e / k, j = e - i * k;
-      if (j <= i)
-        A[i * (i + 1) / 2 + j] = vg_entry(
-            vg, dist2(xs[i], ys[i], xs[j], ys[j]), sk ? 1 : 0, a.mvg);
+    // C in LDS as a packed lower triangle by columns: entry (r, s), r >= s,
+    // at cofs(s) + r - s, so that a fixed column is contiguous over the
+    // lanes that own its rows
+    const spk_vg &vg = a.vgs[t];
+    const int ntri = k * (k + 1) / 2;
+    bool bounded = true;
+    for (int qd = 0; qd < vg.n; ++qd) {
+      const int ty = vg.type[qd];
+      if (ty == SPK_VG_RNG || ty == SPK_VG_POW || ty == SPK_VG_HOL)
+        bounded = false;
+    }
+    for (int e = lane; e < ntri; e += 64) {
+      // column s of entry e: the largest s with cofs(s) <= e
+      const float b2 = (float)(2 * k + 1);
+      int s = (int)((b2 - sqrtf(b2 * b2 - 8.0f * (float)e)) * 0.5f);
+      s = max(0, min(s, k - 1));
+      while (s > 0 && cofs(s, k) > e) --s;
+      while (s + 1 < k && cofs(s + 1, k) <= e) ++s;
+      const int r = s + (e - cofs(s, k));
+      const double g = vg_entry_t<PH>(vg, dist2(xs[r], ys[r], xs[s], ys[s]),
+                                      sk ? 1 : 0, a.mvg);
+      A[e] = (sk || !bounded) ? g : vg.sill_sum - g;
+    }
+    double c0 = sk ? 0.0 : vg.sill_sum;
+    if (!sk && !bounded) {
+      __syncthreads();
+      double mxa = 0.0;
+      for (int e = lane; e < ntri; e += 64) mxa = fmax(mxa, fabs(A[e]));
+      c0 = 2.0 * wave_max(mxa) + fabs(vg.sill_sum);
+      for (int e = lane; e < ntri; e += 64) A[e] = c0 - A[e];
     }
     __syncthreads();
-    double c0 = 0.0;
-    if (!sk) {
-      bool bounded = true;
-      for (int qd = 0; qd < vg.n; ++qd) {
-        const int ty = vg.type[qd];
-        if (ty == SPK_VG_RNG || ty == SPK_VG_POW || ty == SPK_VG_HOL)
-          bounded = false;
-      }
-      if (bounded) {
-        c0 = vg.sill_sum;
-      } else {
-        double mxa = 0.0;
-        for (int e = lane; e < k * (k + 1) / 2; e += 64) mxa = fmax(mxa, fabs(A[e]));
-        c0 = 2.0 * wave_max(mxa) + fabs(vg.sill_sum);
-      }
-      __syncthreads();
-      for (int e = lane; e < k * (k + 1) / 2; e += 64) A[e] = c0 - A[e];
-      __syncthreads();
-    }
-    const double dmaxc = wave_max(lane < k ? A[lane * (lane + 1) / 2 + lane] : 0.0);
-    // Cholesky, lane r owns row r
+    const double dmaxc = wave_max(lane < k ? A[cofs(lane, k)] : 0.0);
+    // Cholesky by columns (left-looking): lane r forms
+    // L_rj = (C_rj - sum_{t<j} L_rt L_jt) / L_jj as one dot product over
+    // its row, whose loads are independent of each other
     bool fail = false;
+    double pmin = INFINITY;
+    double *dinv = tv;  // 1 / L_jj
     for (int j = 0; j < k; ++j) {
-      __syncthreads();
-      const double d = A[j * (j + 1) / 2 + j];
-      if (!(d > 0.0) || !isfinite(d)) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      const bool act = lane >= j && lane < k;
+      if (act) {
+        int o = lane;       // entry (lane, t) at cofs(t) + lane - t
+        int oj = j;         // entry (j, t)
+        int t = 0;
+        for (; t + 4 <= j; t += 4) {
+          const int d1 = k - t - 1, d2 = k - t - 2, d3 = k - t - 3;
+          const double r0 = A[o], q0 = A[oj];
+          const double r1 = A[o + d1], q1 = A[oj + d1];
+          const double r2 = A[o + d1 + d2], q2 = A[oj + d1 + d2];
+          const double r3 = A[o + d1 + d2 + d3], q3 = A[oj + d1 + d2 + d3];
+          s0 += r0 * q0;
+          s1 += r1 * q1;
+          s2 += r2 * q2;
+          s3 += r3 * q3;
+          const int step = d1 + d2 + d3 + (k - t - 4);
+          o += step;
+          oj += step;
+        }
+        for (; t < j; ++t) {
+          s0 += A[o] * A[oj];
+          o += k - t - 1;
+          oj += k - t - 1;
+        }
+      }
+      const double sr = (s0 + s1) + (s2 + s3);
+      const double arj = act ? A[cofs(j, k) + lane - j] - sr : 0.0;
+      const double dj = bcast(arj, j);  // the pivot C_jj - |L_j,<j|^2
+      if (!(dj > 0.0) || !isfinite(dj)) {
         fail = true;
         break;
       }
-      const double ljj = sqrt(d);
+      pmin = fmin(pmin, dj);
+      const double ljj = sqrt(dj);
+      const double rl = 1.0 / ljj;
+      if (act) A[cofs(j, k) + lane - j] = lane == j ? ljj : arj * rl;
+      if (lane == 0) dinv[j] = rl;
       __syncthreads();
-      double lr = 0.0;
-      if (lane > j && lane < k) {
-        lr = A[lane * (lane + 1) / 2 + j] / ljj;
-        A[lane * (lane + 1) / 2 + j] = lr;
-      }
-      if (lane == j) A[j * (j + 1) / 2 + j] = ljj;
-      __syncthreads();
-      if (lane > j && lane < k) {
-        double *row = A + lane * (lane + 1) / 2;
-        for (int s = j + 1; s <= lane; ++s) row[s] -= lr * A[s * (s + 1) / 2 + j];
-      }
     }
     if (fail) {
       if (lane == 0) a.fail_list[atomicAdd(a.fail_cnt, 1)] = ch;
       continue;
     }
-    // M = L^-1 in place, columns from the last
-    for (int j = k - 1; j >= 0; --j) {
-      __syncthreads();
-      const double ajj = 1.0 / A[j * (j + 1) / 2 + j];
-      double xi = 0.0;
-      if (lane > j && lane < k) {
-        const double *row = A + lane * (lane + 1) / 2;
-        double s = 0.0;
-        for (int l = j + 1; l <= lane; ++l) s += row[l] * A[l * (l + 1) / 2 + j];
-        xi = -s * ajj;
-      }
-      __syncthreads();
-      if (lane > j && lane < k) A[lane * (lane + 1) / 2 + j] = xi;
-      if (lane == j) A[j * (j + 1) / 2 + j] = ajj;
-    }
-    __syncthreads();
-    // condition estimate max diag(C^-1) max diag(C); diag(C^-1)_j is the
-    // squared norm of column j of M
-    double dj = 0.0;
-    if (lane < k)
-      for (int i = lane; i < k; ++i) {
-        const double v = A[i * (i + 1) / 2 + lane];
-        dj += v * v;
-      }
-    const double kap = wave_max(dj) * dmaxc;
-    if (!(kap <= 1e14)) {
-      // beyond the Cholesky solve's reach: the pseudo-inverse as the
-      // reference computes it
+    // condition estimate max diag(C) / min pivot (a lower bound of the
+    // condition number); sets beyond 1e10 take the pseudo-inverse as the
+    // reference computes it
+    const double kap = dmaxc / pmin;
+    if (!(kap <= 1e10)) {
       if (lane == 0) a.fail_list[atomicAdd(a.fail_cnt, 1)] = ch;
       continue;
     }
-    // row vectors of M: w = M 1, u = M (z - shift), G = M F
+    // forward solves by columns, lane r holding element r:
+    // w = L^-1 1, u = L^-1 (z - shift), G = L^-1 F (column 0 of F is 1)
     const double zsh = a.skm && sk ? mean : 0.0;
-    double ul = 0.0;
-    if (lane < k) {
-      const double *row = A + lane * (lane + 1) / 2;
-      double w = 0.0, u = 0.0;
-      for (int l = 0; l <= lane; ++l) {
-        w += row[l];
-        u += row[l] * (zs[l] - zsh);
+    double wl = lane < k ? 1.0 : 0.0;
+    double ul = lane < k ? zs[lane] - zsh : 0.0;
+    double gl[PA];
+    for (int qd = 0; qd < PA; ++qd)
+      gl[qd] = (qd > 0 && qd < p && lane < k) ? F[lane * P + qd] : 0.0;
+    for (int j = 0; j < k; ++j) {
+      const double di = dinv[j];
+      const double lrj = (lane > j && lane < k) ? A[cofs(j, k) + lane - j] : 0.0;
+      const double wj = bcast(wl, j) * di, uj = bcast(ul, j) * di;
+      if (lane == j) {
+        wl = wj;
+        ul = uj;
       }
-      wv[lane] = w;
-      ul = u;
-      for (int qd = 0; qd < p; ++qd) {
-        double g = 0.0;
-        for (int l = 0; l <= lane; ++l) g += row[l] * F[l * P + qd];
-        G[lane * P + qd] = g;
+      wl -= lrj * wj;
+      ul -= lrj * uj;
+      for (int qd = 1; qd < p; ++qd) {
+        const double gj = bcast(gl[qd], j) * di;
+        if (lane == j) gl[qd] = gj;
+        gl[qd] -= lrj * gj;
       }
     }
-    __syncthreads();
+    if (p > 0) gl[0] = wl;
     // beta = (G^T G)^-1 G^T u, by a p x p Cholesky on every lane
     double beta[PA];
     double bsc = 0.0;
@@ -779,10 +928,8 @@ __global__ __launch_bounds__(64) void k_neb_krige(NebArgs a) {
     if (p > 0) {
       double B[PA][PA], g[PA];
       for (int i = 0; i < p; ++i) {
-        const double gi = lane < k ? G[lane * P + i] : 0.0;
-        g[i] = wave_sum(gi * ul);
-        for (int j = 0; j <= i; ++j)
-          B[i][j] = wave_sum(gi * (lane < k ? G[lane * P + j] : 0.0));
+        g[i] = wave_sum(gl[i] * ul);
+        for (int j = 0; j <= i; ++j) B[i][j] = wave_sum(gl[i] * gl[j]);
       }
       bsc = B[0][0];
       for (int j = 0; j < p && !bfail; ++j) {
@@ -816,85 +963,111 @@ __global__ __launch_bounds__(64) void k_neb_krige(NebArgs a) {
       if (lane == 0) a.fail_list[atomicAdd(a.fail_cnt, 1)] = ch;
       continue;
     }
-    // alpha = M^T (u - G beta); for SK also C^-1 1 = M^T w
-    if (lane < k) {
-      double v = ul;
-      for (int qd = 0; qd < p; ++qd) v -= G[lane * P + qd] * beta[qd];
-      tv[lane] = v;
-    }
-    __syncthreads();
-    if (lane < k) {
-      double s = 0.0, r = 0.0;
-      for (int i = lane; i < k; ++i) {
-        const double mil = A[i * (i + 1) / 2 + lane];
-        s += mil * tv[i];
-        r += mil * wv[i];
-      }
-      al[lane] = s;
-      rv[lane] = r;
-    }
-    if (lane == 0) {
-      atomicMax(&a.skappa[t], dbits(kap));
-      if (!(kap <= 1e12)) atomicOr(&a.sflags[t], SPK_ST_ILLCOND);
-    }
-    __syncthreads();
-    const bool want_var = a.want_var && vo;
-    // ---- cells of the chunk ----
-    for (int64_t i = 0; i < np; ++i) {
-      const int c = a.cells[q.p0 + i];
-      const double x = a.cx[c], y = a.cy[c];
-      double d = INFINITY, cl = 0.0;
-      if (lane < k) {
-        d = dist2(x, y, xs[lane], ys[lane]);
-        const double gm = vg_entry(vg, d, sk ? 1 : 0, a.mvg);
-        cl = sk ? gm : c0 - gm;
-      }
-      double e1 = lane < k ? cl * al[lane] : 0.0;
-      double e2 = lane < k ? cl * rv[lane] : 0.0;
-      double qn = 0.0, an = 0.0;
-      if (want_var) {
-        __syncthreads();
-        if (lane < k) cc[lane] = cl;
-        __syncthreads();
-        if (lane < k) {
-          const double *row = A + lane * (lane + 1) / 2;
-          double u = 0.0;
-          for (int l = 0; l <= lane; ++l) u += row[l] * cc[l];
-          qn = u * u;
-          an = wv[lane] * u;
-        }
-      }
-      // one pass of the butterfly for all four sums
-      for (int off = 32; off > 0; off >>= 1) {
-        e1 += __shfl_xor(e1, off, 64);
-        e2 += __shfl_xor(e2, off, 64);
-        qn += __shfl_xor(qn, off, 64);
-        an += __shfl_xor(an, off, 64);
-      }
-      double e;
+    // backward solves with L^T: alpha = L^-T (u - G beta) (the dual
+    // weights); for SK also C^-1 1 = L^-T w (the sum of the weights)
+    double al_ = ul;
+    for (int qd = 0; qd < p; ++qd) al_ -= gl[qd] * beta[qd];
+    double rv_ = wl;
+    if (lane >= k) al_ = rv_ = 0.0;
+    for (int j = k - 1; j >= 0; --j) {
+      const double di = dinv[j];
+      // entry (j, lane) for lane < j
+      const double ljr = lane < j ? A[cofs(lane, k) + j - lane] : 0.0;
+      const double aj = bcast(al_, j) * di;
+      if (lane == j) al_ = aj;
+      al_ -= ljr * aj;
       if (sk) {
-        if (a.skm) {
-          e = mean + e1;
-        } else if (neb_close1(e2)) {
-          e = e1;
-        } else {
-          // the sum test fails: nearest station of the set
-          // (interp/steps.py:418-427)
-          double bd = d;
-          int bj = lane < k ? lane : 0x7fffffff;
-          wave_argmin(bd, bj);
-          e = zs[bj < k ? bj : 0];
-        }
-      } else {
-        e = e1 + beta[0];
-        for (int qd = 1; qd < p; ++qd)
-          e += a.cell_drift[(int64_t)(qd - 1) * a.C + c] * beta[qd];
-      }
-      if (lane == 0) {
-        eo[c] = e;
-        if (want_var) vo[c] = c0 - qn + (an - 1.0) * (an - 2.0) / bsc;
+        const double rj = bcast(rv_, j) * di;
+        if (lane == j) rv_ = rj;
+        rv_ -= ljr * rj;
       }
     }
+    if (lane == 0) atomicMax(&a.skappa[t], dbits(kap));
+    const bool want_var = a.want_var && vo;
+    // ---- cells of the chunk, four at a time ----
+    constexpr int NB = 4;
+    for (int i0 = 0; i0 < np; i0 += NB) {
+      double d[NB], cl[NB], e1[NB], e2[NB], qn[NB], an[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        d[u] = INFINITY;
+        cl[u] = 0.0;
+        if (lane < k && i0 + u < np) {
+          d[u] = dist2(cxl[i0 + u], cyl[i0 + u], xs[lane], ys[lane]);
+          const double gm = vg_entry_t<PH>(vg, d[u], sk ? 1 : 0, a.mvg);
+          cl[u] = sk ? gm : c0 - gm;
+        }
+        e1[u] = cl[u] * al_;
+        e2[u] = cl[u] * rv_;
+        qn[u] = 0.0;
+        an[u] = 0.0;
+      }
+      if (want_var) {
+        // y = L^-1 c per cell, one pass over the columns of L for the
+        // four; q = |y|^2, a = w . y = 1^T C^-1 c
+        double yl[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) yl[u] = cl[u];
+        for (int j = 0; j < k; ++j) {
+          const double lrj =
+              (lane > j && lane < k) ? A[cofs(j, k) + lane - j] : 0.0;
+          const double di = dinv[j];
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const double yj = bcast(yl[u], j) * di;
+            if (lane == j) yl[u] = yj;
+            yl[u] -= lrj * yj;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          qn[u] = yl[u] * yl[u];
+          an[u] = wl * yl[u];
+        }
+      }
+      // one pass of the butterfly for all sums
+      for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          e1[u] += __shfl_xor(e1[u], off, 64);
+          if (sk) e2[u] += __shfl_xor(e2[u], off, 64);
+          if (want_var) {
+            qn[u] += __shfl_xor(qn[u], off, 64);
+            an[u] += __shfl_xor(an[u], off, 64);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        if (i0 + u >= np) break;
+        const int c = cil[i0 + u];
+        double e;
+        if (sk) {
+          if (a.skm) {
+            e = mean + e1[u];
+          } else if (neb_close1(e2[u])) {
+            e = e1[u];
+          } else {
+            // the sum test fails: nearest station of the set
+            // (interp/steps.py:418-427)
+            double bd = d[u];
+            int bj = lane < k ? lane : 0x7fffffff;
+            wave_argmin(bd, bj);
+            e = zs[bj < k ? bj : 0];
+          }
+        } else {
+          e = e1[u] + beta[0];
+          for (int qd = 1; qd < p; ++qd)
+            e += a.cell_drift[(int64_t)(qd - 1) * a.C + c] * beta[qd];
+        }
+        if (lane == 0) {
+          eo[c] = e;
+          if (want_var)
+            vo[c] = c0 - qn[u] + (an[u] - 1.0) * (an[u] - 2.0) / bsc;
+        }
+      }
+    }
+    }  // KRIG
   }
 }
 
@@ -914,7 +1087,7 @@ __global__ __launch_bounds__(256) void k_neb_pinv_assemble(
   const int32_t *nl = a.nb + ((int64_t)t * a.C + a.cells[q.p0]) * a.K;
   __shared__ int sid[kNebMax];
   for (int j = threadIdx.x; j < k; j += 256)
-    sid[j] = a.idx[(int64_t)t * a.N + nl[j]];
+    sid[j] = a.idx[(int64_t)t * a.N + min(max(nl[j], 0), q.n - 1)];
   __syncthreads();
   const spk_vg &vg = a.vgs[t];
   double *Kf = Kb + (int64_t)f * P * P;
@@ -960,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_neb_pinv_cells(
   const double *V = Vb + (int64_t)f * P * P;
   const double *D = Db + (int64_t)f * P;
   for (int j = tid; j < k; j += 256) {
-    const int pos = nl[j];
+    const int pos = min(max(nl[j], 0), q.n - 1);
     const int s = a.idx[(int64_t)t * a.N + pos];
     sid[j] = s;
     xs[j] = a.sx[s];
@@ -982,7 +1155,8 @@ __global__ __launch_bounds__(256) void k_neb_pinv_cells(
     Pi[e] = s;
   }
   double mean = 0.0;
-  for (int j = 0; j < k; ++j) mean += a.zact[(int64_t)t * a.ldz + nl[j]];
+  for (int j = 0; j < k; ++j)
+    mean += a.zact[(int64_t)t * a.ldz + min(max(nl[j], 0), q.n - 1)];
   mean /= k;
   if (tid == 0) {
     double mn = INFINITY;
@@ -1193,15 +1367,27 @@ int spk_interp_neb_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     }
   }
   int *h_cnt = static_cast<int *>(ctx->h_sc);
-  size_t sort_bytes = 0, scan1 = 0, scan2 = 0;
+  // cells in Z order: the nrst selection tiles
+  int32_t *perm, *mid;
+  unsigned *mcode;
+  double *bbd;
+  SPK_TRY(ws_get(ctx, "neb_perm", 2 * C * 4, (void **)&perm));
+  mid = perm + C;
+  SPK_TRY(ws_get(ctx, "neb_mcode", 2 * C * 4 + 64, (void **)&mcode));
+  bbd = (double *)(mcode + 2 * C + 8);
+  size_t sort_bytes = 0, scan1 = 0, scan2 = 0, zsort = 0;
   SPK_HIP(hipcub::DeviceRadixSort::SortPairs(
       nullptr, sort_bytes, key0, key1, cell0, cell1, (int)SC, 0, 64, sm));
+  SPK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, zsort, mcode,
+                                             mcode + C, mid, perm, (int)C,
+                                             0, 32, sm));
   SPK_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, scan1, gs, gst,
                                             hipcub::Max(), (int)SC, sm));
   SPK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan2, cf, cid, (int)SC,
                                            sm));
   void *tmp;
-  const size_t tmp_bytes = std::max(sort_bytes, std::max(scan1, scan2));
+  const size_t tmp_bytes =
+      std::max(std::max(sort_bytes, zsort), std::max(scan1, scan2));
   SPK_TRY(ws_get(ctx, "neb_cub", tmp_bytes, &tmp));
   const size_t pie_lds = pie_lds_bytes((int)N, n_pies, K);
   if (sel == SPK_NEB_PIE && pie_lds > 64 * 1024)
@@ -1212,8 +1398,8 @@ int spk_interp_neb_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     ProfScope ps(ctx, "neb_select");
     if (sel == SPK_NEB_NRST)
       hipLaunchKernelGGL(k_neb_nrst, dim3((unsigned)ceil_div(C, kNebTile), Sb),
-                         dim3(256), 0, sm, sx, sy, ix, N, s_, cx, cy, C, K,
-                         nbuf, key0, cell0);
+                         dim3(256), 0, sm, sx, sy, ix, N, s_, cx, cy, C,
+                         (const int32_t *)perm, K, nbuf, key0, cell0);
     else
       hipLaunchKernelGGL(k_neb_pie, dim3((unsigned)C, Sb), dim3(64), pie_lds,
                          sm, sx, sy, ix, N, s_, cx, cy, C, K, n_pies, nbuf,
@@ -1236,6 +1422,15 @@ int spk_interp_neb_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     return SPK_OK;
   };
 
+  if (sel == SPK_NEB_NRST) {
+    hipLaunchKernelGGL(k_neb_bbox, dim3(1), dim3(1024), 0, sm, cx, cy, C, bbd);
+    hipLaunchKernelGGL(k_neb_morton, g1(C), dim3(256), 0, sm, cx, cy, C,
+                       (const double *)bbd, mcode, mid);
+    SPK_TRY(check_launch("k_neb_morton"));
+    size_t b = tmp_bytes;
+    SPK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, b, mcode, mcode + C, mid,
+                                               perm, (int)C, 0, 32, sm));
+  }
   // stations that any cell selects with every station present
   // (interp/steps.py:592-608)
   hipLaunchKernelGGL(k_neb_reset, dim3(1), dim3(64), 0, sm, sfl,
@@ -1253,6 +1448,18 @@ int spk_interp_neb_steps_dev(spk_ctx *ctx, const spk_problem *prob,
 
   const int p = neb_p(method, m);
   const int kfix = sel == SPK_NEB_PIE ? 1 : 0;
+  // Pow or Hol components anywhere in the call: the kernel variant with
+  // pow and sin (they cost registers, and so occupancy, when compiled in)
+  bool ph = false;
+  if (krig) {
+    std::vector<spk_vg> hv((size_t)T);
+    SPK_HIP(hipMemcpyAsync(hv.data(), vgs, T * sizeof(spk_vg),
+                           hipMemcpyDeviceToHost, sm));
+    SPK_HIP(hipStreamSynchronize(sm));
+    for (const spk_vg &v : hv)
+      for (int qd = 0; qd < v.n && qd < SPK_VG_MAX; ++qd)
+        ph = ph || v.type[qd] == SPK_VG_POW || v.type[qd] == SPK_VG_HOL;
+  }
   const int lds_krige = neb_lds_doubles(K, p) * 8;
   int hbad = 0;
   for (int64_t t0 = 0; t0 < T; t0 += S) {
@@ -1324,20 +1531,35 @@ int spk_interp_neb_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       ProfScope ps(ctx, "neb_krige");
       const unsigned grid =
           (unsigned)std::max<int64_t>(1, std::min<int64_t>(tot, 16384));
-      if (method == SPK_M_OK)
-        hipLaunchKernelGGL(k_neb_krige<1>, dim3(grid), dim3(64), lds_krige,
+      const dim3 g(grid), b(64);
+      if (!krig)
+        hipLaunchKernelGGL((k_neb_krige<0, false, false>), g, b, lds_krige,
+                           sm, a);
+      else if (method == SPK_M_OK && ph)
+        hipLaunchKernelGGL((k_neb_krige<1, true, true>), g, b, lds_krige, sm,
+                           a);
+      else if (method == SPK_M_OK)
+        hipLaunchKernelGGL((k_neb_krige<1, false, true>), g, b, lds_krige, sm,
+                           a);
+      else if (method == SPK_M_EDK && ph)
+        hipLaunchKernelGGL((k_neb_krige<kMaxP, true, true>), g, b, lds_krige,
                            sm, a);
       else if (method == SPK_M_EDK)
-        hipLaunchKernelGGL(k_neb_krige<kMaxP>, dim3(grid), dim3(64),
+        hipLaunchKernelGGL((k_neb_krige<kMaxP, false, true>), g, b,
                            lds_krige, sm, a);
+      else if (ph)
+        hipLaunchKernelGGL((k_neb_krige<0, true, true>), g, b, lds_krige, sm,
+                           a);
       else
-        hipLaunchKernelGGL(k_neb_krige<0>, dim3(grid), dim3(64), lds_krige,
-                           sm, a);
+        hipLaunchKernelGGL((k_neb_krige<0, false, true>), g, b, lds_krige, sm,
+                           a);
       SPK_TRY(check_launch("k_neb_krige"));
     }
     SPK_TRY(read_cnt());
     hbad |= h_cnt[2];
     const int nf = h_cnt[1];
+    prof_count(ctx, "neb_chunks", h_cnt[0]);
+    prof_count(ctx, "neb_pinv_chunks", nf);
     if (nf > 0) {
       ProfScope ps(ctx, "neb_pinv");
       const int P = K + p;

@@ -23,10 +23,24 @@ struct Prof {
   // FP32 triangular contraction, [1] 64-k chunks times plane products of
   // the int8 contraction
   unsigned long long *work = nullptr;
+  // host-side counts of work items (e.g. neighbourhood chunks)
+  std::vector<std::pair<std::string, double>> counts;
 };
 
 Prof &prof_of(spk_ctx *ctx);
 hipEvent_t prof_event(Prof &pf);
+
+// Adds v to the host-side count `name` while profiling is on.
+inline void prof_count(spk_ctx *ctx, const char *name, double v) {
+  Prof &pf = prof_of(ctx);
+  if (!pf.on) return;
+  for (auto &kv : pf.counts)
+    if (kv.first == name) {
+      kv.second += v;
+      return;
+    }
+  pf.counts.push_back({name, v});
+}
 
 struct ProfScope {
   spk_ctx *ctx;
