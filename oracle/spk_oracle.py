@@ -872,3 +872,40 @@ def time_ok_step_loop(stn_x, stn_y, cell_x, cell_y, z_t, vg_str,
             var[i] = (lmds * R[i]).sum() + lmds[n]
     t2 = time.perf_counter()
     return t1 - t0, (t2 - t1) / cell_x.size, est, var
+
+
+# ---------------------------------------------------------------------------
+# vgclus weight heuristic, vgclus/tvgs_clus.py:92-137
+# ---------------------------------------------------------------------------
+
+def threshold_wts(wts, min_kg_abs_wt):
+    """tvgs_clus.py:109-130 on sorted weights: weights whose share of
+    sum |w| is at most min_kg_abs_wt become 0 (unless that is all of them)
+    and the rest are rescaled to sum to one."""
+    wts = np.array(wts, dtype=np.float64)
+    if min_kg_abs_wt:
+        abs_wts = np.abs(wts)
+        rel_wts = abs_wts / abs_wts.sum()
+        zero_idxs = rel_wts <= min_kg_abs_wt
+        if zero_idxs.sum() == zero_idxs.size:
+            zero_idxs = np.zeros_like(zero_idxs, dtype=bool)
+        sclr = 1.0 / wts[~zero_idxs].sum()
+        assert zero_idxs.sum() < zero_idxs.size
+        wts[zero_idxs] = 0.0
+        wts[~zero_idxs] *= sclr
+    assert (wts[1:] - wts[:-1]).min() >= 0.0
+    assert np.isclose(wts.sum(), 1.0)
+    return wts
+
+
+def srtd_krg_wts(tvg_str, rnd_pts, min_kg_abs_wt=None):
+    """tvgs_clus.py:92-137: the ordinary-kriging weights (class API,
+    pyx:251-362) of rnd_pts[1:] for the point rnd_pts[0], sorted ascending,
+    then thresholded."""
+    rnd_pts = np.asarray(rnd_pts, dtype=np.float64)
+    out = ordinary_kriging(rnd_pts[1:, 0], rnd_pts[1:, 1],
+                           np.zeros(rnd_pts.shape[0] - 1),
+                           rnd_pts[:1, 0], rnd_pts[:1, 1], tvg_str)
+    wts = out['lambdas'].ravel().copy()
+    wts.sort()
+    return threshold_wts(wts, min_kg_abs_wt)

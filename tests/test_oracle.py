@@ -629,3 +629,39 @@ def test_krige_solve_matches_pinv_when_well_conditioned():
     assert ok1.all() and ok2.all()
     np.testing.assert_allclose(e2, e1, rtol=1e-10)
     np.testing.assert_allclose(v2, v1, rtol=1e-10)
+
+
+# ---- vgclus weight heuristic (vgclus/tvgs_clus.py:92-137) ----------------
+
+def test_threshold_wts_by_hand():
+    w = np.array([-0.05, 0.05, 0.1, 0.3, 0.6])   # sum |w| = 1.1
+    # shares 0.045, 0.045, 0.091, 0.27, 0.55: a cut at 0.05 drops the two
+    # smallest and rescales the rest by 1 / 1.0
+    got = O.threshold_wts(w, 0.05)
+    np.testing.assert_allclose(got, [0.0, 0.0, 0.1, 0.3, 0.6])
+    # a cut at 0.1 also drops 0.1 and rescales 0.3, 0.6 by 1 / 0.9
+    got = O.threshold_wts(w, 0.1)
+    np.testing.assert_allclose(got, [0.0, 0.0, 0.0, 1 / 3, 2 / 3])
+    # a cut above every share keeps them all (the reference's guard)
+    got = O.threshold_wts(np.array([0.2, 0.2, 0.2, 0.2, 0.2]), 0.5)
+    np.testing.assert_allclose(got, [0.2] * 5)
+    # no cut
+    np.testing.assert_array_equal(O.threshold_wts(w, None), w)
+
+
+def test_srtd_krg_wts_symmetric_and_sum_one():
+    """Four stations on a circle around the point: equal weights 1/4; a
+    random set: weights sorted and summing to one, a station at the point
+    takes all the weight (the class API's zero-diagonal Gamma)."""
+    ang = np.arange(4) * np.pi / 2
+    pts = np.r_[[[0.0, 0.0]], np.c_[np.cos(ang), np.sin(ang)] * 3.0]
+    w = O.srtd_krg_wts('1.0 Sph(10.0)', pts)
+    np.testing.assert_allclose(w, 0.25, rtol=1e-12)
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(0, 10, (12, 2))
+    w = O.srtd_krg_wts('0.1 Nug(0.0) + 1.0 Exp(5.0)', pts)
+    assert np.all(np.diff(w) >= 0) and abs(w.sum() - 1) < 1e-12
+    pts[5] = pts[0]
+    w = O.srtd_krg_wts('1.0 Exp(5.0)', pts)
+    np.testing.assert_allclose(w[-1], 1.0, atol=1e-10)
+    np.testing.assert_allclose(w[:-1], 0.0, atol=1e-10)
