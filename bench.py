@@ -54,6 +54,9 @@ def main():
                     help='time steps per bench step')
     ap.add_argument('--plain-steps', type=int, default=16)
     ap.add_argument('--grid', type=int, default=1000)
+    ap.add_argument('--models', default='Sph,Exp,Gau',
+                    help='variogram models the hourly steps draw from '
+                         '(the workload: Sph,Exp,Gau)')
     ap.add_argument('--cpu-cells', type=int, default=200)
     ap.add_argument('--gate-cells', type=int, default=4096,
                     help='sampled cells of every timed step compared with '
@@ -85,7 +88,7 @@ def main():
                          '1e-6, 10 %% of stations in 1-10 m pairs) and '
                          'report errors vs numpy.linalg.solve')
     ap.add_argument('--stress-cells', type=int, default=200)
-    ap.add_argument('--e2e', type=int, default=0,
+    ap.add_argument('--e2e', type=int, default=256,
                     help='also run this many hourly steps end to end: '
                          'proposal-seeded variogram fits of the step '
                          'values, then batched mixed OK with the fitted '
@@ -126,8 +129,10 @@ def main():
         f'warmup={W} plain_steps={P}')
     n_tot = (W + K) * B
     t = time.time()
-    vals, vgs = configs.hourly_steps(w, 0, n_tot + P)
-    log(f'generated {n_tot + P} steps in {time.time() - t:.1f}s')
+    models = tuple(args.models.split(','))
+    vals, vgs = configs.hourly_steps(w, 0, n_tot + P, models=models)
+    log(f'generated {n_tot + P} steps ({"/".join(models)}) in '
+        f'{time.time() - t:.1f}s')
     prob = DeviceProblem.from_workload(w, device=0)
     vrec = _lib.VG_DTYPE.itemsize
     n_act = (~np.isnan(vals)).sum(1)
@@ -480,6 +485,7 @@ def main():
         'config': {'workload': 'hourly: 2000 stations, 1000x1000 grid at '
                                '500 m, per-step variogram and 200-2000 '
                                'active stations, OK estimate + variance',
+                   'models': args.models,
                    'time_steps_per_bench_step': B, 'n_cells': C,
                    'n_stations': w.n_stns},
         'timed_region': 'H2D of the steps\' values and variogram '
