@@ -101,13 +101,20 @@ def main(argv=None):
     ap.add_argument('--train-seed', type=int, default=0)
     ap.add_argument('--out', default='proposal_net.npz')
     ap.add_argument('--data-out', default=None)
-    ap.add_argument('--data-in', default=None,
-                    help='train on features/labels saved by --data-out')
+    ap.add_argument('--data-in', default=None, nargs='+',
+                    help='train on features/labels saved by --data-out '
+                         '(several files are concatenated, so a large set '
+                         'can be labelled in parts with distinct --seed)')
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     if a.data_in:
-        with np.load(a.data_in, allow_pickle=False) as z:
-            feat, cls, reg = z['feat'], z['cls'], z['reg']
+        parts = []
+        for fn in a.data_in:
+            with np.load(fn, allow_pickle=False) as z:
+                parts.append((z['feat'], z['cls'], z['reg']))
+        feat = np.concatenate([q[0] for q in parts])
+        cls = np.concatenate([q[1] for q in parts])
+        reg = np.concatenate([q[2] for q in parts])
     else:
         feat, cls, reg = build_training_set(
             a.fields, a.chunk, a.seed, fit_chunk=a.fit_chunk,
@@ -115,6 +122,10 @@ def main(argv=None):
     t_data = time.perf_counter() - t0
     if a.data_out:
         np.savez_compressed(a.data_out, feat=feat, cls=cls, reg=reg)
+    if a.epochs <= 0:
+        print(f'labelled {feat.shape[0]} of {a.fields} fields in '
+              f'{t_data:.1f}s; no training (--epochs 0)', flush=True)
+        return
     t1 = time.perf_counter()
     net, hist = train(feat, cls, reg, epochs=a.epochs, seed=a.train_seed,
                       log=lambda s: print(s, flush=True))
