@@ -39,6 +39,8 @@ sys.path.insert(0, str(ROOT))
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
 TRI32_TRAFFIC = 2 * 5403411 * 1024 + 202497 * 1024
 TRI8_TRAFFIC = 2 * (7126920 + 150551) * 1024 + (50293 + 17090) * 1024
+# the split-BF16 contraction: not yet measured by counters
+TRIBF_TRAFFIC = None
 
 
 def log(*a):
@@ -94,6 +96,11 @@ def main():
                          'values, then batched mixed OK with the fitted '
                          'variograms')
     ap.add_argument('--stress-oracle-steps', type=int, default=20)
+    ap.add_argument('--precision', default='mixed',
+                    choices=('mixed', 'mixed_f32'),
+                    help='mixed: the well-conditioned steps\' variance '
+                         'contraction on split-BF16 operands; mixed_f32: '
+                         'on FP32 operands')
     ap.add_argument('--calib', action='store_true',
                     help='also run k_clip_cast over 2^28 float64 values '
                          '(2 GiB read, 1 GiB written) as the reference '
@@ -163,7 +170,8 @@ def main():
         vg_dd[:m * vrec].copy_(vg_h[s0 * vrec:s1 * vrec], non_blocking=True)
         ev0.record()
         _, _, info = interp_steps(prob, vals_d[:m], None, 'OK',
-                                  precision='mixed', est=est[:m], var=var[:m],
+                                  precision=args.precision, est=est[:m],
+                                  var=var[:m],
                                   vg_dev=vg_dd[:m * vrec], kappa=kap)
         ev1.record()
         clip_cast(est[:m], out=e32[:m])
@@ -192,13 +200,14 @@ def main():
         f'the inputs and D2H of the float32 fields, {value:.3e} '
         f'cell-estimates/s; device only {dev_ms / K:.1f} ms per batch')
     kern = {}
-    for nm in ('nearest', 'kgen', 'gemm_tri_f32', 'gemm_tri_f64',
-               'slice_cols', 'gemm_tri_i8', 'var_reduce', 'potrf', 'trtri',
-               'dual', 'slice_rows', 'assemble', 'special', 'cast_tri',
-               'pinv', 'pinv_contract'):
+    for nm in ('nearest', 'kgen', 'gemm_tri_bf16', 'gemm_tri_f32',
+               'gemm_tri_f64', 'slice_cols', 'gemm_tri_i8', 'var_reduce',
+               'potrf', 'trtri', 'dual', 'slice_rows', 'assemble', 'special',
+               'cast_tri', 'split_tri', 'pinv', 'pinv_contract'):
         ms, cnt = prof_get(nm)
         kern[nm] = (ms, cnt)
     macs32 = prof_work('gemm_tri_f32_macs')
+    macsbf = prof_work('gemm_tri_bf16_macs')
     ops8x = prof_work('gemm_tri_i8_ops')
     prof_enable(0, False)
     info_t = info_d.cpu().numpy()
@@ -221,12 +230,18 @@ def main():
     fl64 = float(np.sum(nsq[f64_steps]) * C)
     fl8 = float(np.sum(nsq[i8_steps]) * C)
     ms32, c32 = kern['gemm_tri_f32']
+    msbf, cbf = kern['gemm_tri_bf16']
     ms64, c64 = kern['gemm_tri_f64']
     ms8, c8 = kern['gemm_tri_i8']
     # the FP32 contraction skips the zero chunks of delta: its MFMA rate
     # counts the multiply-adds it executed, its algorithmic rate n^2 C
     ach32 = 2 * macs32 / (ms32 * 1e-3) / 1e12 if ms32 > 0 else 0.0
     alg32 = fl32 / (ms32 * 1e-3) / 1e12 if ms32 > 0 else 0.0
+    # split-BF16 contraction of the same steps: its BF16 MFMA rate counts
+    # the executed plane products (six per 32-row chunk), its algorithmic
+    # rate n^2 C (the FP32-equivalent L^-1 delta)
+    achbf = 2 * macsbf / (msbf * 1e-3) / 1e12 if msbf > 0 else 0.0
+    algbf = fl32 / (msbf * 1e-3) / 1e12 if msbf > 0 else 0.0
     ach64 = fl64 / (ms64 * 1e-3) / 1e12 if ms64 > 0 else 0.0
     # int8: it skips the zero chunks of delta too, so its MFMA rate counts
     # the plane products executed; the algorithmic count is 15 plane
@@ -237,8 +252,10 @@ def main():
     ach8 = ops8x / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
     alg8 = ops8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
     eq8 = fl8 / (ms8 * 1e-3) / 1e12 if ms8 > 0 else 0.0
-    # kgen: algorithmic HBM bytes = delta written (4 or 8 B) + cell coords
-    kb = float(np.sum(timed_n[f32_steps]) * C * 4 +
+    # kgen: algorithmic HBM bytes = delta written (FP32-tier steps: 4 B,
+    # or 6 B of bfloat16 planes; others 8 B) + cell coords
+    kb = float(np.sum(timed_n[f32_steps]) * C *
+               (6 if args.precision == 'mixed' else 4) +
                np.sum(timed_n[f64_steps | i8_steps]) * C * 8 +
                16 * C * TK)
     mskg, ckg = kern['kgen']
@@ -460,13 +477,19 @@ def main():
            'algorithmic_tflops': alg32, 'traffic': TRI32_TRAFFIC,
            'kernel': 'k_gemm<float,TRI> (L^-1 delta, FP32 MFMA, FP64 sums '
                      'every 32 rows)', 'traffic_note': note}
+    rbf = {'bound': 'mfma', 'achieved': achbf, 'peak': 2500.0,
+           'unit': 'TFLOP/s', 'frac': achbf / 2500.0,
+           'algorithmic_tflops': algbf, 'traffic': TRIBF_TRAFFIC,
+           'kernel': 'k_tri_bf (L^-1 delta on split-BF16 operands: six '
+                     'plane products on BF16 MFMA, FP32 sums per 32 rows, '
+                     'FP64 across them)', 'traffic_note': note}
     r8 = {'bound': 'mfma', 'achieved': ach8, 'peak': 5000.0,
           'unit': 'TOPS', 'frac': ach8 / 5000.0, 'traffic': TRI8_TRAFFIC,
           'algorithmic_tops': alg8, 'fp64_equivalent_tflops': eq8,
           'kernel': 'k_tri8 (L^-1 delta, exact int8 plane products, int8 '
                     'MFMA; 5- and 6-plane launches together)',
           'traffic_note': note}
-    dominant = max(((ms32, r32), (ms64, r64), (ms8, r8)),
+    dominant = max(((msbf, rbf), (ms32, r32), (ms64, r64), (ms8, r8)),
                    key=lambda x: x[0])[1]
     out = {
         'metric': 'OK cell-estimates/s incl. variance: batched mixed vs '
@@ -485,7 +508,7 @@ def main():
         'config': {'workload': 'hourly: 2000 stations, 1000x1000 grid at '
                                '500 m, per-step variogram and 200-2000 '
                                'active stations, OK estimate + variance',
-                   'models': args.models,
+                   'models': args.models, 'precision': args.precision,
                    'time_steps_per_bench_step': B, 'n_cells': C,
                    'n_stations': w.n_stns},
         'timed_region': 'H2D of the steps\' values and variogram '
@@ -500,6 +523,10 @@ def main():
         'accuracy_gate': gate,
         'roofline': dominant,
         'rooflines': {
+            'gemm_tri_bf16': {'bound': 'mfma', 'achieved': achbf,
+                              'peak': 2500.0, 'unit': 'TFLOP/s',
+                              'frac': achbf / 2500.0,
+                              'algorithmic_tflops': algbf},
             'gemm_tri_f32': {'bound': 'mfma', 'achieved': ach32,
                              'peak': 157.3, 'unit': 'TFLOP/s',
                              'frac': ach32 / 157.3,

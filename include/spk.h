@@ -138,10 +138,18 @@ enum { SPK_M_OK = 0, SPK_M_SK = 1, SPK_M_EDK = 2, SPK_M_IDW = 3,
 /* SPK_P_FP64 : every step on its own, FP64 end to end, the reference's
  *              per-cell lambda = K^-1 rhs as one FP64 MFMA GEMM per step.
  * SPK_P_MIXED: all steps of the call in batched launches; estimates in the
- *              dual form (FP64), variance by an FP32 MFMA contraction with
- *              FP64 Cholesky/Schur terms.
- * SPK_P_FP64B: the batched dual-form path with an FP64 contraction. */
-enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2 };
+ *              dual form (FP64), variance by a contraction with FP64
+ *              Cholesky/Schur terms whose precision follows the step's
+ *              condition: well-conditioned steps on split-BF16 operands
+ *              (three bfloat16 terms per value, six plane products on BF16
+ *              MFMA, FP32 sums per 32 rows, FP64 across them), the others
+ *              on exact int8 plane products or FP64.
+ * SPK_P_FP64B: the batched dual-form path with an FP64 contraction.
+ * SPK_P_MIXED_F32: SPK_P_MIXED with FP32 operands on FP32 MFMA for the
+ *              well-conditioned steps (also used by SPK_P_MIXED for
+ *              systems above 8 192 rows). */
+enum { SPK_P_FP64 = 0, SPK_P_MIXED = 1, SPK_P_FP64B = 2,
+       SPK_P_MIXED_F32 = 3 };
 
 /* Per-step status written to info[t] (bit flags). */
 #define SPK_ST_NOSTNS    1   /* no station: outputs untouched (NaN)        */

@@ -23,8 +23,9 @@ VG_MAX = 8
 
 M_OK, M_SK, M_EDK, M_IDW, M_NNB = range(5)
 METHODS = {'OK': M_OK, 'SK': M_SK, 'EDK': M_EDK, 'IDW': M_IDW, 'NNB': M_NNB}
-P_FP64, P_MIXED, P_FP64B = range(3)
-PRECISIONS = {'fp64': P_FP64, 'mixed': P_MIXED, 'fp64b': P_FP64B}
+P_FP64, P_MIXED, P_FP64B, P_MIXED_F32 = range(4)
+PRECISIONS = {'fp64': P_FP64, 'mixed': P_MIXED, 'fp64b': P_FP64B,
+              'mixed_f32': P_MIXED_F32}
 
 ST_NOSTNS, ST_MEANFILL, ST_FAILED, ST_ONESTN = 1, 2, 4, 8
 ST_ILLCOND, ST_FP64VAR, ST_INT8VAR, ST_PINV = 16, 32, 64, 128

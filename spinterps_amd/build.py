@@ -18,7 +18,7 @@ SRC = HERE / 'csrc'
 OUT = HERE / 'libspk.so'
 SOURCES = ['spk_core.hip', 'spk_factor.hip', 'spk_cells.hip', 'spk_neb.hip',
            'spk_vgfit.hip', 'spk_mlp.hip', 'spk_class.hip', 'spk_int8.hip',
-           'spk_pinv.hip', 'spk_idw.hip', 'spk_nebk.hip']
+           'spk_pinv.hip', 'spk_idw.hip', 'spk_nebk.hip', 'spk_bf16.hip']
 # rocSOLVER (dsyevd) and rocBLAS (dgemm) for the pseudo-inverse fallback of
 # steps whose covariance block does not factorise
 LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocsolver',

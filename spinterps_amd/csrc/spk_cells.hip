@@ -70,6 +70,10 @@ constexpr double kPinvKappa = 1e14;
 // the error left by the first step; the refined estimate is at least as
 // good), so unflagged steps stay well inside 1e-6.
 constexpr double kRefineTol = 1e-7;
+// L^-1 of the split-BF16 contraction: 1 = FP32 octets split into the
+// bfloat16 planes as k_tri_bf stages them (4 bytes per value read from
+// memory instead of 6), 0 = the planes themselves.
+constexpr int kBfA32 = 1;
 
 // np.isclose(x, 1.0) with rtol 1e-5, atol 1e-8 (interp/steps.py:418)
 __device__ inline bool isclose1(double x) {
@@ -762,7 +766,8 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
     const double *__restrict__ V, int64_t ldv, int64_t ld, int method, int m,
     double min_vg_val, int want_var, int all64, int skm,
     float *__restrict__ K32,
-    double *__restrict__ K64, int64_t ldk, const double *__restrict__ Gfull,
+    double *__restrict__ K64, int64_t ldk, uint4 *__restrict__ Kb,
+    const double *__restrict__ Gfull,
     double *__restrict__ abuf, double *__restrict__ est, int64_t ldo,
     double *__restrict__ dmax, uint8_t *__restrict__ nzb, int64_t nzt,
     int64_t nzk, int zskip) {
@@ -795,7 +800,13 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   // FP64 delta for the FP64 and the int8 contractions
   const bool use64 =
       all64 || (stp->flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR));
-  float *K32s = (var_on && !use64) ? K32 + (int64_t)sl * ld * ldk : nullptr;
+  // FP32-tier delta: FP32 (K32), or its three bfloat16 planes (Kb: octet
+  // o of cell c of plane p at (p ld / 8 + o) ldk + c)
+  float *K32s = (var_on && !use64 && !Kb) ? K32 + (int64_t)sl * ld * ldk
+                                          : nullptr;
+  uint4 *Kbs = (var_on && !use64 && Kb) ? Kb + (int64_t)sl * 3 * (ld / 8) * ldk + c
+                                        : nullptr;
+  const int64_t kbp = (ld / 8) * ldk;  // plane stride of Kbs
   double *K64s = (var_on && use64) ? K64 + (int64_t)sl * ld * ldk : nullptr;
   const double *Gs = var_on ? Gfull + (int64_t)sl * ld * ld : nullptr;
   double beta[kJ];
@@ -1046,39 +1057,78 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
         // sub-chunks of 16 stations
         auto var_loop = [&](auto tag) {
           constexpr int MT = decltype(tag)::value;
+          // delta of station j of the pass (and its estimate terms)
+          auto eval = [&](int j) {
+            const double d = dist2(x, y, stx[j], sty[j]);
+            double g;
+            if constexpr (MT < 0)
+              g = kg_vg<GEN>(vf, d, 0, min_vg_val);
+            else
+              g = vg_one<MT>(vo, d, min_vg_val);
+            const double k = c0 - g;
+            e += k * sal[j];
+            a += k * sw[j];
+            ke += k * sdal[j];
+            double delta = k;
+#pragma unroll
+            for (int q = 0; q < kJ; ++q)
+              if (slt[q] >= 0) delta -= sG[slt[q]][j] * beta[q];
+            dm = fmax(dm, fabs(delta));
+            if (delta != 0.0) nzf |= 1u << (j >> 4);
+            return delta;
+          };
           for (int j0 = 0; j0 < cnt; j0 += 16) {
             const int j1 = min(cnt, j0 + 16);
             if (MT >= 0 && cull_on && s_cull[j0 >> 4]) {
-              if (zwrite)
-                for (int j = j0; j < j1; ++j) {
-                  if (K32s)
-                    K32s[(int64_t)(base + j) * ldk + c] = 0.0f;
-                  else
-                    K64s[(int64_t)(base + j) * ldk + c] = 0.0;
+              if (zwrite) {
+                if (Kbs) {
+                  const int64_t o = (base + j0) >> 3;
+#pragma unroll
+                  for (int p = 0; p < 3; ++p) {
+                    Kbs[p * kbp + o * ldk] = uint4{0u, 0u, 0u, 0u};
+                    Kbs[p * kbp + (o + 1) * ldk] = uint4{0u, 0u, 0u, 0u};
+                  }
+                } else {
+                  for (int j = j0; j < j1; ++j) {
+                    if (K32s)
+                      K32s[(int64_t)(base + j) * ldk + c] = 0.0f;
+                    else
+                      K64s[(int64_t)(base + j) * ldk + c] = 0.0;
+                  }
                 }
+              }
               continue;
             }
-            for (int j = j0; j < j1; ++j) {
-              const double d = dist2(x, y, stx[j], sty[j]);
-              double g;
-              if constexpr (MT < 0)
-                g = kg_vg<GEN>(vf, d, 0, min_vg_val);
-              else
-                g = vg_one<MT>(vo, d, min_vg_val);
-              const double k = c0 - g;
-              e += k * sal[j];
-              a += k * sw[j];
-              ke += k * sdal[j];
-              double delta = k;
+            if (Kbs) {
+              // both octets of the 16 stations, zero beyond the pass
+              for (int h = 0; h < 2; ++h) {
+                const int jb = j0 + 8 * h;
+                uint32_t hw[4], mw[4], lw[4];
 #pragma unroll
-              for (int q = 0; q < kJ; ++q)
-                if (slt[q] >= 0) delta -= sG[slt[q]][j] * beta[q];
-              if (K32s)
-                K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
-              else
-                K64s[(int64_t)(base + j) * ldk + c] = delta;
-              dm = fmax(dm, fabs(delta));
-              if (delta != 0.0) nzf |= 1u << (j >> 4);
+                for (int u = 0; u < 8; ++u) {
+                  const int j = jb + u;
+                  const float df = j < j1 ? (float)eval(j) : 0.0f;
+                  uint16_t hh, mm2, ll;
+                  split3f(df, hh, mm2, ll);
+                  const int sh = (u & 1) * 16;
+                  if (!(u & 1)) hw[u / 2] = mw[u / 2] = lw[u / 2] = 0u;
+                  hw[u / 2] |= (uint32_t)hh << sh;
+                  mw[u / 2] |= (uint32_t)mm2 << sh;
+                  lw[u / 2] |= (uint32_t)ll << sh;
+                }
+                const int64_t o = (base + jb) >> 3;
+                Kbs[o * ldk] = uint4{hw[0], hw[1], hw[2], hw[3]};
+                Kbs[kbp + o * ldk] = uint4{mw[0], mw[1], mw[2], mw[3]};
+                Kbs[2 * kbp + o * ldk] = uint4{lw[0], lw[1], lw[2], lw[3]};
+              }
+            } else {
+              for (int j = j0; j < j1; ++j) {
+                const double delta = eval(j);
+                if (K32s)
+                  K32s[(int64_t)(base + j) * ldk + c] = (float)delta;
+                else
+                  K64s[(int64_t)(base + j) * ldk + c] = delta;
+              }
             }
           }
         };
@@ -1507,6 +1557,9 @@ int spk_prof_work(spk_ctx *ctx, const char *name, double *value) {
   if (nm == "gemm_tri_f32_macs") {
     slot = 0;  // 16 k x 64 rows x 128 cells multiply-adds per chunk
     per = 16.0 * 64.0 * 128.0;
+  } else if (nm == "gemm_tri_bf16_macs") {
+    slot = 2;  // 32 k x 64 rows x 128 cells x 6 plane products per chunk
+    per = 32.0 * 64.0 * 128.0 * 6.0;
   } else if (nm == "gemm_tri_i8_ops") {
     slot = 1;  // 2 x 64 k x 128 rows x 64 cells ops per chunk and product
     per = 2.0 * 64.0 * 128.0 * 64.0;
@@ -1578,7 +1631,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       (m < 1 || m > kMaxP - 1 || !prob->stn_drift || !prob->cell_drift))
     return set_error(SPK_EARG, "EDK needs 1..%d drifts", kMaxP - 1);
   const int prec = opts->precision;
-  if (prec < SPK_P_FP64 || prec > SPK_P_FP64B)
+  if (prec < SPK_P_FP64 || prec > SPK_P_MIXED_F32)
     return set_error(SPK_EARG, "unknown precision %d", prec);
   const bool want_var = opts->want_var && method == SPK_M_OK && var;
   const int skm = method == SPK_M_SK && opts->sk_mean ? 1 : 0;
@@ -1608,6 +1661,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int p = method == SPK_M_OK ? 1 : method == SPK_M_EDK ? 1 + m : 0;
   const int64_t ld = round_up(N + (p > 0 ? p : 1), 64);
   const int64_t strideM = ld * ld;
+  // the mixed path; its FP32-tier steps contract split-BF16 planes
+  // (SPK_P_MIXED, systems up to kBfMaxLd) or FP32 operands
+  const bool mixed = prec == SPK_P_MIXED || prec == SPK_P_MIXED_F32;
+  const bool bfop = prec == SPK_P_MIXED && ld <= kBfMaxLd;
   const double thr = opts->min_var_thr;
   const double mvg = opts->min_vg_val;
   // ---- IDW / nearest neighbour: no systems ---------------------------------
@@ -1634,8 +1691,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       strideM * 8 * 2 +
       (prec == SPK_P_FP64 ? strideM * 8
                           : (want_var ? strideM * 8 : 0) +
-                                (prec == SPK_P_MIXED && want_var ? strideM * 4
-                                                                 : 0));
+                                (mixed && want_var
+                                     ? strideM * (bfop ? 6 : 4)
+                                     : 0));
   const int64_t budget = (int64_t)20 << 30;
   int64_t S = std::max<int64_t>(1, std::min<int64_t>(budget / per_step, 64));
   S = std::min<int64_t>(S, T);
@@ -1644,6 +1702,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     int32_t *idx;
     double *zact, *A, *Li, *M, *V, *T0, *Gm;
     float *L32;
+    uint4 *Lb;   // split-BF16 planes of L^-1 (bfop)
     int8_t *L8;  // int8 planes of L^-1 rows (mixed path)
     int *ea;     // their row exponents
     StepSc *sc;
@@ -1684,8 +1743,11 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     } else {
       SPK_TRY(get("V", S * 4 * ld * 8, (void **)&q.V));
       if (want_var) SPK_TRY(get("Gm", S * strideM * 8, (void **)&q.Gm));
-      if (prec == SPK_P_MIXED && want_var) {
-        SPK_TRY(get("L32", S * strideM * 4, (void **)&q.L32));
+      if (mixed && want_var) {
+        if (bfop)
+          SPK_TRY(get("Lb", S * strideM * 6, (void **)&q.Lb));
+        else
+          SPK_TRY(get("L32", S * strideM * 4, (void **)&q.L32));
         SPK_TRY(get("L8", S * kInt8Planes * R8 * KS8, (void **)&q.L8));
         SPK_TRY(get("ea", S * R8 * sizeof(int), (void **)&q.ea));
       }
@@ -1707,9 +1769,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   if (!batched) {
     Cs = std::min<int64_t>(C, (int64_t)(rhs_budget / ((N + p + 64) * 8)));
   } else if (want_var) {
-    // bytes per (step, station, cell): FP64 delta, plus FP32 delta and
-    // int8 planes in the mixed path
-    const int64_t eb = prec == SPK_P_MIXED ? 12 + kInt8Planes : 8;
+    // bytes per (step, station, cell): FP64 delta, plus the FP32 delta
+    // or its three bfloat16 planes and the int8 planes in the mixed path
+    const int64_t eb = mixed ? (bfop ? 14 : 12) + kInt8Planes : 8;
     Cs = std::min<int64_t>(C, (int64_t)(2 * rhs_budget / (S * ld * eb)));
   }
   // whole 256-cell blocks of k_kgen: a block's cells share a table of
@@ -1721,6 +1783,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int nrb = (int)ceil_div(N, 64);
   double *R = nullptr, *abuf = nullptr, *K64 = nullptr, *qpart = nullptr;
   float *K32 = nullptr;
+  uint4 *Kb = nullptr;
   int32_t *nnb = nullptr, *nrst = nullptr;
   int8_t *B8 = nullptr;
   int *eb8 = nullptr;
@@ -1736,7 +1799,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // int8 slicing) skips the unmarked chunks when the FP32 contraction's
   // chunk list (256 chunks of 16 rows) covers the triangle: kgen then
   // leaves culled chunks unwritten
-  const int zskip = prec == SPK_P_MIXED && ceil_div(N, 16) <= 256 ? 1 : 0;
+  const int zskip = mixed && ceil_div(N, 16) <= 256 ? 1 : 0;
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -1744,8 +1807,11 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     // FP64 buffer for the batched FP64 path and for the steps the mixed
     // path promotes to FP64 (SPK_ST_FP64VAR)
     SPK_TRY(ws_get(ctx, "K64", S * ld * ldk * 8, (void **)&K64));
-    if (prec == SPK_P_MIXED) {
-      SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
+    if (mixed) {
+      if (bfop)
+        SPK_TRY(ws_get(ctx, "Kb", S * ld * ldk * 6, (void **)&Kb));
+      else
+        SPK_TRY(ws_get(ctx, "K32", S * ld * ldk * 4, (void **)&K32));
       SPK_TRY(ws_get(ctx, "B8", S * kInt8Planes * ldk * KS8, (void **)&B8));
       SPK_TRY(ws_get(ctx, "eb8", S * ldk * sizeof(int), (void **)&eb8));
       SPK_TRY(ws_get(ctx, "dmax", S * ldk * 8, (void **)&dmax));
@@ -1771,9 +1837,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // applies to all models.
   const double kap = opts->fp64var_kappa;
   const double fp64var_thr =
-      prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e5) : INFINITY;
+      mixed ? (kap > 0.0 ? kap : 1e5) : INFINITY;
   const double fp64var_thr_gau =
-      prec == SPK_P_MIXED ? (kap > 0.0 ? kap : 1e3) : INFINITY;
+      mixed ? (kap > 0.0 ? kap : 1e3) : INFINITY;
   // Above the FP32 limit and up to this one the variance uses exact int8
   // plane products (five 7-bit planes per operand and the plane products
   // up to the fourth diagonal up to condition estimate 1e7, six planes and
@@ -1789,7 +1855,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // for up to 22 000 active stations; larger systems use FP64.
   const double k8 = opts->int8var_kappa;
   const double int8var_thr =
-      prec != SPK_P_MIXED || N > 20000
+      !mixed || N > 20000
           ? -INFINITY
           : (k8 > 0.0 ? k8 : (k8 < 0.0 ? -INFINITY : INFINITY));
 
@@ -1918,17 +1984,22 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         hipLaunchKernelGGL(k_cast_tri, dim3(1024, Sb), dim3(256), 0, sf, q.Li,
                            q.L32, ld, strideM, q.sc);
       }
+      if (q.Lb) {
+        ProfScope ps(ctx, "split_tri", sf);
+        hipLaunchKernelGGL(k_split_tri<kBfA32>, dim3(1024, Sb), dim3(256), 0, sf,
+                           q.Li,
+                           ld, strideM, q.sc, q.Lb);
+      }
     }
     SPK_TRY(check_launch("factorise"));
     // pseudo-inverses of the failed steps' bordered systems, written where
     // the plain contraction reads K^-1: the M slab (plain path) or the
     // step's L^-1 slab, which a failed step leaves unused (batched path)
-    for (int s2 : q.pinv) {
+    if (!q.pinv.empty()) {
       ProfScope ps(ctx, "pinv", sf);
-      SPK_TRY(pinv_step(ctx, sf, sx, sy, q.idx + (int64_t)s2 * N, q.sc + s2,
-                        h_sc[s2].n, gb + s2, method, m, prob->stn_drift, mvg,
-                        q.A + s2 * strideM,
-                        (batched ? q.Li : q.M) + s2 * strideM, ld));
+      SPK_TRY(pinv_steps(ctx, sf, sx, sy, q.idx, N, q.sc, h_sc, q.pinv, gb,
+                         method, m, prob->stn_drift, mvg,
+                         batched ? q.Li : q.M, strideM, ld));
     }
     return SPK_OK;
   };
@@ -1987,7 +2058,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           hipLaunchKernelGGL(k_nearest, dim3((unsigned)ceil_div(cn, 256), Sb),
                              dim3(256), 0, sm, sx, sy, q.idx, N, q.sc,
                              (perm ? pcx : cx) + cb, (perm ? pcy : cy) + cb,
-                             cn, nrst, ldk, prec == SPK_P_MIXED ? 1 : 0);
+                             cn, nrst, ldk, mixed ? 1 : 0);
           SPK_TRY(check_launch("k_nearest"));
         }
         const int64_t nzt = ceil_div(cn, 128);
@@ -2003,7 +2074,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                              perm ? perm + cb : (const int32_t *)nullptr,
                              (const int32_t *)nrst, ldk, q.V,
                              4 * ld, ld, method, m, mvg, want_var ? 1 : 0,
-                             prec == SPK_P_MIXED ? 0 : 1, skm, K32, K64, ldk,
+                             mixed ? 0 : 1, skm, K32, K64, ldk, Kb,
                              q.Gm,
                              abuf, perm ? eb : eb + cb, C, dmax, nzb,
                              nzt, nzk, zskip);
@@ -2039,7 +2110,14 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           SPK_TRY(check_launch("k_tri8"));
         }
         const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
-        if (prec == SPK_P_MIXED) {
+        if (bfop) {
+          ProfScope ps(ctx, "gemm_tri_bf16");
+          hipLaunchKernelGGL((k_tri_bf<0, kBfA32>), grid, dim3(256), 0, sm,
+                             (const uint4 *)q.Lb, ld, (const uint4 *)Kb, ldk,
+                             q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
+                             prof_of(ctx).on ? prof_of(ctx).work + 2
+                                             : nullptr);
+        } else if (mixed) {
           ProfScope ps(ctx, "gemm_tri_f32");
           // k chunks of 16: FP32 sums of 32 products, 3 waves per SIMD
           hipLaunchKernelGGL((k_gemm<float, false, 16>), grid, dim3(256), 0, sm,
@@ -2057,7 +2135,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          (unsigned)ceil_div(cn, 128), Sb);
           hipLaunchKernelGGL(k_tri64, g64, dim3(256), 0, sm, q.Li, ld, strideM,
                              (const double *)K64, ldk, ld * ldk, q.sc, cn,
-                             qpart, ldk, nrb, prec == SPK_P_MIXED ? 2 : 0);
+                             qpart, ldk, nrb, mixed ? 2 : 0);
         }
         SPK_TRY(check_launch("k_gemm<tri>"));
         {

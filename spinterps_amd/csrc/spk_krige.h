@@ -1,6 +1,8 @@
 // Shared declarations of the kriging kernels.
 #pragma once
 
+#include <vector>
+
 #include "spk_internal.h"
 
 namespace spk {
@@ -37,14 +39,16 @@ __global__ void k_idw_mfma(const double *sx, const double *sy, int64_t N,
                            int64_t C, double p, double *est, int *bad);
 
 // ---- pseudo-inverse fallback of failed steps (spk_pinv.hip) ----------------
-// Builds step s's bordered system in A_s (ld), eigendecomposes it and
-// writes pinv(K) to M_s (ld), all on stream st; records sc_s->pinv and
-// sc_s->kappa on the device.  n_s: the step's active stations.
-int pinv_step(spk_ctx *ctx, hipStream_t st, const double *sx,
-              const double *sy, const int32_t *idx_s, StepSc *sc_s,
-              int n_s, const spk_vg *vg_s, int method, int m,
-              const double *stn_drift, double min_vg_val, double *A_s,
-              double *M_s, int64_t ld);
+// For the steps `steps` of a batch (step records sc on the device, their
+// host copies h_sc): builds each bordered system, zero-padded to the
+// largest, eigendecomposes them in one batched call and writes pinv(K) of
+// step s to out + s stride (leading dimension ld), all on stream st;
+// records sc[s].pinv and sc[s].kappa on the device.
+int pinv_steps(spk_ctx *ctx, hipStream_t st, const double *sx,
+               const double *sy, const int32_t *idx, int64_t N, StepSc *sc,
+               const StepSc *h_sc, const std::vector<int> &steps,
+               const spk_vg *vgs, int method, int m, const double *stn_drift,
+               double min_vg_val, double *out, int64_t stride, int64_t ld);
 // The step leaves SPK_ST_FAILED for the plain contraction when its
 // pseudo-inverse is ready (SPK_ST_PINV | SPK_ST_ILLCOND).
 __global__ void k_mark_pinv(StepSc *sc);
@@ -121,6 +125,44 @@ int cell_order(spk_ctx *ctx, const double *cx, const double *cy, int64_t C,
                int32_t *perm, double *pcx, double *pcy);
 __global__ void k_cast_tri(const double *Li, float *L32, int64_t ld,
                            int64_t stride, const StepSc *sc);
+
+// ---- split-BF16 operands of the FP32-tier variance contraction -----------
+// (spk_bf16.hip).  A value x is held as three bfloat16 terms hi + mid + lo,
+// each the nearest bfloat16 (ties to even) to what the previous ones
+// leave; the remainders are exact in FP32.  The planes store 8 consecutive
+// k of one row (L^-1) or one cell (delta) in 16 bytes.
+__device__ inline uint16_t bf16_bits(float x) {
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
+__device__ inline float bf16_float(uint16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+__device__ inline void split3f(float x, uint16_t &h, uint16_t &m,
+                               uint16_t &l) {
+  h = bf16_bits(x);
+  const float r = x - bf16_float(h);
+  m = bf16_bits(r);
+  l = bf16_bits(r - bf16_float(m));
+}
+// from FP64: the remainders are taken in FP64
+__device__ inline void split3d(double x, uint16_t &h, uint16_t &m,
+                               uint16_t &l) {
+  h = bf16_bits((float)x);
+  const double r = x - (double)bf16_float(h);
+  m = bf16_bits((float)r);
+  l = bf16_bits((float)(r - (double)bf16_float(m)));
+}
+// Largest system (ld) the split contraction takes: its chunk list holds
+// ld / 32 entries.
+constexpr int64_t kBfMaxLd = 8192;
+template <int AF>
+__global__ void k_split_tri(const double *Li, int64_t ld, int64_t stride,
+                            const StepSc *sc, uint4 *Lb);
+template <int PF, int AF>
+__global__ void k_tri_bf(const uint4 *Lb, int64_t ld, const uint4 *Db,
+                         int64_t ldk, const StepSc *sc, int64_t Cs,
+                         double *qpart, int nrb, const uint8_t *nzb,
+                         int64_t nzk, unsigned long long *nexec);
 // exact int8 plane products (spk_int8.hip): up to 6 planes per operand;
 // steps up to condition estimate kInt8Deep use 5 planes and the plane
 // products up to the fourth diagonal, steps above it 6 planes and the

@@ -11,6 +11,9 @@
 // dropped), and leaves it where the plain FP64 contraction reads its K^-1.
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "spk_krige.h"
 
 namespace spk {
@@ -19,18 +22,19 @@ namespace spk {
 // dimension ld (interp/steps.py:181-234): OK [[G, 1], [1^T, 0]], EDK
 // [[G, 1, S], [1^T, 0, 0], [S^T, 0, 0]], SK the covariance matrix, with
 // G / Cov the entries fill_vg_var_arr gives (pyx:146-226) at the distances
-// of the step's active stations.  Grid (tiles, tiles) of 64 x 64 tiles.
+// of the step's active stations, zero-padded to Pw x Pw (Pw >= n + p).
+// Grid (tiles, tiles) of 64 x 64 tiles covering Pw.
 __global__ __launch_bounds__(256) void k_pinv_assemble(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, const StepSc *__restrict__ sc,
     const spk_vg *__restrict__ vgp, int method, int m,
     const double *__restrict__ stn_drift, double min_vg_val,
-    double *__restrict__ K, int64_t ld) {
+    double *__restrict__ K, int64_t ld, int Pw) {
   const int n = sc->n;
   const int p = method == SPK_M_OK ? 1 : method == SPK_M_EDK ? 1 + m : 0;
   const int P = n + p;
   const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-  if (i0 >= P || j0 >= P) return;
+  if (i0 >= Pw || j0 >= Pw) return;
   __shared__ double rx[64], ry[64], cx[64], cy[64];
   __shared__ spk_vg vg;
   const int tid = threadIdx.x;
@@ -49,9 +53,11 @@ __global__ __launch_bounds__(256) void k_pinv_assemble(
   for (int e = tid; e < 64 * 64; e += 256) {
     const int r = e & 63, c = e >> 6;
     const int i = i0 + r, j = j0 + c;
-    if (i >= P || j >= P) continue;
+    if (i >= Pw || j >= Pw) continue;
     double v;
-    if (i < n && j < n) {
+    if (i >= P || j >= P) {
+      v = 0.0;  // padding
+    } else if (i < n && j < n) {
       // the upper triangle's value, mirrored (fill_vg_var_arr with the
       // diagonal flag computes g >= h and copies it below)
       const int a = min(i, j), b = max(i, j);
@@ -71,19 +77,21 @@ __global__ __launch_bounds__(256) void k_pinv_assemble(
 }
 
 // W[:, k] = V[:, k] / D[k] where |D[k]| > 1e-15 max |D| (numpy.linalg.pinv's
-// rcond), else 0.  D ascending (dsyevd), so max |D| is at an end.  Block k
-// per column.  Block 0 also records the outcome in the step record: pinv
-// = 1 when the eigendecomposition converged (2 when eigenvalues were
-// dropped), and kappa = the ratio of the largest to the smallest kept
+// rcond), else 0, for the Pw eigenpairs of a system zero-padded from P to
+// Pw (the padding adds Pw - P zero eigenvalues, which the cutoff drops).
+// D ascending (dsyevd), so max |D| is at an end.  Block k per column.
+// Block 0 also records the outcome in the step record: pinv = 1 when the
+// eigendecomposition converged (2 when eigenvalues of the system itself
+// were dropped), and kappa = the ratio of the largest to the smallest kept
 // |eigenvalue|.
 __global__ __launch_bounds__(256) void k_pinv_scale(
-    const double *__restrict__ V, int64_t ld, int P,
+    const double *__restrict__ V, int64_t ld, int Pw, int P,
     const double *__restrict__ D, const int *__restrict__ info,
     double *__restrict__ W, StepSc *__restrict__ sc) {
   const int n = sc->n;
   const int k = blockIdx.x;
-  if (k >= P) return;
-  const double mx = fmax(fabs(D[0]), fabs(D[P - 1]));
+  if (k >= Pw) return;
+  const double mx = fmax(fabs(D[0]), fabs(D[Pw - 1]));
   const double cut = 1e-15 * mx;
   const double dk = D[k];
   const double wk = (fabs(dk) > cut) ? 1.0 / dk : 0.0;
@@ -91,14 +99,15 @@ __global__ __launch_bounds__(256) void k_pinv_scale(
     W[i + (int64_t)k * ld] = V[i + (int64_t)k * ld] * wk;
   if (k == 0 && threadIdx.x == 0) {
     double mn = INFINITY;
-    bool dropped = false;
-    for (int q = 0; q < P; ++q) {
+    int ncut = 0;
+    for (int q = 0; q < Pw; ++q) {
       if (fabs(D[q]) > cut)
         mn = fmin(mn, fabs(D[q]));
       else
-        dropped = true;
+        ++ncut;
     }
     // 1: full rank, 2: eigenvalues under the cutoff were dropped
+    const bool dropped = ncut > Pw - P;
     sc->pinv = (*info == 0 && n > 0) ? (dropped ? 2 : 1) : 0;
     sc->kappa = mx / mn;
   }
@@ -249,14 +258,16 @@ static int blas_status(rocblas_status st, const char *what) {
   return SPK_OK;
 }
 
-int pinv_step(spk_ctx *ctx, hipStream_t st, const double *sx,
-              const double *sy, const int32_t *idx_s, StepSc *sc_s,
-              int n_s, const spk_vg *vg_s, int method, int m,
-              const double *stn_drift, double min_vg_val, double *A_s,
-              double *M_s, int64_t ld) {
+int pinv_steps(spk_ctx *ctx, hipStream_t st, const double *sx,
+               const double *sy, const int32_t *idx, int64_t N, StepSc *sc,
+               const StepSc *h_sc, const std::vector<int> &steps,
+               const spk_vg *vgs, int method, int m, const double *stn_drift,
+               double min_vg_val, double *out, int64_t stride, int64_t ld) {
   const int p = method == SPK_M_OK ? 1 : method == SPK_M_EDK ? 1 + m : 0;
-  const int P = n_s + p;
-  if (P < 1) return SPK_OK;
+  const int nb = (int)steps.size();
+  int Pw = 0;
+  for (int s : steps) Pw = std::max(Pw, h_sc[s].n + p);
+  if (nb == 0 || Pw < 1) return SPK_OK;
   if (!ctx->blas) {
     rocblas_handle h = nullptr;
     SPK_TRY(blas_status(rocblas_create_handle(&h), "rocblas_create_handle"));
@@ -264,31 +275,50 @@ int pinv_step(spk_ctx *ctx, hipStream_t st, const double *sx,
   }
   rocblas_handle h = (rocblas_handle)ctx->blas;
   SPK_TRY(blas_status(rocblas_set_stream(h, st), "rocblas_set_stream"));
-  double *D, *E, *W;
+  // the systems side by side (stride ld x ld), their eigenvalues and the
+  // off-diagonals of the tridiagonal forms, one scaled eigenvector matrix
+  double *Ab, *D, *E, *W;
   int *info;
-  SPK_TRY(ws_get(ctx, "pinv_D", 2 * ld * sizeof(double), (void **)&D));
-  E = D + ld;
-  SPK_TRY(ws_get(ctx, "pinv_info", 256, (void **)&info));
-  SPK_TRY(ws_get(ctx, "pinv_W", ld * ld * sizeof(double), (void **)&W));
-  const int tiles = (int)ceil_div(P, 64);
-  hipLaunchKernelGGL(k_pinv_assemble, dim3(tiles, tiles), dim3(256), 0, st,
-                     sx, sy, idx_s, sc_s, vg_s, method, m, stn_drift,
-                     min_vg_val, A_s, ld);
+  SPK_TRY(ws_get(ctx, "pinv_A", (size_t)nb * ld * ld * sizeof(double),
+                 (void **)&Ab));
+  SPK_TRY(ws_get(ctx, "pinv_D", (size_t)2 * nb * ld * sizeof(double),
+                 (void **)&D));
+  E = D + (int64_t)nb * ld;
+  SPK_TRY(ws_get(ctx, "pinv_info", (size_t)nb * sizeof(int) + 256,
+                 (void **)&info));
+  SPK_TRY(ws_get(ctx, "pinv_W", (size_t)ld * ld * sizeof(double),
+                 (void **)&W));
+  const int tiles = (int)ceil_div(Pw, 64);
+  for (int k = 0; k < nb; ++k) {
+    const int s = steps[k];
+    hipLaunchKernelGGL(k_pinv_assemble, dim3(tiles, tiles), dim3(256), 0, st,
+                       sx, sy, idx + (int64_t)s * N, sc + s, vgs + s, method,
+                       m, stn_drift, min_vg_val, Ab + (int64_t)k * ld * ld,
+                       ld, Pw);
+  }
   SPK_TRY(check_launch("k_pinv_assemble"));
-  SPK_TRY(blas_status(rocsolver_dsyevd(h, rocblas_evect_original,
-                                       rocblas_fill_lower, P, A_s, (int)ld,
-                                       D, E, info),
-                      "rocsolver_dsyevd"));
-  hipLaunchKernelGGL(k_pinv_scale, dim3(P), dim3(256), 0, st, A_s, ld, P,
-                     D, info, W, sc_s);
-  SPK_TRY(check_launch("k_pinv_scale"));
-  // M = W V^T = V diag(1 / D) V^T over the kept eigenvalues
+  SPK_TRY(blas_status(
+      rocsolver_dsyevd_strided_batched(
+          h, rocblas_evect_original, rocblas_fill_lower, Pw, Ab, (int)ld,
+          (rocblas_stride)(ld * ld), D, (rocblas_stride)ld, E,
+          (rocblas_stride)ld, info, nb),
+      "rocsolver_dsyevd_strided_batched"));
   const double one = 1.0, zero = 0.0;
-  SPK_TRY(blas_status(rocblas_dgemm(h, rocblas_operation_none,
-                                    rocblas_operation_transpose, P, P, P,
-                                    &one, W, (int)ld, A_s, (int)ld, &zero,
-                                    M_s, (int)ld),
-                      "rocblas_dgemm"));
+  for (int k = 0; k < nb; ++k) {
+    const int s = steps[k];
+    const int P = h_sc[s].n + p;
+    const double *V = Ab + (int64_t)k * ld * ld;
+    hipLaunchKernelGGL(k_pinv_scale, dim3(Pw), dim3(256), 0, st, V, ld, Pw,
+                       P, D + (int64_t)k * ld, info + k, W, sc + s);
+    SPK_TRY(check_launch("k_pinv_scale"));
+    // M = W V^T = V diag(1 / D) V^T over the kept eigenvalues; the rows
+    // and columns of the system itself (P of them) are formed
+    SPK_TRY(blas_status(rocblas_dgemm(h, rocblas_operation_none,
+                                      rocblas_operation_transpose, P, P, Pw,
+                                      &one, W, (int)ld, V, (int)ld, &zero,
+                                      out + (int64_t)s * stride, (int)ld),
+                        "rocblas_dgemm"));
+  }
   return SPK_OK;
 }
 

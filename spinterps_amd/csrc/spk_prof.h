@@ -21,7 +21,7 @@ struct Prof {
   std::vector<std::pair<std::string, std::pair<double, int64_t>>> stats;
   // device counters of executed work: [0] 16-row chunks multiplied by the
   // FP32 triangular contraction, [1] 64-k chunks times plane products of
-  // the int8 contraction
+  // the int8 contraction, [2] 32-row chunks of the split-BF16 contraction
   unsigned long long *work = nullptr;
   // host-side counts of work items (e.g. neighbourhood chunks)
   std::vector<std::pair<std::string, double>> counts;

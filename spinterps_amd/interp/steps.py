@@ -123,6 +123,11 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     where sum(lambda) is not ~1), 'SKM' (simple kriging with the step mean
     as known mean, m + lambda^T (z - m)), 'EDK', 'IDW' or 'NNB'.
 
+    precision: 'fp64' (each step on its own, FP64 end to end), 'mixed'
+    (batched; the well-conditioned steps' variance contraction on
+    split-BF16 operands), 'mixed_f32' (the same with FP32 operands) or
+    'fp64b' (batched, FP64 contraction).
+
     Returns (est [T x C] float64, var [T x C] float64 or None,
     info [T] int32) as device tensors.  info holds SPK_ST_* flags per step:
     1 no station (outputs left NaN), 2 mean fill, 4 failed system
@@ -281,7 +286,7 @@ def prof_get(name, device=0):
 
 def prof_work(name, device=0):
     """Executed work counted while profiling is on ('gemm_tri_f32_macs',
-    'gemm_tri_i8_ops')."""
+    'gemm_tri_bf16_macs', 'gemm_tri_i8_ops', 'neb_chunks')."""
     ctx = context(device)
     v = ctypes.c_double()
     check(lib().spk_prof_work(ctx.h, name.encode(), ctypes.byref(v)))

@@ -384,6 +384,55 @@ def test_mixed_precision_policy_500m_grid(gpu, mode):
     assert rv[cmp].max() < 1e-6, (worst, k[worst], vgs[worst])
 
 
+def test_split_bf16_contraction_matches_fp32_operands(gpu):
+    """The FP32-tier steps' variance contraction on split-BF16 operands
+    ('mixed': k_split_tri, k_kgen's planes, k_tri_bf) against the same
+    contraction on FP32 operands ('mixed_f32') and the batched FP64 path,
+    on the hourly workload's geometry (2 000 stations, 500 m cells, 96
+    steps, every model): estimates are identical (the contraction only
+    feeds the variance), the steps' flags are identical, and on every
+    FP32-tier step neither is flagged on, the variance stays within 1e-6 of
+    FP64 (relative, floor 1e-3 x the step's mean) and within 4x the FP32
+    operands' error (plus 1e-10).  The work counters show which kernel
+    ran."""
+    import torch
+    from spinterps_amd import _lib
+    from spinterps_amd.interp.steps import (DeviceProblem, interp_steps,
+                                            prof_enable, prof_get, prof_work)
+    w = configs.hourly(n_side=64, cell=500.0)
+    T = 96
+    vals, vgs = configs.hourly_steps(w, 0, T)
+    prob = DeviceProblem.from_workload(w)
+    out = {}
+    for prec in ('mixed', 'mixed_f32', 'fp64b'):
+        prof_enable(0, True)
+        e, v, info = interp_steps(prob, vals, vgs, 'OK', precision=prec)
+        torch.cuda.synchronize()
+        out[prec] = (e.cpu().numpy(), v.cpu().numpy(), info.cpu().numpy(),
+                     prof_get('gemm_tri_bf16')[1], prof_get('gemm_tri_f32')[1],
+                     prof_work('gemm_tri_bf16_macs'))
+        prof_enable(0, False)
+    eb, vb, ib, nb, nf, wb = out['mixed']
+    e3, v3, i3, nb3, nf3, _ = out['mixed_f32']
+    e64, v64, i64 = out['fp64b'][:3]
+    assert nb > 0 and nf == 0 and wb > 0, (nb, nf, wb)
+    assert nb3 == 0 and nf3 > 0, (nb3, nf3)
+    np.testing.assert_array_equal(ib, i3)
+    np.testing.assert_array_equal(eb, e3)
+    tier = (ib & (1 | 2 | 4 | 8 | 16 | 128 | _lib.ST_FP64VAR
+                  | _lib.ST_INT8VAR)) == 0
+    assert tier.sum() > 40, tier.sum()
+    floor = 1e-3 * np.abs(v64).mean(axis=1, keepdims=True)
+    with np.errstate(invalid='ignore'):
+        rb = (np.abs(vb - v64) / np.maximum(np.abs(v64), floor)).max(axis=1)
+        r3 = (np.abs(v3 - v64) / np.maximum(np.abs(v64), floor)).max(axis=1)
+    print(f'split-BF16 var max {rb[tier].max():.2e}, FP32 operands '
+          f'{r3[tier].max():.2e} over {tier.sum()} steps')
+    assert rb[tier].max() < 1e-6 and r3[tier].max() < 1e-6
+    bad = np.where(tier & (rb > 4 * r3 + 1e-10))[0]
+    assert bad.size == 0, [(int(t), rb[t], r3[t], vgs[t]) for t in bad]
+
+
 def test_mixed_slot_overflow_promotes(gpu):
     """On a coarse grid a 256-cell block has far more than 16 distinct
     nearest stations, so not every cell gets a staged local fit: the
