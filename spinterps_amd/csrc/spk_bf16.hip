@@ -106,14 +106,11 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
     double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nzb,
     int64_t nzk, unsigned long long *__restrict__ nexec) {
   // blocks on the same XCD (linear id mod 8) take consecutive row blocks
-  // of one cell tile, the longest (last) row block first: the tile's row
-  // blocks then start together and stream delta's chunks in step, so that
-  // the tile's delta is read once into that XCD's L2
+  // of one cell tile, so that the tile's delta stays in that XCD's L2
   const int G = gridDim.x * gridDim.y;
   int lin = blockIdx.x + blockIdx.y * gridDim.x;
   if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
-  const int rbi = (int)gridDim.x - 1 - lin % (int)gridDim.x;
-  const int cti = lin / gridDim.x;
+  const int rbi = lin % gridDim.x, cti = lin / gridDim.x;
   const int sl = blockIdx.z;
   const int status = sc[sl].status, flags = sc[sl].flags, n = sc[sl].n;
   if (status || (flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR))) return;
@@ -123,10 +120,19 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
   if (r0 >= n) return;
   const int kend = min(n, r0 + 64);
   const int64_t KO = ld / 8;
-  const uint4 *Ls = Lb + (int64_t)sl * 3 * KO * ld + r0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const uint4 *Ds = Db + (int64_t)sl * 3 * KO * ldk + c0 + wave * 32 + li;
+  // thread tid stages octet tid / 64 of row tid % 64 of each A plane
+  const int so = tid >> 6, sr = tid & 63;
+  // per-thread bases of the operand loads; chunk q adds q times the
+  // chunk stride, plane p p times the plane stride
+  const uint4 *La = Lb + (int64_t)sl * 3 * KO * ld +
+                    (AF ? 2 * ((int64_t)so * ld + r0 + sr)
+                        : (int64_t)so * ld + r0 + sr);
+  const int64_t a_q = AF ? 8 * ld : 4 * ld, a_p = KO * ld;
+  const uint4 *Dl = Db + (int64_t)sl * 3 * KO * ldk + (int64_t)lg * ldk + c0 +
+                    wave * 32 + li;
+  const int64_t b_q = 4 * ldk, b_p = KO * ldk;
 
   // A planes of one chunk: 64 rows x 4 octets, rows 80 bytes apart so that
   // the 16 lanes of a 128-bit LDS access hit distinct banks
@@ -159,35 +165,27 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[ti][tj][r] = 0.0;
 
-  // thread tid stages octet tid / 64 of row tid % 64 of each A plane
-  const int so = tid >> 6, sr = tid & 63;
   constexpr int NA = AF ? 2 : 3;  // 16-byte loads per thread and chunk
   auto load_a = [&](uint4 (&ra)[NA], int e) {
-    const int q = e >> 2;
+    const uint4 *src = La + (e >> 2) * a_q;
     if constexpr (AF) {
-      const uint4 *src = Lb + (int64_t)sl * 3 * KO * ld +
-                         2 * ((q * 4 + so) * ld + r0 + sr);
       ra[0] = src[0];
       ra[1] = src[1];
     } else {
 #pragma unroll
-      for (int p = 0; p < 3; ++p) ra[p] = Ls[(p * KO + q * 4 + so) * ld + sr];
+      for (int p = 0; p < 3; ++p) ra[p] = src[p * a_p];
     }
   };
   auto store_a = [&](const uint4 (&ra)[NA], int buf) {
     if constexpr (AF) {
+      // two values per conversion: the packed words are the planes'
       const uint32_t f[8] = {ra[0].x, ra[0].y, ra[0].z, ra[0].w,
                              ra[1].x, ra[1].y, ra[1].z, ra[1].w};
       uint32_t hw[4], mw[4], lw[4];
 #pragma unroll
-      for (int u = 0; u < 8; u += 2) {
-        uint16_t h0, m0, l0, h1, m1, l1;
-        split3f(__uint_as_float(f[u]), h0, m0, l0);
-        split3f(__uint_as_float(f[u + 1]), h1, m1, l1);
-        hw[u / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-        mw[u / 2] = (uint32_t)m0 | ((uint32_t)m1 << 16);
-        lw[u / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-      }
+      for (int u = 0; u < 4; ++u)
+        split3f2(__uint_as_float(f[2 * u]), __uint_as_float(f[2 * u + 1]),
+                 hw[u], mw[u], lw[u]);
       As[buf][0][sr][so] = uint4{hw[0], hw[1], hw[2], hw[3]};
       As[buf][1][sr][so] = uint4{mw[0], mw[1], mw[2], mw[3]};
       As[buf][2][sr][so] = uint4{lw[0], lw[1], lw[2], lw[3]};
@@ -196,16 +194,21 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
       for (int p = 0; p < 3; ++p) As[buf][p][sr][so] = ra[p];
     }
   };
-  // lane group lg holds octet lg of the chunk: half lg / 2
+  // lane group lg holds octet lg of the chunk: half lg / 2, taken as zero
+  // (not read) when unmarked
   auto load_b = [&](uint4 (&rb)[3][2], int e) {
-    const int q = e >> 2;
+    const uint4 *src = Dl + (e >> 2) * b_q;
     const bool on = (e >> (lg >> 1)) & 1;
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int tj = 0; tj < 2; ++tj)
-        rb[p][tj] = on ? Ds[(p * KO + q * 4 + lg) * ldk + tj * 16]
-                       : uint4{0u, 0u, 0u, 0u};
+      for (int tj = 0; tj < 2; ++tj) rb[p][tj] = uint4{0u, 0u, 0u, 0u};
+    if (on) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) rb[p][tj] = src[p * b_p + tj * 16];
+    }
   };
   auto mma = [&](int buf, const uint4 (&rb)[3][2]) {
     bf8_t b[3][2];

@@ -144,6 +144,22 @@ __device__ inline void split3f(float x, uint16_t &h, uint16_t &m,
   m = bf16_bits(r);
   l = bf16_bits(r - bf16_float(m));
 }
+// two values at once (one packed conversion per term): the low halves of
+// h, m, l hold x0's terms, the high halves x1's
+__device__ inline void split3f2(float x0, float x1, uint32_t &h, uint32_t &m,
+                                uint32_t &l) {
+  typedef float f2_t __attribute__((ext_vector_type(2)));
+  typedef __bf16 b2_t __attribute__((ext_vector_type(2)));
+  auto cvt = [](float a, float b) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_convertvector(f2_t{a, b}, b2_t));
+  };
+  h = cvt(x0, x1);
+  const float r0 = x0 - __uint_as_float(h << 16);
+  const float r1 = x1 - __uint_as_float(h & 0xffff0000u);
+  m = cvt(r0, r1);
+  l = cvt(r0 - __uint_as_float(m << 16), r1 - __uint_as_float(m & 0xffff0000u));
+}
 // from FP64: the remainders are taken in FP64
 __device__ inline void split3d(double x, uint16_t &h, uint16_t &m,
                                uint16_t &l) {
