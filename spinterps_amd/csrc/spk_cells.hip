@@ -2092,7 +2092,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           }
           {
             ProfScope ps(ctx, "gemm_tri_i8");
-            const dim3 g8((unsigned)ceil_div(N, 128), (unsigned)(ldk / 64),
+            const dim3 g8((unsigned)ceil_div(N, 64), (unsigned)(ldk / 128),
                           Sb);
             unsigned long long *w8 =
                 prof_of(ctx).on ? prof_of(ctx).work + 1 : nullptr;

@@ -223,22 +223,27 @@ template __global__ void k_slice_cols<double>(
     const double *, int8_t *, int64_t, int *, const uint8_t *, int64_t,
     uint8_t *);
 
-// q_part = |rows r0..r0+127 of L^-1 delta|^2 per cell from the first S
-// planes, diagonals s + t <= P.  Block tile 128 rows x 64 cells, k in
+// q_part = |rows r0..r0+63 of L^-1 delta|^2 per cell from the first S
+// planes, diagonals s + t <= P.  Block tile 64 rows x 128 cells, k in
 // chunks of 64 (one MFMA depth) up to the tile's last row; 8 waves in
-// 4 x 2, each 32 rows x 32 cells = 2 x 2 MFMA tiles per diagonal, the
+// 2 x 4, each 32 rows x 32 cells = 2 x 2 MFMA tiles per diagonal, the
 // P + 1 diagonal sums held in int32 for the whole k range.  The wave tile
 // keeps a wave near 200 registers, so two waves share each SIMD and one
-// wave's loads, barriers and epilogue overlap the other's products.  Each
-// chunk's planes are loaded into registers while the previous chunk is
+// wave's loads, barriers and epilogue overlap the other's products.  A
+// step's L^-1 planes do not fit in an XCD's L2 and are read again for
+// every cell tile, so the tile is wide in cells (128: half the reads of
+// a 64-cell tile) and short in rows; the cell tile's delta planes stay in
+// L2 while its row blocks run on that XCD (the XCD remap of k_tri64).
+// Each chunk's planes are loaded into registers while the previous chunk is
 // multiplied out of one of two LDS buffers and then stored into the other,
 // one barrier per chunk (64-byte rows with their 16-byte segments swizzled
 // by row, so that both the 16-row reads and the 4-row stores of a 16-lane
-// group hit distinct banks).  The two 32-row halves of a 64-row partial are
-// added through LDS in a fixed order.  XCD remap and the 64-row partial
-// layout are those of k_tri64.  With the digit map nz8 of k_slice_cols only
-// the 64-k chunks whose delta planes hold a nonzero digit in the block's 64
-// cells are multiplied (the others add exactly nothing); nexec, when given,
+// group hit distinct banks).  The two 32-row halves of the 64-row partial
+// are added through LDS in a fixed order.  With the digit map nz8 of
+// k_slice_cols (per 64 cells) only the 64-k chunks whose delta planes hold
+// a nonzero digit in one of the block's two 64-cell halves are multiplied,
+// and a half without one is taken as zero, not read (its planes may be
+// unwritten); the skipped products are exactly zero.  nexec, when given,
 // counts the plane products executed.
 template <int S, int P>
 __global__ __launch_bounds__(512) void k_tri8(
@@ -247,15 +252,16 @@ __global__ __launch_bounds__(512) void k_tri8(
     int64_t KS, const StepSc *__restrict__ sc, int64_t Cs,
     double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nz8,
     unsigned long long *nexec) {
-  constexpr int BM = 128, BN = 64, BK = 64, RS = 64, NT = 512;
+  constexpr int BM = 64, BN = 128, BK = 64, RS = 64, NT = 512;
   // 16-byte segment seg of LDS row r is kept at seg ^ ((r >> 2) & 3): the
   // 16-row reads and the 4-row stores of a 16-lane group then touch 64
   // distinct banks
   auto swz = [](int r, int seg) {
     return r * RS + ((seg ^ ((r >> 2) & 3)) << 4);
   };
-  // B chunk: S planes x 64 cells x 4 x 16 B = 256 S loads over NT threads
-  constexpr int NBQ = (S * 256 + NT - 1) / NT;
+  // A chunk: S planes x 64 rows x 4 x 16 B; B chunk: S planes x 128 cells
+  // x 4 x 16 B; NAQ / NBQ 16-byte loads per thread
+  constexpr int NAQ = (S * 256 + NT - 1) / NT, NBQ = (S * 512) / NT;
   static_assert(S <= kPlanes && P <= 2 * (S - 1), "shape");
   const int G = gridDim.x * gridDim.y;
   int lin = blockIdx.x + blockIdx.y * gridDim.x;
@@ -275,39 +281,47 @@ __global__ __launch_bounds__(512) void k_tri8(
   // two LDS buffers: chunk ch + 1 is stored while chunk ch is multiplied
   __shared__ __align__(16) int8_t As[2][S][BM * RS];
   __shared__ __align__(16) int8_t Bs[2][S][BN * RS];
-  __shared__ int s_list[kMaxChunks8];  // nonzero 64-k chunks (with a map)
+  __shared__ int s_list[kMaxChunks8];  // chunk << 2 | halves with a digit
   __shared__ int s_cnt[NT / 64];
-  __shared__ double s_half[2][2][2][16];  // [row pair][wc][tj][column]
+  __shared__ double s_half[4][2][16];  // [wc][tj][column]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int wr = wave >> 1, wc = wave & 1;
-  // staging: A chunk = 128 rows x 64 B per plane (one 16 B load per thread
-  // and plane), B chunk = 64 cells x 64 B per plane
-  i4_t ra[S], rb[NBQ];
-  auto load = [&](int k0) {
-    const int row = tid >> 2, seg = tid & 3;
+  const int wr = wave >> 2, wc = wave & 3;
+  i4_t ra[NAQ], rb[NBQ];
+  auto load = [&](int e) {
+    const int k0 = (e >> 2) * BK;
 #pragma unroll
-    for (int s = 0; s < S; ++s)
-      ra[s] = *reinterpret_cast<const i4_t *>(
-          A + (int64_t)s * R * KS + (int64_t)(r0 + row) * KS + k0 + seg * 16);
+    for (int q = 0; q < NAQ; ++q) {
+      const int i = tid + q * NT, s = i >> 8, row = (i & 255) >> 2,
+                seg = i & 3;
+      if (s < S)
+        ra[q] = *reinterpret_cast<const i4_t *>(
+            A + (int64_t)s * R * KS + (int64_t)(r0 + row) * KS + k0 +
+            seg * 16);
+    }
 #pragma unroll
     for (int q = 0; q < NBQ; ++q) {
-      const int e = tid + q * NT, t = e >> 8, cell = (e & 255) >> 2;
-      if (t < S)
+      const int i = tid + q * NT, t = i >> 9, cell = (i & 511) >> 2,
+                seg = i & 3;
+      if ((e >> (cell >> 6)) & 1)
         rb[q] = *reinterpret_cast<const i4_t *>(
             B + (int64_t)t * ldk * KS + (c0 + cell) * KS + k0 + seg * 16);
+      else
+        rb[q] = i4_t{0, 0, 0, 0};
     }
   };
   auto store = [&](int buf) {
-    const int row = tid >> 2, seg = tid & 3;
 #pragma unroll
-    for (int s = 0; s < S; ++s)
-      *reinterpret_cast<i4_t *>(&As[buf][s][swz(row, seg)]) = ra[s];
+    for (int q = 0; q < NAQ; ++q) {
+      const int i = tid + q * NT, s = i >> 8, row = (i & 255) >> 2,
+                seg = i & 3;
+      if (s < S) *reinterpret_cast<i4_t *>(&As[buf][s][swz(row, seg)]) = ra[q];
+    }
 #pragma unroll
     for (int q = 0; q < NBQ; ++q) {
-      const int e = tid + q * NT, t = e >> 8, cell = (e & 255) >> 2;
-      if (t < S)
-        *reinterpret_cast<i4_t *>(&Bs[buf][t][swz(cell, seg)]) = rb[q];
+      const int i = tid + q * NT, t = i >> 9, cell = (i & 511) >> 2,
+                seg = i & 3;
+      *reinterpret_cast<i4_t *>(&Bs[buf][t][swz(cell, seg)]) = rb[q];
     }
   };
   i4_t acc[P + 1][2][2];
@@ -317,27 +331,28 @@ __global__ __launch_bounds__(512) void k_tri8(
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[d][a][b] = i4_t{0, 0, 0, 0};
-  int nch = (kend + BK - 1) / BK;
-  // the list holds up to kMaxChunks8 chunks (nch <= KS / 64)
+  // the list holds up to kMaxChunks8 chunks: the library sends only steps
+  // with at most 20 000 stations (313 chunks) to the int8 planes
+  const int nkc = min((kend + BK - 1) / BK, kMaxChunks8);
   const bool skip = nz8 != nullptr && (KS >> 6) <= kMaxChunks8;
-  const uint8_t *nzd = skip ? nz8 + ((int64_t)sl * (ldk >> 6) + cti) * (KS >> 6)
-                            : nullptr;
-  if (skip) {
-    int tot = 0;
-    for (int b0 = 0; b0 < nch; b0 += NT) {
-      const int ch = b0 + tid;
-      const bool f = ch < nch && nzd[ch] != 0;
-      const unsigned long long bm = __ballot(f);
-      if (lane == 0) s_cnt[wave] = __popcll(bm);
-      __syncthreads();
-      int off = tot;
-      for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
-      if (f) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] = ch;
+  const uint8_t *nzd =
+      skip ? nz8 + ((int64_t)sl * (ldk >> 6) + 2 * cti) * (KS >> 6) : nullptr;
+  int nch = 0;
+  for (int b0 = 0; b0 < nkc; b0 += NT) {
+    const int ch = b0 + tid;
+    int mk = 0;
+    if (ch < nkc)
+      mk = skip ? (nzd[ch] ? 1 : 0) | (nzd[(KS >> 6) + ch] ? 2 : 0) : 3;
+    const unsigned long long bm = __ballot(mk != 0);
+    if (lane == 0) s_cnt[wave] = __popcll(bm);
+    __syncthreads();
+    int off = nch;
+    for (int w2 = 0; w2 < wave; ++w2) off += s_cnt[w2];
+    if (mk) s_list[off + __popcll(bm & ((1ull << lane) - 1ull))] =
+        (ch << 2) | mk;
 #pragma unroll
-      for (int w2 = 0; w2 < NT / 64; ++w2) tot += s_cnt[w2];
-      __syncthreads();
-    }
-    nch = tot;
+    for (int w2 = 0; w2 < NT / 64; ++w2) nch += s_cnt[w2];
+    __syncthreads();
   }
   if (nexec && tid == 0) {
     int prods = 0;
@@ -345,15 +360,14 @@ __global__ __launch_bounds__(512) void k_tri8(
       for (int t = 0; t < S; ++t) prods += s + t <= P;
     atomicAdd(nexec, (unsigned long long)nch * prods);
   }
-  auto kchunk = [&](int ch) { return skip ? s_list[ch] * BK : ch * BK; };
   if (nch > 0) {
-    load(kchunk(0));
+    load(s_list[0]);
     store(0);
   }
   __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
     const int buf = ch & 1;
-    if (ch + 1 < nch) load(kchunk(ch + 1));
+    if (ch + 1 < nch) load(s_list[ch + 1]);
     i4_t af[S][2];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -409,21 +423,20 @@ __global__ __launch_bounds__(512) void k_tri8(
     part[tj] += __shfl_xor(part[tj], 16, 64);
     part[tj] += __shfl_xor(part[tj], 32, 64);
   }
-  // rows 32-63 of each 64-row partial go through LDS to the wave holding
-  // rows 0-31 of it, which adds them to its own
-  if ((wr & 1) && lg == 0) {
+  // rows 32-63 of the 64-row partial go through LDS to the wave holding
+  // rows 0-31, which adds them to its own
+  if (wr == 1 && lg == 0) {
 #pragma unroll
-    for (int tj = 0; tj < 2; ++tj) s_half[wr >> 1][wc][tj][li] = part[tj];
+    for (int tj = 0; tj < 2; ++tj) s_half[wc][tj][li] = part[tj];
   }
   __syncthreads();
-  const int prow = 2 * rti + (wr >> 1);
-  if ((wr & 1) || lg != 0 || prow >= nrb) return;
+  if (wr != 0 || lg != 0 || rti >= nrb) return;
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj) {
     const int64_t col = c0 + wc * 32 + tj * 16 + li;
-    const double v = part[tj] + s_half[wr >> 1][wc][tj][li];
+    const double v = part[tj] + s_half[wc][tj][li];
     if (col < Cs)
-      qpart[((int64_t)sl * nrb + prow) * ldk + col] =
+      qpart[((int64_t)sl * nrb + rti) * ldk + col] =
           ldexp(v, 2 * eb[(int64_t)sl * ldk + col]);
   }
 }
