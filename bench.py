@@ -101,6 +101,14 @@ def main():
                     help='mixed: the well-conditioned steps\' variance '
                          'contraction on split-BF16 operands; mixed_f32: '
                          'on FP32 operands')
+    ap.add_argument('--fp32-limit', type=float, default=0.0,
+                    help='condition estimate above which a step\'s variance '
+                         'leaves the split-BF16 / FP32 contraction for the '
+                         'int8 planes, for every model (0: the library\'s '
+                         'policy, 1e5 and 1e3 with a Gaussian component); '
+                         'with --models Sph,Exp and 1000 a counter run gets '
+                         'both contractions without Gaussian steps, whose '
+                         'pseudo-inverse fallback calls rocBLAS')
     ap.add_argument('--calib', action='store_true',
                     help='also run k_clip_cast over 2^28 float64 values '
                          '(2 GiB read, 1 GiB written) as the reference '
@@ -171,7 +179,7 @@ def main():
         ev0.record()
         _, _, info = interp_steps(prob, vals_d[:m], None, 'OK',
                                   precision=args.precision, est=est[:m],
-                                  var=var[:m],
+                                  var=var[:m], fp64var_kappa=args.fp32_limit,
                                   vg_dev=vg_dd[:m * vrec], kappa=kap)
         ev1.record()
         clip_cast(est[:m], out=e32[:m])
@@ -509,6 +517,7 @@ def main():
                                '500 m, per-step variogram and 200-2000 '
                                'active stations, OK estimate + variance',
                    'models': args.models, 'precision': args.precision,
+                   'fp32_limit': args.fp32_limit or 'library policy',
                    'time_steps_per_bench_step': B, 'n_cells': C,
                    'n_stations': w.n_stns},
         'timed_region': 'H2D of the steps\' values and variogram '

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_split_tri(
 
 // q partials of the FP32-tier steps: qpart[(s nrb + rb) ldk + c] =
 // sum over the rows of row block rb of (L^-1 delta)[r, c]^2.
-// Grid (row blocks of 64, cell tiles of 128, steps), 256 threads; wave w
+// Grid (row blocks of 64, cell tiles of 128, the steps of lst), 256
+// threads; wave w
 // takes cells 32 w .. 32 w + 31 of the tile (two 16-column MFMA tiles) and
 // all 64 rows (four).  Lb: planes of L^-1 (k_split_tri); Db: planes of
 // delta, octet o of cell c of plane p at Db[s][(p KO + o) ldk + c].
@@ -102,16 +103,16 @@ __global__ __launch_bounds__(256) void k_split_tri(
 template <int PF, int AF>
 __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
     const uint4 *__restrict__ Lb, int64_t ld, const uint4 *__restrict__ Db,
-    int64_t ldk, const StepSc *__restrict__ sc, int64_t Cs,
+    int64_t ldk, const StepSc *__restrict__ sc, StepList lst, int64_t Cs,
     double *__restrict__ qpart, int nrb, const uint8_t *__restrict__ nzb,
     int64_t nzk, unsigned long long *__restrict__ nexec) {
   // blocks on the same XCD (linear id mod 8) take consecutive row blocks
   // of one cell tile, so that the tile's delta stays in that XCD's L2
   const int G = gridDim.x * gridDim.y;
   int lin = blockIdx.x + blockIdx.y * gridDim.x;
-  if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
+  lin = xcd_remap(lin, G);
   const int rbi = lin % gridDim.x, cti = lin / gridDim.x;
-  const int sl = blockIdx.z;
+  const int sl = lst.s[blockIdx.z];
   const int status = sc[sl].status, flags = sc[sl].flags, n = sc[sl].n;
   if (status || (flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR))) return;
   const int64_t c0 = cti * (int64_t)128;
@@ -122,8 +123,9 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
   const int64_t KO = ld / 8;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  // thread tid stages octet tid / 64 of row tid % 64 of each A plane
-  const int so = tid >> 6, sr = tid & 63;
+  // thread tid stages octet (tid / 16) % 4 of row tid % 16 + 16 (tid / 64)
+  // of each A plane: its LDS store has the pattern of the reads below
+  const int so = (tid >> 4) & 3, sr = (tid & 15) + 16 * (tid >> 6);
   // per-thread bases of the operand loads; chunk q adds q times the
   // chunk stride, plane p p times the plane stride
   const uint4 *La = Lb + (int64_t)sl * 3 * KO * ld +
@@ -134,9 +136,11 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
                     wave * 32 + li;
   const int64_t b_q = 4 * ldk, b_p = KO * ldk;
 
-  // A planes of one chunk: 64 rows x 4 octets, rows 80 bytes apart so that
-  // the 16 lanes of a 128-bit LDS access hit distinct banks
-  __shared__ uint4 As[2][3][64][5];
+  // A planes of one chunk: 64 rows x 4 octets, rows 96 bytes apart: the
+  // 16-byte accesses of 16 rows by the four lane groups (one octet each)
+  // then run at the conflict-free rate (measured on gfx950: rows 64 or
+  // 80 bytes apart are 1.7x slower)
+  __shared__ uint4 As[2][3][64][6];
   __shared__ int s_list[256];
   __shared__ int s_cnt[4];
   // chunk list: entry (q << 2) | (half 0 marked) | (half 1 marked) << 1
@@ -296,17 +300,17 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_tri_bf(
 }
 
 template __global__ void k_tri_bf<0, 0>(
-    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, int64_t,
-    double *, int, const uint8_t *, int64_t, unsigned long long *);
+    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, StepList,
+    int64_t, double *, int, const uint8_t *, int64_t, unsigned long long *);
 template __global__ void k_tri_bf<0, 1>(
-    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, int64_t,
-    double *, int, const uint8_t *, int64_t, unsigned long long *);
+    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, StepList,
+    int64_t, double *, int, const uint8_t *, int64_t, unsigned long long *);
 template __global__ void k_tri_bf<1, 0>(
-    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, int64_t,
-    double *, int, const uint8_t *, int64_t, unsigned long long *);
+    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, StepList,
+    int64_t, double *, int, const uint8_t *, int64_t, unsigned long long *);
 template __global__ void k_tri_bf<1, 1>(
-    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, int64_t,
-    double *, int, const uint8_t *, int64_t, unsigned long long *);
+    const uint4 *, int64_t, const uint4 *, int64_t, const StepSc *, StepList,
+    int64_t, double *, int, const uint8_t *, int64_t, unsigned long long *);
 template __global__ void k_split_tri<0>(const double *, int64_t, int64_t,
                                         const StepSc *, uint4 *);
 template __global__ void k_split_tri<1>(const double *, int64_t, int64_t,

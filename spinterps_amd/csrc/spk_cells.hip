@@ -781,7 +781,9 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
   const double *dal = w + ld;   // refinement correction of alpha (k_dual)
   __shared__ double stx[kChunk], sty[kChunk], sal[kChunk], sw[kChunk];
   __shared__ double sdal[kChunk];
-  __shared__ double sG[kSlots][kChunk];
+  // one double of padding per slot row: lanes reading different slots at
+  // the same station hit different banks
+  __shared__ double sG[kSlots][kChunk + 1];
   __shared__ int slot_j[kSlots];
   __shared__ int n_slots;
   __shared__ VgFast vf;
@@ -1695,7 +1697,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                                      ? strideM * (bfop ? 6 : 4)
                                      : 0));
   const int64_t budget = (int64_t)20 << 30;
-  int64_t S = std::max<int64_t>(1, std::min<int64_t>(budget / per_step, 64));
+  int64_t S = std::max<int64_t>(
+      1, std::min<int64_t>(budget / per_step, kMaxStepsPerBatch));
   S = std::min<int64_t>(S, T);
   const int nsets = T > S ? 2 : 1;
   struct Set {
@@ -1709,6 +1712,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     // steps of the batch whose Cholesky failed: solved by the
     // pseudo-inverse fallback (spk_pinv.hip) and the plain contraction
     std::vector<int> pinv;
+    // the steps each variance contraction takes (split-BF16 or FP32
+    // operands, int8 with 5 and with 6 planes) and the largest n of each
+    StepList lst32, lst8a, lst8b;
+    int n32, n8a, n8b;
   } set[2] = {};
   // pinned host copy of a batch's step records, read after its Cholesky
   {
@@ -1973,6 +1980,26 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, sf,
                              &q.sc[s2].status, 1, SPK_ST_FAILED);
         }
+      // the contraction each step takes, from its flags (k_nearest may
+      // still promote a step to FP64 during the contraction; the kernels
+      // check the flags again)
+      q.lst32.cnt = q.lst8a.cnt = q.lst8b.cnt = 0;
+      q.n32 = q.n8a = q.n8b = 0;
+      for (int s2 = 0; s2 < Sb; ++s2) {
+        const StepSc &h = h_sc[s2];
+        if (h.status != 0 || !(h.kappa <= kPinvKappa) ||
+            (h.flags & SPK_ST_FP64VAR))
+          continue;
+        if (h.flags & SPK_ST_INT8VAR) {
+          StepList &l = h.planes == 5 ? q.lst8a : q.lst8b;
+          int &nm = h.planes == 5 ? q.n8a : q.n8b;
+          l.s[l.cnt++] = s2;
+          nm = std::max(nm, h.n);
+        } else {
+          q.lst32.s[q.lst32.cnt++] = s2;
+          q.n32 = std::max(q.n32, h.n);
+        }
+      }
       if (q.L8) {
         ProfScope ps(ctx, "slice_rows", sf);
         hipLaunchKernelGGL(k_slice_rows, dim3((unsigned)(R8 / 64), Sb),
@@ -2092,31 +2119,40 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           }
           {
             ProfScope ps(ctx, "gemm_tri_i8");
-            const dim3 g8((unsigned)ceil_div(N, 64), (unsigned)(ldk / 128),
-                          Sb);
             unsigned long long *w8 =
                 prof_of(ctx).on ? prof_of(ctx).work + 1 : nullptr;
-            hipLaunchKernelGGL((k_tri8<5, 4>), g8, dim3(512), 0, sm,
-                               (const int8_t *)q.L8, R8, (const int *)q.ea,
-                               (const int8_t *)B8, ldk, (const int *)eb8, KS8,
-                               q.sc, cn, qpart, nrb, (const uint8_t *)nz8,
-                               w8);
-            hipLaunchKernelGGL((k_tri8<6, 5>), g8, dim3(512), 0, sm,
-                               (const int8_t *)q.L8, R8, (const int *)q.ea,
-                               (const int8_t *)B8, ldk, (const int *)eb8, KS8,
-                               q.sc, cn, qpart, nrb, (const uint8_t *)nz8,
-                               w8);
+            if (q.lst8a.cnt > 0)
+              hipLaunchKernelGGL(
+                  (k_tri8<5, 4>),
+                  dim3((unsigned)ceil_div(q.n8a, 64), (unsigned)(ldk / 128),
+                       (unsigned)q.lst8a.cnt),
+                  dim3(512), 0, sm, (const int8_t *)q.L8, R8,
+                  (const int *)q.ea, (const int8_t *)B8, ldk,
+                  (const int *)eb8, KS8, q.sc, q.lst8a, cn, qpart, nrb,
+                  (const uint8_t *)nz8, w8);
+            if (q.lst8b.cnt > 0)
+              hipLaunchKernelGGL(
+                  (k_tri8<6, 5>),
+                  dim3((unsigned)ceil_div(q.n8b, 64), (unsigned)(ldk / 128),
+                       (unsigned)q.lst8b.cnt),
+                  dim3(512), 0, sm, (const int8_t *)q.L8, R8,
+                  (const int *)q.ea, (const int8_t *)B8, ldk,
+                  (const int *)eb8, KS8, q.sc, q.lst8b, cn, qpart, nrb,
+                  (const uint8_t *)nz8, w8);
           }
           SPK_TRY(check_launch("k_tri8"));
         }
         const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
         if (bfop) {
           ProfScope ps(ctx, "gemm_tri_bf16");
-          hipLaunchKernelGGL((k_tri_bf<0, kBfA32>), grid, dim3(256), 0, sm,
-                             (const uint4 *)q.Lb, ld, (const uint4 *)Kb, ldk,
-                             q.sc, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
-                             prof_of(ctx).on ? prof_of(ctx).work + 2
-                                             : nullptr);
+          if (q.lst32.cnt > 0)
+            hipLaunchKernelGGL(
+                (k_tri_bf<0, kBfA32>),
+                dim3((unsigned)ceil_div(q.n32, 64), (unsigned)ceil_div(cn, 128),
+                     (unsigned)q.lst32.cnt),
+                dim3(256), 0, sm, (const uint4 *)q.Lb, ld, (const uint4 *)Kb,
+                ldk, q.sc, q.lst32, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
+                prof_of(ctx).on ? prof_of(ctx).work + 2 : nullptr);
         } else if (mixed) {
           ProfScope ps(ctx, "gemm_tri_f32");
           // k chunks of 16: FP32 sums of 32 products, 3 waves per SIMD

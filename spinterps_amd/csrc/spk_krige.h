@@ -13,6 +13,25 @@ constexpr int kMaxP = 8;  // constraint rows: 1 + up to 7 drifts
 // the refinement's residuals and their images.
 constexpr int kT0Cols = 2 * (kMaxP + 1) + 6;
 
+// Logical block id of linear block id lin in a grid of G blocks, so that
+// the blocks dispatched to one XCD (lin mod 8: the hardware deals blocks
+// to the eight XCDs in turn) take a contiguous range of logical ids.  XCD
+// x holds q + (x < r) of the blocks (q = G / 8, r = G mod 8); the map is a
+// bijection of [0, G) for any G.
+__device__ inline int xcd_remap(int lin, int G) {
+  const int x = lin & 7, q = G >> 3, r = G & 7;
+  return x * q + min(x, r) + (lin >> 3);
+}
+
+// The steps of a batch one contraction launch takes (blockIdx.z indexes
+// s[]), chosen on the host from the step records after the factorisation:
+// blocks of the other steps would only exit.
+constexpr int kMaxStepsPerBatch = 64;
+struct StepList {
+  int cnt;
+  int s[kMaxStepsPerBatch];
+};
+
 // Per-step scalars kept on the device.
 struct StepSc {
   int n;          // active stations
@@ -176,9 +195,10 @@ __global__ void k_split_tri(const double *Li, int64_t ld, int64_t stride,
                             const StepSc *sc, uint4 *Lb);
 template <int PF, int AF>
 __global__ void k_tri_bf(const uint4 *Lb, int64_t ld, const uint4 *Db,
-                         int64_t ldk, const StepSc *sc, int64_t Cs,
-                         double *qpart, int nrb, const uint8_t *nzb,
-                         int64_t nzk, unsigned long long *nexec);
+                         int64_t ldk, const StepSc *sc, StepList lst,
+                         int64_t Cs, double *qpart, int nrb,
+                         const uint8_t *nzb, int64_t nzk,
+                         unsigned long long *nexec);
 // exact int8 plane products (spk_int8.hip): up to 6 planes per operand;
 // steps up to condition estimate kInt8Deep use 5 planes and the plane
 // products up to the fourth diagonal, steps above it 6 planes and the
@@ -196,8 +216,8 @@ __global__ void k_slice_cols(const T *K, int64_t ld, int64_t ldk, int64_t Cs,
 template <int S, int P>
 __global__ void k_tri8(const int8_t *A8, int64_t R, const int *ea,
                        const int8_t *B8, int64_t ldk, const int *eb,
-                       int64_t KS, const StepSc *sc, int64_t Cs,
-                       double *qpart, int nrb, const uint8_t *nz8,
-                       unsigned long long *nexec);
+                       int64_t KS, const StepSc *sc, StepList lst,
+                       int64_t Cs, double *qpart, int nrb,
+                       const uint8_t *nz8, unsigned long long *nexec);
 
 }  // namespace spk
