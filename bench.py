@@ -30,17 +30,19 @@ sys.path.insert(0, str(ROOT))
 
 
 # HBM bytes per dispatch of the variance contractions, measured by
-# rocprofv3 --pmc on this workload (batches of 64 steps, 31 stripes per
-# batch on MI355X; FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with
-# k_clip_cast); the
-# int8 figure adds its two launches (5 and 6 planes).  The FP64 figure is
-# from a run in which k_tri64 contracted every step above the FP32 limit
+# rocprofv3 --pmc (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated with
+# k_clip_cast).  The split-BF16 and int8 figures come from the hourly
+# workload with Sph/Exp models and the FP32-tier limit at 1e3 (--models
+# Sph,Exp --fp32-limit 1000: both contractions, no pseudo-inverse steps,
+# whose rocBLAS calls end counter runs in a host fault of the profiler),
+# batches of 64 steps, 34 stripes per batch; the int8 figure is its 5-plane
+# launch (no step of that run needed six planes).  The FP64 figure is from
+# a run in which k_tri64 contracted every step above the FP32 limit
 # (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
 TRI32_TRAFFIC = 2 * 5403411 * 1024 + 202497 * 1024
-TRI8_TRAFFIC = 2 * (7126920 + 150551) * 1024 + (50293 + 17090) * 1024
-# the split-BF16 contraction: not yet measured by counters
-TRIBF_TRAFFIC = None
+TRI8_TRAFFIC = 2 * 14086536 * 1024 + 171559 * 1024
+TRIBF_TRAFFIC = 2 * 1639751 * 1024 + 116188 * 1024
 
 
 def log(*a):
@@ -475,7 +477,9 @@ def main():
     # the roofline object names the contraction kernel with the larger
     # share of the timed region
     note = ('HBM bytes per dispatch, FETCH_SIZE x2 + WRITE_SIZE x1 '
-            '(calibrated), from profiles/kernels_hourly_mixed_b64.md')
+            '(calibrated), from profiles/kernels_hourly_mixed_b64.md (the '
+            'split-BF16 and int8 figures on this workload with Sph/Exp '
+            'models and --fp32-limit 1000)')
     r64 = {'bound': 'mfma', 'achieved': ach64, 'peak': 78.6,
            'unit': 'TFLOP/s', 'frac': ach64 / 78.6,
            'traffic': TRI64_TRAFFIC,
