@@ -1947,7 +1947,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       if (h_sc[s2].status == SPK_ST_FAILED) q.pinv.push_back(s2);
     {
       ProfScope ps(ctx, "trtri", sf);
-      hipLaunchKernelGGL(k_trtri, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
+      const dim3 gt((unsigned)ceil_div(N, 64), Sb);
+      hipLaunchKernelGGL(k_trtri_diag, gt, dim3(256), 0, sf, q.A, q.Li, ld,
+                         strideM, q.sc);
+      hipLaunchKernelGGL(k_trtri_cols, gt, dim3(256), 0, sf, q.A, q.Li, ld,
                          strideM, q.sc);
     }
     SPK_TRY(check_launch("k_trtri"));

@@ -295,7 +295,11 @@ int spk_krige_class(spk_ctx *ctx, int kind, const char *model,
   // C = c0 - Gamma (lower triangle), Cholesky, triangular inverse
   hipLaunchKernelGGL(k_potrf, dim3(1), dim3(256), 0, st, A, ld, slab, sc, dvg,
                      1);
-  hipLaunchKernelGGL(k_trtri, dim3(1), dim3(256), 0, st, A, Li, ld, slab, sc);
+  {
+    const dim3 gt((unsigned)ceil_div(n, 64), 1);
+    hipLaunchKernelGGL(k_trtri_diag, gt, dim3(256), 0, st, A, Li, ld, slab, sc);
+    hipLaunchKernelGGL(k_trtri_cols, gt, dim3(256), 0, st, A, Li, ld, slab, sc);
+  }
   hipLaunchKernelGGL(k_minv, dim3(1), dim3(256), 0, st, Li, ld, slab, sc, idx,
                      (int64_t)n, dsi ? dsi + (int64_t)m * n : nullptr, m,
                      kind == 1 ? SPK_M_SK : (kind == 2 ? SPK_M_EDK : SPK_M_OK),

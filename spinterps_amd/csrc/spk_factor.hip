@@ -355,21 +355,67 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
 }
 
 // ---- triangular inverse ----------------------------------------------------------
-// X = L^-1 (lower, zeros above the diagonal) by block rows of 64:
-// X[r, c] = inv(L_rr) (delta_rc I - sum_{c <= k < r} L[r, k] X[k, c]).
-// The n x n region of X is fully written (upper part zero).
-__global__ __launch_bounds__(256) void k_trtri(const double *__restrict__ L,
-                                               double *__restrict__ X,
-                                               int64_t ld, int64_t stride,
-                                               const StepSc *__restrict__ sc) {
-  const int s = blockIdx.x;
+// X = L^-1 (lower, zeros above the diagonal) by 64 x 64 blocks: the diagonal
+// blocks are the inverses of L's, X[r, r] = inv(L_rr), and below them
+// X[r, c] = X[r, r] (- sum_{c <= k < r} L[r, k] X[k, c]).  Column block c is
+// a chain over r that needs only its own earlier blocks, so the column
+// blocks of every step run in parallel: k_trtri_diag inverts the diagonal
+// blocks (grid (blocks, steps)), then k_trtri_cols runs one column block's
+// chain per workgroup (grid (blocks, steps)).  The n x n region of X is
+// fully written (upper part zero).
+__global__ __launch_bounds__(256) void k_trtri_diag(
+    const double *__restrict__ L, double *__restrict__ X, int64_t ld,
+    int64_t stride, const StepSc *__restrict__ sc) {
+  const int s = blockIdx.y;
   if (sc[s].status) return;
   const int n = sc[s].n;
+  const int r0 = blockIdx.x * 64;
+  if (r0 >= n) return;
+  const int rn = min(64, n - r0);
+  const double *Ls = L + (int64_t)s * stride;
+  double *Xs = X + (int64_t)s * stride;
+  const int tid = threadIdx.x;
+  __shared__ double Di[64][65];
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    Di[i][j] = (i < rn && j < rn && i >= j)
+                   ? Ls[(r0 + i) + (int64_t)(r0 + j) * ld]
+                   : 0.0;
+  }
+  __syncthreads();
+  // in-place inverse, last column first:
+  // X_jj = 1/L_jj, X[i, j] = -X_jj sum_{j<k<=i} X[i, k] L[k, j]
+  for (int j = rn - 1; j >= 0; --j) {
+    double v = 0.0;
+    const double djj = 1.0 / Di[j][j];
+    if (tid > j && tid < rn)
+      for (int k = j + 1; k <= tid; ++k) v += Di[tid][k] * Di[k][j];
+    __syncthreads();
+    if (tid > j && tid < rn) Di[tid][j] = -v * djj;
+    if (tid == j) Di[j][j] = djj;
+    __syncthreads();
+  }
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    if (i < rn && j < rn)
+      Xs[(r0 + i) + (int64_t)(r0 + j) * ld] = (i >= j) ? Di[i][j] : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_trtri_cols(
+    const double *__restrict__ L, double *__restrict__ X, int64_t ld,
+    int64_t stride, const StepSc *__restrict__ sc) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int cb = blockIdx.x;
+  const int c0 = cb * 64;
+  if (c0 >= n) return;
   const double *Ls = L + (int64_t)s * stride;
   double *Xs = X + (int64_t)s * stride;
   const int tid = threadIdx.x;
   const int ty = tid >> 4, tx = tid & 15;
-  __shared__ double Di[64][65];  // L_rr, inverted in place
+  __shared__ double Di[64][65];  // X[r, r]
   __shared__ union {
     struct {
       double a[64][33];
@@ -378,116 +424,97 @@ __global__ __launch_bounds__(256) void k_trtri(const double *__restrict__ L,
     double acc[64][65];
   } sm;
   const int nb = (n + 63) / 64;
-  // zero the strictly-upper block tiles
-  for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
-    const int i = (int)(e % n), j = (int)(e / n);
-    if (i / 64 < j / 64) Xs[i + (int64_t)j * ld] = 0.0;
+  const int cn = min(64, n - c0);
+  // the strictly-upper blocks of this column block
+  for (int64_t e = tid; e < (int64_t)c0 * cn; e += 256) {
+    const int i = (int)(e % c0), j = (int)(e / c0);
+    Xs[i + (int64_t)(c0 + j) * ld] = 0.0;
   }
-  for (int rb = 0; rb < nb; ++rb) {
+  for (int rb = cb + 1; rb < nb; ++rb) {
     const int r0 = rb * 64;
     const int rn = min(64, n - r0);
     for (int e = tid; e < 64 * 64; e += 256) {
       const int i = e & 63, j = e >> 6;
-      Di[i][j] = (i < rn && j < rn && i >= j)
-                     ? Ls[(r0 + i) + (int64_t)(r0 + j) * ld]
-                     : 0.0;
+      Di[i][j] = (i < rn && j < rn) ? Xs[(r0 + i) + (int64_t)(r0 + j) * ld]
+                                    : 0.0;
     }
-    __syncthreads();
-    // in-place inverse, last column first:
-    // X_jj = 1/L_jj, X[i, j] = -X_jj sum_{j<k<=i} X[i, k] L[k, j]
-    for (int j = rn - 1; j >= 0; --j) {
-      double v = 0.0;
-      const double djj = 1.0 / Di[j][j];
-      if (tid > j && tid < rn)
-        for (int k = j + 1; k <= tid; ++k) v += Di[tid][k] * Di[k][j];
-      __syncthreads();
-      if (tid > j && tid < rn) Di[tid][j] = -v * djj;
-      if (tid == j) Di[j][j] = djj;
-      __syncthreads();
-    }
-    for (int cb = 0; cb <= rb; ++cb) {
-      const int c0 = cb * 64;
-      double acc[4][4];
+    double acc[4][4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-          acc[a][b] = (rb == cb && ty * 4 + a == tx * 4 + b) ? 1.0 : 0.0;
-      // acc -= L[r, k] X[k, c] for k in [c0, r0); the next chunk's tiles
-      // are loaded into registers while this one is multiplied
-      double pa_[8], pb_[8];
-      auto fetch = [&](int k0) {
+      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    // acc -= L[r, k] X[k, c] for k in [c0, r0); the next chunk's tiles
+    // are loaded into registers while this one is multiplied
+    double pa_[8], pb_[8];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int e = tid + q * 256;
-          const int r = e & 63, k = e >> 6;
-          pa_[q] = (r < rn) ? Ls[(r0 + r) + (int64_t)(k0 + k) * ld] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int e = tid + q * 256;
-          const int k = e & 31, c = e >> 5;
-          pb_[q] =
-              (c0 + c < n) ? Xs[(k0 + k) + (int64_t)(c0 + c) * ld] : 0.0;
-        }
-      };
-      if (c0 < r0) fetch(c0);
-      for (int k0 = c0; k0 < r0; k0 += 32) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int e = tid + q * 256;
-          sm.t.a[e & 63][e >> 6] = pa_[q];
-          sm.t.b[e & 31][e >> 5] = pb_[q];
-        }
-        __syncthreads();
-        if (k0 + 32 < r0) fetch(k0 + 32);
-        for (int k = 0; k < 32; ++k) {
-          double pa[4], pb[4];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) pa[a] = sm.t.a[ty * 4 + a][k];
-#pragma unroll
-          for (int b = 0; b < 4; ++b) pb[b] = sm.t.b[k][tx * 4 + b];
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] -= pa[a] * pb[b];
-        }
-        __syncthreads();
+      for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 256;
+        const int r = e & 63, k = e >> 6;
+        pa_[q] = (r < rn) ? Ls[(r0 + r) + (int64_t)(k0 + k) * ld] : 0.0;
       }
-      // X[r, c] = inv(L_rr) acc
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 256;
+        const int k = e & 31, c = e >> 5;
+        pb_[q] = (c0 + c < n) ? Xs[(k0 + k) + (int64_t)(c0 + c) * ld] : 0.0;
+      }
+    };
+    fetch(c0);
+    for (int k0 = c0; k0 < r0; k0 += 32) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) sm.acc[ty * 4 + a][tx * 4 + b] = acc[a][b];
+      for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 256;
+        sm.t.a[e & 63][e >> 6] = pa_[q];
+        sm.t.b[e & 31][e >> 5] = pb_[q];
+      }
       __syncthreads();
-      double out[4][4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
-      for (int k = 0; k < 64; ++k) {
+      if (k0 + 32 < r0) fetch(k0 + 32);
+      for (int k = 0; k < 32; ++k) {
         double pa[4], pb[4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) pa[a] = Di[ty * 4 + a][k];
+        for (int a = 0; a < 4; ++a) pa[a] = sm.t.a[ty * 4 + a][k];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) pb[b] = sm.acc[k][tx * 4 + b];
+        for (int b = 0; b < 4; ++b) pb[b] = sm.t.b[k][tx * 4 + b];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-          for (int b = 0; b < 4; ++b) out[a][b] += pa[a] * pb[b];
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int i = r0 + ty * 4 + a;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int j = c0 + tx * 4 + b;
-          if (i < n && j < n)
-            Xs[i + (int64_t)j * ld] = (i >= j) ? out[a][b] : 0.0;
-        }
+          for (int b = 0; b < 4; ++b) acc[a][b] -= pa[a] * pb[b];
       }
       __syncthreads();
     }
+    // X[r, c] = X[r, r] acc
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) sm.acc[ty * 4 + a][tx * 4 + b] = acc[a][b];
+    __syncthreads();
+    double out[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
+    for (int k = 0; k < 64; ++k) {
+      double pa[4], pb[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) pa[a] = Di[ty * 4 + a][k];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) pb[b] = sm.acc[k][tx * 4 + b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) out[a][b] += pa[a] * pb[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = r0 + ty * 4 + a;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = c0 + tx * 4 + b;
+        if (i < n && j < n) Xs[i + (int64_t)j * ld] = out[a][b];
+      }
+    }
+    __syncthreads();
   }
 }
 

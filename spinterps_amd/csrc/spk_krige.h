@@ -119,8 +119,12 @@ __global__ void k_assemble(const double *sx, const double *sy,
                            double *Gfull);
 __global__ void k_potrf(double *A, int64_t ld, int64_t strideA, StepSc *sc,
                         const spk_vg *vgs, int shift);
-__global__ void k_trtri(const double *L, double *X, int64_t ld, int64_t stride,
-                        const StepSc *sc);
+// X = L^-1: k_trtri_diag then k_trtri_cols, each with grid
+// (ceil(n_max / 64), steps)
+__global__ void k_trtri_diag(const double *L, double *X, int64_t ld,
+                             int64_t stride, const StepSc *sc);
+__global__ void k_trtri_cols(const double *L, double *X, int64_t ld,
+                             int64_t stride, const StepSc *sc);
 __global__ void k_dual(const double *Lc, const double *Li, int64_t ld,
                        int64_t stride,
                        StepSc *sc, const double *zact, int64_t ldz,
