@@ -1933,8 +1933,23 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(check_launch("k_assemble"));
     {
       ProfScope ps(ctx, "potrf", sf);
-      hipLaunchKernelGGL(k_potrf, dim3(Sb), dim3(256), 0, sf, q.A, ld, strideM,
-                         q.sc, gb, method == SPK_M_SK ? 0 : 1);
+      // the systems have at most N rows: the panels of the largest
+      if (method != SPK_M_SK)
+        hipLaunchKernelGGL(k_chol_shift, dim3(Sb), dim3(256), 0, sf, q.A, ld,
+                           strideM, q.sc, gb);
+      for (int k0 = 0; k0 < N; k0 += 32) {
+        hipLaunchKernelGGL(k_chol_diag, dim3(1, Sb), dim3(256), 0, sf, q.A,
+                           ld, strideM, q.sc, k0);
+        const int64_t below = N - (k0 + 32);
+        if (below <= 0) break;
+        hipLaunchKernelGGL(k_chol_solve,
+                           dim3((unsigned)ceil_div(below, 256), Sb),
+                           dim3(256), 0, sf, q.A, ld, strideM, q.sc, k0);
+        const int64_t nt = ceil_div(below, 64);
+        hipLaunchKernelGGL(k_chol_trail,
+                           dim3((unsigned)(nt * (nt + 1) / 2), Sb), dim3(256),
+                           0, sf, q.A, ld, strideM, q.sc, k0);
+      }
     }
     SPK_TRY(check_launch("k_potrf"));
     // which steps failed the Cholesky (the host waits for this batch's

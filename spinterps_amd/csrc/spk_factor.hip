@@ -354,6 +354,226 @@ __global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A,
   if (tid == 0 && fail) sc[s].status = SPK_ST_FAILED;
 }
 
+// ---- Cholesky over (block, step) workgroups ------------------------------------
+// The same factorisation as k_potrf, with the same operations in the same
+// order (bit-identical results), spread over many workgroups: k_chol_shift
+// forms C = c0 - G (one workgroup per step); then per 32-wide panel k0,
+// k_chol_diag factors the panel's diagonal block, k_chol_solve solves 256
+// of the rows below it per workgroup, and k_chol_trail updates the
+// trailing triangle, one 64 x 64 tile per workgroup.  A failed pivot sets
+// SPK_ST_FAILED, after which the step's later launches return at once.
+// (Panels of 32 rows only: a panel shorter than 32 is the last one and
+// leaves nothing below it.)
+__global__ __launch_bounds__(256) void k_chol_shift(double *__restrict__ A,
+                                                    int64_t ld,
+                                                    int64_t strideA,
+                                                    StepSc *__restrict__ sc,
+                                                    const spk_vg *__restrict__ vgs) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  double *As = A + (int64_t)s * strideA;
+  const int tid = threadIdx.x;
+  __shared__ double red[256];
+  __shared__ double c0s;
+  const spk_vg &vg = vgs[s];
+  bool bounded = true;
+  for (int k = 0; k < vg.n; ++k) {
+    const int ty = vg.type[k];
+    if (ty == SPK_VG_RNG || ty == SPK_VG_POW || ty == SPK_VG_HOL)
+      bounded = false;
+  }
+  if (bounded) {
+    if (tid == 0) c0s = vg.sill_sum;
+  } else {
+    double mx = 0.0;
+    for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
+      const int i = (int)(e % n), j = (int)(e / n);
+      if (i >= j) mx = fmax(mx, fabs(As[i + (int64_t)j * ld]));
+    }
+    red[tid] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+      __syncthreads();
+    }
+    if (tid == 0) c0s = 2.0 * red[0] + fabs(vg.sill_sum);
+  }
+  __syncthreads();
+  const double c0 = c0s;
+  for (int64_t e = tid; e < (int64_t)n * n; e += 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i >= j) As[i + (int64_t)j * ld] = c0 - As[i + (int64_t)j * ld];
+  }
+  if (tid == 0) sc[s].c0 = c0;
+}
+
+// the panel's 32 x 32 diagonal block, factored in LDS and written back;
+// grid (1, steps)
+__global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A,
+                                                   int64_t ld, int64_t strideA,
+                                                   StepSc *__restrict__ sc,
+                                                   int k0) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  if (k0 >= n) return;
+  const int jb = min(32, n - k0);
+  double *As = A + (int64_t)s * strideA;
+  const int tid = threadIdx.x;
+  __shared__ double D[32][33];
+  __shared__ int fail;
+  if (tid == 0) fail = 0;
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int i = e & 31, j = e >> 5;
+    D[i][j] = (i < jb && j < jb && i >= j)
+                  ? As[(k0 + i) + (int64_t)(k0 + j) * ld]
+                  : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < jb; ++j) {
+    if (tid == 0) {
+      const double d = D[j][j];
+      if (!(d > 0.0) || !isfinite(d))
+        fail = k0 + j + 1;
+      else
+        D[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    if (fail) break;
+    if (tid > j && tid < jb) D[tid][j] /= D[j][j];
+    __syncthreads();
+    for (int e = tid; e < 32 * 32; e += 256) {
+      const int i = e & 31, c = e >> 5;
+      if (c > j && i >= c && i < jb) D[i][c] -= D[i][j] * D[c][j];
+    }
+    __syncthreads();
+  }
+  if (fail) {
+    if (tid == 0) sc[s].status = SPK_ST_FAILED;
+    return;
+  }
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int i = e & 31, j = e >> 5;
+    if (i < jb && j < jb && i >= j)
+      As[(k0 + i) + (int64_t)(k0 + j) * ld] = D[i][j];
+  }
+}
+
+// rows r >= k0 + 32 solve x L_kk^T = a, 256 rows per workgroup;
+// grid (rows below the panel / 256, steps)
+__global__ __launch_bounds__(256) void k_chol_solve(double *__restrict__ A,
+                                                    int64_t ld,
+                                                    int64_t strideA,
+                                                    const StepSc *__restrict__ sc,
+                                                    int k0) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  if (k0 + 32 >= n) return;
+  const int jb = 32, m0 = k0 + 32;
+  const int rbase = m0 + (int)blockIdx.x * 256;
+  if (rbase >= n) return;
+  double *As = A + (int64_t)s * strideA;
+  const int tid = threadIdx.x;
+  __shared__ double D[32][33];
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int i = e & 31, j = e >> 5;
+    D[i][j] = (i >= j) ? As[(k0 + i) + (int64_t)(k0 + j) * ld] : 0.0;
+  }
+  __syncthreads();
+  const int r = rbase + tid;
+  if (r < n) {
+    double x[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+      x[c] = c < jb ? As[r + (int64_t)(k0 + c) * ld] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+      if (c < jb) {
+        double v = x[c];
+#pragma unroll
+        for (int q = 0; q < c; ++q) v -= x[q] * D[c][q];
+        x[c] = v / D[c][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+      if (c < jb) As[r + (int64_t)(k0 + c) * ld] = x[c];
+  }
+}
+
+// grid (tiles of the largest trailing triangle, steps): tile t = (bi, bj),
+// bj <= bi, t = bi (bi + 1) / 2 + bj, of the rows / columns >= k0 + 32
+__global__ __launch_bounds__(256) void k_chol_trail(double *__restrict__ A,
+                                                    int64_t ld,
+                                                    int64_t strideA,
+                                                    const StepSc *__restrict__ sc,
+                                                    int k0) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  if (k0 + 32 >= n) return;
+  const int jb = 32, m0 = k0 + 32;
+  const int nt = (n - m0 + 63) / 64;
+  const int t = blockIdx.x;
+  if (t >= nt * (nt + 1) / 2) return;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (bi * (bi + 1) / 2 > t) --bi;
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  double *As = A + (int64_t)s * strideA;
+  const int tid = threadIdx.x;
+  const int ty = tid >> 4, tx = tid & 15;
+  __shared__ double Pa[64][33];
+  __shared__ double Pb[64][33];
+  const int i0 = m0 + bi * 64, j0 = m0 + bj * 64;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = tid + q * 256;
+    const int r = e & 63, c = e >> 6;
+    Pa[r][c] = (i0 + r < n && c < jb) ? As[(i0 + r) + (int64_t)(k0 + c) * ld]
+                                      : 0.0;
+    Pb[r][c] = (j0 + r < n && c < jb) ? As[(j0 + r) + (int64_t)(k0 + c) * ld]
+                                      : 0.0;
+  }
+  double cur[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = i0 + ty * 4 + a, j = j0 + tx * 4 + b;
+      cur[a][b] = (i < n && j < n && i >= j) ? As[i + (int64_t)j * ld] : 0.0;
+    }
+  __syncthreads();
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int c = 0; c < jb; ++c) {
+    double pa[4], pb[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) pa[a] = Pa[ty * 4 + a][c];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pb[b] = Pb[tx * 4 + b][c];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += pa[a] * pb[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + ty * 4 + a;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tx * 4 + b;
+      if (i < n && j < n && i >= j)
+        As[i + (int64_t)j * ld] = cur[a][b] - acc[a][b];
+    }
+  }
+}
+
 // ---- triangular inverse ----------------------------------------------------------
 // X = L^-1 (lower, zeros above the diagonal) by 64 x 64 blocks: the diagonal
 // blocks are the inverses of L's, X[r, r] = inv(L_rr), and below them

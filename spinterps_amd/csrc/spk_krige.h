@@ -119,6 +119,18 @@ __global__ void k_assemble(const double *sx, const double *sy,
                            double *Gfull);
 __global__ void k_potrf(double *A, int64_t ld, int64_t strideA, StepSc *sc,
                         const spk_vg *vgs, int shift);
+// Cholesky over (block, step) workgroups: k_chol_shift (grid (steps)),
+// then per 32-wide panel k0: k_chol_diag (grid (1, steps)), k_chol_solve
+// (grid (rows below / 256, steps)), k_chol_trail (grid (trailing 64 x 64
+// tiles, steps)).  chol_steps launches the sequence.
+__global__ void k_chol_shift(double *A, int64_t ld, int64_t strideA,
+                             StepSc *sc, const spk_vg *vgs);
+__global__ void k_chol_diag(double *A, int64_t ld, int64_t strideA,
+                            StepSc *sc, int k0);
+__global__ void k_chol_solve(double *A, int64_t ld, int64_t strideA,
+                             const StepSc *sc, int k0);
+__global__ void k_chol_trail(double *A, int64_t ld, int64_t strideA,
+                             const StepSc *sc, int k0);
 // X = L^-1: k_trtri_diag then k_trtri_cols, each with grid
 // (ceil(n_max / 64), steps)
 __global__ void k_trtri_diag(const double *L, double *X, int64_t ld,
