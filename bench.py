@@ -212,7 +212,8 @@ def main():
     kern = {}
     for nm in ('nearest', 'kgen', 'gemm_tri_bf16', 'gemm_tri_f32',
                'gemm_tri_f64', 'slice_cols', 'gemm_tri_i8', 'var_reduce',
-               'potrf', 'trtri', 'dual', 'slice_rows', 'assemble', 'special',
+               'potrf', 'trtri', 'kappa', 'dual', 'slice_rows', 'assemble',
+               'special',
                'cast_tri', 'split_tri', 'pinv', 'pinv_contract'):
         ms, cnt = prof_get(nm)
         kern[nm] = (ms, cnt)

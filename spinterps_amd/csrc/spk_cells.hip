@@ -1789,6 +1789,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int64_t ldk = round_up(Cs, 128);
   const int nrb = (int)ceil_div(N, 64);
   double *R = nullptr, *abuf = nullptr, *K64 = nullptr, *qpart = nullptr;
+  double *kap_lam = nullptr;  // the condition estimate's two eigenvalues
   float *K32 = nullptr;
   uint4 *Kb = nullptr;
   int32_t *nnb = nullptr, *nrst = nullptr;
@@ -1807,6 +1808,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // chunk list (256 chunks of 16 rows) covers the triangle: kgen then
   // leaves culled chunks unwritten
   const int zskip = mixed && ceil_div(N, 16) <= 256 ? 1 : 0;
+  // every batched call estimates each step's condition (k_kap_*)
+  if (batched)
+    SPK_TRY(ws_get(ctx, "kap_lam", S * 2 * 8, (void **)&kap_lam));
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -1975,6 +1979,39 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          q.sc, q.idx, N, prob->stn_drift, m, method, q.M, q.T0,
                          gb, mvg);
     } else {
+      {
+        // condition estimate (power iteration over (row blocks, steps))
+        ProfScope ps(ctx, "kappa", sf);
+        const unsigned gl = (unsigned)ceil_div(N, 256);
+        const unsigned gu = (unsigned)ceil_div(N, 4);
+        for (int which = 0; which < 2; ++which) {
+          const double *Tm = which == 0 ? q.A : q.Li;
+          hipLaunchKernelGGL(k_kap_start, dim3(Sb), dim3(256), 0, sf, q.sc,
+                             q.T0, ld, which);
+          for (int it = 0; it < 12; ++it) {
+            hipLaunchKernelGGL(k_kap_norm, dim3(Sb), dim3(256), 0, sf, q.sc,
+                               q.T0, ld);
+            if (which == 0)
+              hipLaunchKernelGGL(k_kap_umv, dim3(gu, Sb), dim3(256), 0, sf, Tm,
+                                 ld, strideM, q.sc, q.T0, ld, 1);
+            else
+              hipLaunchKernelGGL(k_kap_lmv, dim3(gl, Sb), dim3(256), 0, sf, Tm,
+                                 ld, strideM, q.sc, q.T0, ld, 1);
+            hipLaunchKernelGGL(k_kap_rq, dim3(Sb), dim3(256), 0, sf, q.sc,
+                               q.T0, ld, kap_lam, which);
+            if (which == 0)
+              hipLaunchKernelGGL(k_kap_lmv, dim3(gl, Sb), dim3(256), 0, sf, Tm,
+                                 ld, strideM, q.sc, q.T0, ld, 0);
+            else
+              hipLaunchKernelGGL(k_kap_umv, dim3(gu, Sb), dim3(256), 0, sf, Tm,
+                                 ld, strideM, q.sc, q.T0, ld, 0);
+          }
+        }
+        hipLaunchKernelGGL(k_kap_flags, dim3(Sb), dim3(64), 0, sf, q.sc,
+                           (const double *)kap_lam, gb, fp64var_thr,
+                           fp64var_thr_gau, int8var_thr);
+      }
+      SPK_TRY(check_launch("k_kap"));
       {
         ProfScope ps(ctx, "dual", sf);
         hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,

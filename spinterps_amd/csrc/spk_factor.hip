@@ -838,6 +838,152 @@ __device__ void tri_lower_tmv(const double *__restrict__ Li, int64_t ld,
   }
 }
 
+// ---- condition estimate over (block, step) workgroups --------------------------
+// kappa = lambda_max(C) lambda_max(C^-1) by power iteration on C = Lc Lc^T
+// (which 0, Lc the Cholesky factor) and C^-1 = Li^T Li (which 1), 12 rounds
+// each, Rayleigh quotients; the iterate x and u = T^T x or T x are the first
+// two scratch columns of the step's T0 (X and U of k_dual).  Launch order
+// per which: k_kap_start, then 12 x (k_kap_norm, product into u,
+// k_kap_rq, product back into x); then k_kap_flags.  The products run over
+// (row blocks, steps), one row per thread (k_kap_lmv, y = T x) or one
+// column per wave (k_kap_umv, y = T^T x), with the sums in the order of
+// tri_lower_mv / tri_lower_tmv, and the norms as one workgroup per step with
+// block_sum, so the estimate is the one a single workgroup per step gives.
+__device__ inline double *kap_x(double *T0, int64_t ldt, int s) {
+  return T0 + (int64_t)s * ldt * kT0Cols;
+}
+__device__ inline double *kap_u(double *T0, int64_t ldt, int s) {
+  return kap_x(T0, ldt, s) + ldt * (kMaxP + 1);
+}
+
+// grid (steps)
+__global__ __launch_bounds__(256) void k_kap_start(const StepSc *__restrict__ sc,
+                                                   double *__restrict__ T0,
+                                                   int64_t ldt, int which) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  double *X = kap_x(T0, ldt, s);
+  for (int i = threadIdx.x; i < n; i += 256) {
+    // fixed start vector with both smooth and oscillating parts
+    const unsigned h = (unsigned)i * 2654435761u;
+    X[i] = 1.0 + 0.5 * (double)((h >> 8) & 0xffff) / 65536.0 -
+           (which ? 0.5 * (double)(i & 1) : 0.0);
+  }
+}
+
+// x /= |x|; grid (steps)
+__global__ __launch_bounds__(256) void k_kap_norm(const StepSc *__restrict__ sc,
+                                                  double *__restrict__ T0,
+                                                  int64_t ldt) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  double *X = kap_x(T0, ldt, s);
+  __shared__ double red4[4];
+  const int tid = threadIdx.x;
+  double nv = 0.0;
+  for (int i = tid; i < n; i += 256) nv += X[i] * X[i];
+  nv = block_sum(nv, red4);
+  const double inv = 1.0 / sqrt(nv);
+  for (int i = tid; i < n; i += 256) X[i] *= inv;
+}
+
+// lam[2 s + which] = |u|^2; grid (steps)
+__global__ __launch_bounds__(256) void k_kap_rq(const StepSc *__restrict__ sc,
+                                                double *__restrict__ T0,
+                                                int64_t ldt,
+                                                double *__restrict__ lam,
+                                                int which) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const double *U = kap_u(T0, ldt, s);
+  __shared__ double red4[4];
+  double q = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) q += U[i] * U[i];
+  q = block_sum(q, red4);
+  if (threadIdx.x == 0) lam[2 * s + which] = q;
+}
+
+// y = T v (T lower, column-major, ld), v and y the step's x / u columns
+// (to_u: v = x, y = u; else v = u, y = x); grid (ceil(n_max / 256), steps)
+__global__ __launch_bounds__(256) void k_kap_lmv(const double *__restrict__ T,
+                                                 int64_t ld, int64_t stride,
+                                                 const StepSc *__restrict__ sc,
+                                                 double *__restrict__ T0,
+                                                 int64_t ldt, int to_u) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double *Ts = T + (int64_t)s * stride;
+  const double *v = to_u ? kap_x(T0, ldt, s) : kap_u(T0, ldt, s);
+  double *y = to_u ? kap_u(T0, ldt, s) : kap_x(T0, ldt, s);
+  double acc = 0.0;
+#pragma unroll 8
+  for (int k = 0; k <= i; ++k) {
+    const double l = Ts[i + (int64_t)k * ld];
+    acc += l * v[k];
+  }
+  y[i] = acc;
+}
+
+// y = T^T v, one wave per entry k (lanes over rows i >= k);
+// grid (ceil(n_max / 4), steps)
+__global__ __launch_bounds__(256) void k_kap_umv(const double *__restrict__ T,
+                                                 int64_t ld, int64_t stride,
+                                                 const StepSc *__restrict__ sc,
+                                                 double *__restrict__ T0,
+                                                 int64_t ldt, int to_u) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;
+  const double *Ts = T + (int64_t)s * stride;
+  const double *v = to_u ? kap_x(T0, ldt, s) : kap_u(T0, ldt, s);
+  double *y = to_u ? kap_u(T0, ldt, s) : kap_x(T0, ldt, s);
+  double acc = 0.0;
+#pragma unroll 4
+  for (int i = k + lane; i < n; i += 64) {
+    const double l = Ts[i + (int64_t)k * ld];
+    acc += l * v[i];
+  }
+  const double r = wave_sum(acc);
+  if (lane == 0) y[k] = r;
+}
+
+// kappa and the step's precision flags; grid (steps), one thread
+__global__ void k_kap_flags(StepSc *__restrict__ sc,
+                            const double *__restrict__ lam,
+                            const spk_vg *__restrict__ vgs,
+                            double fp64var_thr, double fp64var_thr_gau,
+                            double int8var_thr) {
+  const int s = blockIdx.x;
+  if (threadIdx.x != 0 || sc[s].status) return;
+  const double kap = lam[2 * s] * lam[2 * s + 1];
+  sc[s].kappa = kap;
+  int fl = 0;
+  if (!(kap <= 1e12)) fl |= SPK_ST_ILLCOND;
+  bool gau = false;
+  for (int k = 0; k < vgs[s].n; ++k) gau |= vgs[s].type[k] == SPK_VG_GAU;
+  // above the FP32 limit: exact int8 plane products up to the int8
+  // limit, FP64 beyond it
+  int planes = 0;
+  if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr))) {
+    if (kap <= int8var_thr)
+      planes = kap <= kInt8Deep ? 5 : 6;
+    else
+      fl |= SPK_ST_FP64VAR;
+  }
+  if (planes) fl |= SPK_ST_INT8VAR;
+  sc[s].flags = fl;
+  sc[s].planes = planes;
+}
+
 // Work array T0: [ld x 2 (kMaxP+1)] per step.
 __global__ __launch_bounds__(256) void k_dual(
     const double *__restrict__ Lc, const double *__restrict__ Li, int64_t ld,
@@ -868,68 +1014,8 @@ __global__ __launch_bounds__(256) void k_dual(
   double *dw = dalpha + ldt;
   const int p = (method == SPK_M_EDK) ? 1 + m : 1;
   const int tid = threadIdx.x;
-  // Condition estimate kappa = lambda_max(C) lambda_max(C^-1) by power
-  // iteration on C = L L^T and C^-1 = Li^T Li (Rayleigh quotients, 12
-  // rounds each; X and U serve as scratch before their real use below).
-  {
-    __shared__ double red4[4];
-    double lam[2] = {0.0, 0.0};
-    for (int which = 0; which < 2; ++which) {
-      const double *T = which == 0 ? Lc + (int64_t)s * stride : L;
-      for (int i = tid; i < n; i += 256) {
-        // fixed start vector with both smooth and oscillating parts
-        const unsigned h = (unsigned)i * 2654435761u;
-        X[i] = 1.0 + 0.5 * (double)((h >> 8) & 0xffff) / 65536.0 -
-               (which ? 0.5 * (double)(i & 1) : 0.0);
-      }
-      __syncthreads();
-      for (int it = 0; it < 12; ++it) {
-        double nv = 0.0;
-        for (int i = tid; i < n; i += 256) nv += X[i] * X[i];
-        nv = block_sum(nv, red4);
-        const double inv = 1.0 / sqrt(nv);
-        for (int i = tid; i < n; i += 256) X[i] *= inv;
-        __syncthreads();
-        if (which == 0) {
-          tri_lower_tmv(T, ld, n, X, U, 1);  // U = L^T x
-          __syncthreads();
-          double q = 0.0;
-          for (int i = tid; i < n; i += 256) q += U[i] * U[i];
-          lam[0] = block_sum(q, red4);  // x^T C x
-          tri_lower_mv(T, ld, n, U, X, 1);  // x = L L^T x
-        } else {
-          tri_lower_mv(T, ld, n, X, U, 1);  // U = Li x
-          __syncthreads();
-          double q = 0.0;
-          for (int i = tid; i < n; i += 256) q += U[i] * U[i];
-          lam[1] = block_sum(q, red4);  // x^T C^-1 x
-          tri_lower_tmv(T, ld, n, U, X, 1);  // x = Li^T Li x
-        }
-        __syncthreads();
-      }
-    }
-    if (tid == 0) {
-      const double kap = lam[0] * lam[1];
-      sc[s].kappa = kap;
-      int fl = 0;
-      if (!(kap <= 1e12)) fl |= SPK_ST_ILLCOND;
-      bool gau = false;
-      for (int k = 0; k < vgs[s].n; ++k) gau |= vgs[s].type[k] == SPK_VG_GAU;
-      // above the FP32 limit: exact int8 plane products up to the int8
-      // limit, FP64 beyond it
-      int planes = 0;
-      if (!(kap <= (gau ? fp64var_thr_gau : fp64var_thr))) {
-        if (kap <= int8var_thr)
-          planes = kap <= kInt8Deep ? 5 : 6;
-        else
-          fl |= SPK_ST_FP64VAR;
-      }
-      if (planes) fl |= SPK_ST_INT8VAR;
-      sc[s].flags = fl;
-      sc[s].planes = planes;
-    }
-    __syncthreads();
-  }
+  // (the condition estimate and the step's precision flags come from the
+  // k_kap_* launches before this kernel)
   // X columns: 0..p-1 = F (ones, drifts), p = z
   for (int i = tid; i < n; i += 256) {
     X[i] = 1.0;

@@ -137,6 +137,21 @@ __global__ void k_trtri_diag(const double *L, double *X, int64_t ld,
                              int64_t stride, const StepSc *sc);
 __global__ void k_trtri_cols(const double *L, double *X, int64_t ld,
                              int64_t stride, const StepSc *sc);
+// condition estimate: k_kap_start, 12 x (k_kap_norm, k_kap_umv / k_kap_lmv,
+// k_kap_rq, k_kap_lmv / k_kap_umv) per power iteration, then k_kap_flags
+// (spk_factor.hip)
+__global__ void k_kap_start(const StepSc *sc, double *T0, int64_t ldt,
+                            int which);
+__global__ void k_kap_norm(const StepSc *sc, double *T0, int64_t ldt);
+__global__ void k_kap_rq(const StepSc *sc, double *T0, int64_t ldt,
+                         double *lam, int which);
+__global__ void k_kap_lmv(const double *T, int64_t ld, int64_t stride,
+                          const StepSc *sc, double *T0, int64_t ldt, int to_u);
+__global__ void k_kap_umv(const double *T, int64_t ld, int64_t stride,
+                          const StepSc *sc, double *T0, int64_t ldt, int to_u);
+__global__ void k_kap_flags(StepSc *sc, const double *lam, const spk_vg *vgs,
+                            double fp64var_thr, double fp64var_thr_gau,
+                            double int8var_thr);
 __global__ void k_dual(const double *Lc, const double *Li, int64_t ld,
                        int64_t stride,
                        StepSc *sc, const double *zact, int64_t ldz,
