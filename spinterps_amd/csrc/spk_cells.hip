@@ -1790,6 +1790,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   const int nrb = (int)ceil_div(N, 64);
   double *R = nullptr, *abuf = nullptr, *K64 = nullptr, *qpart = nullptr;
   double *kap_lam = nullptr;  // the condition estimate's two eigenvalues
+  double *dual_ds = nullptr;  // per-step scratch of the dual-vector kernels
   float *K32 = nullptr;
   uint4 *Kb = nullptr;
   int32_t *nnb = nullptr, *nrst = nullptr;
@@ -1809,8 +1810,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // leaves culled chunks unwritten
   const int zskip = mixed && ceil_div(N, 16) <= 256 ? 1 : 0;
   // every batched call estimates each step's condition (k_kap_*)
-  if (batched)
+  if (batched) {
     SPK_TRY(ws_get(ctx, "kap_lam", S * 2 * 8, (void **)&kap_lam));
+    SPK_TRY(ws_get(ctx, "dual_ds", S * kDualSc * 8, (void **)&dual_ds));
+  }
   if (!batched) {
     SPK_TRY(ws_get(ctx, "R", (N + p + 64) * ldk * 8, (void **)&R));
     SPK_TRY(ws_get(ctx, "nnb", ldk * 4, (void **)&nnb));
@@ -2014,11 +2017,35 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       SPK_TRY(check_launch("k_kap"));
       {
         ProfScope ps(ctx, "dual", sf);
-        hipLaunchKernelGGL(k_dual, dim3(Sb), dim3(256), 0, sf, q.A, q.Li, ld,
-                           strideM, q.sc, q.zact, ldz, q.idx, N,
-                           prob->stn_drift, m, method, q.V, 4 * ld, q.T0, gb,
-                           mvg, fp64var_thr, fp64var_thr_gau, int8var_thr,
-                           (const double *)sx, (const double *)sy);
+        const unsigned gl = (unsigned)ceil_div(N, 256);
+        const unsigned gu = (unsigned)ceil_div(N, 4);
+        const int pq = method == SPK_M_EDK ? 1 + m : 1;
+        hipLaunchKernelGGL(k_dual_rhs, dim3(Sb), dim3(256), 0, sf, q.sc,
+                           q.zact, ldz, q.idx, N, prob->stn_drift, m, method,
+                           q.T0, ld);
+        // U = Li [F z], X = C^-1 [F z]
+        hipLaunchKernelGGL(k_dual_lmv, dim3(gl, Sb), dim3(256), 0, sf, q.Li, ld,
+                           strideM, q.sc, q.T0, ld, 0, 1, pq + 1);
+        hipLaunchKernelGGL(k_dual_umv, dim3(gu, Sb), dim3(256), 0, sf, q.Li, ld,
+                           strideM, q.sc, q.T0, ld, 1, 0, pq + 1);
+        hipLaunchKernelGGL(k_dual_mid, dim3(Sb), dim3(256), 0, sf, q.sc, m,
+                           method, q.T0, ld, q.V, 4 * ld, dual_ds);
+        for (int pass = 0; pass < 2; ++pass) {
+          hipLaunchKernelGGL(k_dual_resid, dim3(gl, Sb), dim3(256), 0, sf,
+                             q.sc, q.zact, ldz, q.idx, N, prob->stn_drift, m,
+                             method, q.T0, ld, q.V, 4 * ld, dual_ds, gb, mvg,
+                             (const double *)sx, (const double *)sy);
+          // Y2 = Li [r1 rw], Z2 = C^-1 [r1 rw]
+          hipLaunchKernelGGL(k_dual_lmv, dim3(gl, Sb), dim3(256), 0, sf, q.Li,
+                             ld, strideM, q.sc, q.T0, ld, 2, 3, 2);
+          hipLaunchKernelGGL(k_dual_umv, dim3(gu, Sb), dim3(256), 0, sf, q.Li,
+                             ld, strideM, q.sc, q.T0, ld, 3, 4, 2);
+          hipLaunchKernelGGL(k_dual_upd, dim3(Sb), dim3(256), 0, sf, q.sc,
+                             q.idx, N, prob->stn_drift, m, method, q.T0, ld,
+                             q.V, 4 * ld, dual_ds);
+        }
+        hipLaunchKernelGGL(k_dual_fin, dim3(Sb), dim3(64), 0, sf, q.sc, m,
+                           method, (const double *)dual_ds);
       }
       // Steps whose Cholesky succeeded with a condition estimate above
       // kPinvKappa take the pseudo-inverse route too: their solve is

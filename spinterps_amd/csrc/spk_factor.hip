@@ -984,58 +984,150 @@ __global__ void k_kap_flags(StepSc *__restrict__ sc,
   sc[s].planes = planes;
 }
 
-// Work array T0: [ld x 2 (kMaxP+1)] per step.
-__global__ __launch_bounds__(256) void k_dual(
-    const double *__restrict__ Lc, const double *__restrict__ Li, int64_t ld,
-    int64_t stride,
-    StepSc *__restrict__ sc, const double *__restrict__ zact, int64_t ldz,
-    const int32_t *__restrict__ idx, int64_t N,
+// ---- dual vectors and refinement over (block, step) workgroups ----------------
+// Per step, with T0 holding X, U (kMaxP + 1 columns each), R2, Y2, Z2 (two
+// each) and the per-step scratch ds (kDualSc doubles: G, Bi, hcoef, hcur,
+// dhs):
+//   k_dual_rhs   X = [F z]                                    (steps)
+//   k_dual_lmv   U = Li X            (p + 1 columns)          (row blocks, steps)
+//   k_dual_umv   X = Li^T U = C^-1 [F z]                      (column groups, steps)
+//   k_dual_mid   G = U^T U, Bi = (U_F^T U_F)^-1, h, alpha, w  (steps)
+//   then twice: k_dual_resid (r1 = z - C alpha - F h, rw = 1 - C w;
+//   row blocks, steps), k_dual_lmv / k_dual_umv on [r1 rw], k_dual_upd
+//   (the corrections of h, alpha and w; steps); then k_dual_fin (steps).
+// Every sum runs in the order of the one-workgroup-per-step form (a row per
+// thread, a column per wave, one workgroup per step for the small
+// reductions), so the results are the ones that form gives.
+constexpr int kDsG = 0;                                  // (p+1)^2
+constexpr int kDsBi = kDsG + (kMaxP + 1) * (kMaxP + 1);  // p^2
+constexpr int kDsH = kDsBi + kMaxP * kMaxP;              // hcoef
+constexpr int kDsHc = kDsH + kMaxP;                      // hcur
+constexpr int kDsDh = kDsHc + kMaxP;                     // dhs
+
+struct DualPtrs {
+  double *X, *U, *R2, *Y2, *Z2, *alpha, *w, *dalpha, *dw, *ds;
+};
+__device__ inline DualPtrs dual_ptrs(double *T0, int64_t ldt, double *V,
+                                     int64_t ldv, double *DS, int s) {
+  DualPtrs d;
+  d.X = T0 + (int64_t)s * ldt * kT0Cols;
+  d.U = d.X + ldt * (kMaxP + 1);
+  d.R2 = d.U + ldt * (kMaxP + 1);  // residuals r1, rw
+  d.Y2 = d.R2 + 2 * ldt;           // Li [r1 rw]
+  d.Z2 = d.Y2 + 2 * ldt;           // C^-1 [r1 rw]
+  d.alpha = V + (int64_t)s * ldv;
+  d.w = d.alpha + ldt;
+  d.dalpha = d.w + ldt;
+  d.dw = d.dalpha + ldt;
+  d.ds = DS + (int64_t)s * kDualSc;
+  return d;
+}
+
+// X columns: 0..p-1 = F (ones, drifts), p = z; grid (steps)
+__global__ __launch_bounds__(256) void k_dual_rhs(
+    const StepSc *__restrict__ sc, const double *__restrict__ zact,
+    int64_t ldz, const int32_t *__restrict__ idx, int64_t N,
     const double *__restrict__ stn_drift, int m, int method,
-    double *__restrict__ V, int64_t ldv, double *__restrict__ T0,
-    const spk_vg *__restrict__ vgs, double min_vg_val, double fp64var_thr,
-    double fp64var_thr_gau, double int8var_thr,
-    const double *__restrict__ sx, const double *__restrict__ sy) {
-  const int64_t ldt = ld;
+    double *__restrict__ T0, int64_t ldt) {
   const int s = blockIdx.x;
   if (sc[s].status) return;
   const int n = sc[s].n;
-  const double *L = Li + (int64_t)s * stride;
   const double *z = zact + (int64_t)s * ldz;
   const int32_t *my = idx + (int64_t)s * N;
-  // per-step work area: kT0Cols columns of ld
   double *X = T0 + (int64_t)s * ldt * kT0Cols;
-  double *U = X + ldt * (kMaxP + 1);
-  double *R2 = U + ldt * (kMaxP + 1);  // residuals r1, rw
-  double *Y2 = R2 + 2 * ldt;           // Li [r1 rw]
-  double *Z2 = Y2 + 2 * ldt;           // C^-1 [r1 rw]
-  double *alpha = V + (int64_t)s * ldv;
-  double *w = alpha + ldt;
-  double *dalpha = w + ldt;
-  double *dw = dalpha + ldt;
   const int p = (method == SPK_M_EDK) ? 1 + m : 1;
-  const int tid = threadIdx.x;
-  // (the condition estimate and the step's precision flags come from the
-  // k_kap_* launches before this kernel)
-  // X columns: 0..p-1 = F (ones, drifts), p = z
-  for (int i = tid; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += 256) {
     X[i] = 1.0;
     if (method == SPK_M_EDK)
       for (int k = 0; k < m; ++k)
         X[i + (int64_t)(1 + k) * ldt] = stn_drift[(int64_t)my[i] * m + k];
     X[i + (int64_t)p * ldt] = z[i];
   }
-  __syncthreads();
-  tri_lower_mv(L, ld, n, X, U, p + 1);  // U = Li [F z]   (ld of L = ld)
-  __syncthreads();
-  tri_lower_tmv(L, ld, n, U, X, p + 1);  // X = C^-1 [F z]
-  __syncthreads();
+}
+
+// column offsets in T0 of the products' operands (col: 0 X, 1 U, 2 R2,
+// 3 Y2, 4 Z2)
+__device__ inline double *t0_col(double *T0, int64_t ldt, int s, int col) {
+  double *X = T0 + (int64_t)s * ldt * kT0Cols;
+  switch (col) {
+    case 0: return X;
+    case 1: return X + ldt * (kMaxP + 1);
+    case 2: return X + 2 * ldt * (kMaxP + 1);
+    case 3: return X + 2 * ldt * (kMaxP + 1) + 2 * ldt;
+    default: return X + 2 * ldt * (kMaxP + 1) + 4 * ldt;
+  }
+}
+
+// Y[:, r] = Li X[:, r] (r < nr), one row per thread; grid (ceil(n_max /
+// 256), steps)
+__global__ __launch_bounds__(256) void k_dual_lmv(
+    const double *__restrict__ Li, int64_t ld, int64_t stride,
+    const StepSc *__restrict__ sc, double *__restrict__ T0, int64_t ldt,
+    int src, int dst, int nr) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double *L = Li + (int64_t)s * stride;
+  const double *X = t0_col(T0, ldt, s, src);
+  double *Y = t0_col(T0, ldt, s, dst);
+  double acc[kMaxP + 1];
+  for (int r = 0; r < nr; ++r) acc[r] = 0.0;
+#pragma unroll 8
+  for (int k = 0; k <= i; ++k) {
+    const double l = L[i + (int64_t)k * ld];
+    for (int r = 0; r < nr; ++r) acc[r] += l * X[k + (int64_t)r * ldt];
+  }
+  for (int r = 0; r < nr; ++r) Y[i + (int64_t)r * ldt] = acc[r];
+}
+
+// Y[:, r] = Li^T X[:, r], one wave per output entry (lanes over rows);
+// grid (ceil(n_max / 4), steps)
+__global__ __launch_bounds__(256) void k_dual_umv(
+    const double *__restrict__ Li, int64_t ld, int64_t stride,
+    const StepSc *__restrict__ sc, double *__restrict__ T0, int64_t ldt,
+    int src, int dst, int nr) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;
+  const double *L = Li + (int64_t)s * stride;
+  const double *X = t0_col(T0, ldt, s, src);
+  double *Y = t0_col(T0, ldt, s, dst);
+  double acc[kMaxP + 1];
+  for (int r = 0; r < nr; ++r) acc[r] = 0.0;
+#pragma unroll 4
+  for (int i = k + lane; i < n; i += 64) {
+    const double l = L[i + (int64_t)k * ld];
+    for (int r = 0; r < nr; ++r) acc[r] += l * X[i + (int64_t)r * ldt];
+  }
+  for (int r = 0; r < nr; ++r) {
+    const double v = wave_sum(acc[r]);
+    if (lane == 0) Y[k + (int64_t)r * ldt] = v;
+  }
+}
+
+// G = U^T U (a, b <= p: B = U_F^T U_F, g = U_F^T U_z), Bi = B^-1, the
+// Schur coefficients h = Bi g, alpha = C^-1 z - C^-1 F h, w = C^-1 1;
+// grid (steps)
+__global__ __launch_bounds__(256) void k_dual_mid(
+    const StepSc *__restrict__ sc, int m, int method, double *__restrict__ T0,
+    int64_t ldt, double *__restrict__ V, int64_t ldv, double *__restrict__ DS) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const DualPtrs d = dual_ptrs(T0, ldt, V, ldv, DS, s);
+  const int p = (method == SPK_M_EDK) ? 1 + m : 1;
+  const int tid = threadIdx.x;
   __shared__ double G[(kMaxP + 1) * (kMaxP + 1)];
-  // G[a][b] = U[:, a] . U[:, b]  (a, b <= p): B = U_F^T U_F, g = U_F^T U_z
   for (int e = tid >> 6; e < (p + 1) * (p + 1); e += 4) {
     const int a = e / (p + 1), b = e % (p + 1);
     double acc = 0.0;
     for (int i = (tid & 63); i < n; i += 64)
-      acc += U[i + (int64_t)a * ldt] * U[i + (int64_t)b * ldt];
+      acc += d.U[i + (int64_t)a * ldt] * d.U[i + (int64_t)b * ldt];
     acc = wave_sum(acc);
     if ((tid & 63) == 0) G[e] = acc;
   }
@@ -1057,135 +1149,173 @@ __global__ __launch_bounds__(256) void k_dual(
         hcoef[a] = v;
       }
     }
+    for (int e = 0; e < (p + 1) * (p + 1); ++e) d.ds[kDsG + e] = G[e];
+    if (method != SPK_M_SK)
+      for (int e = 0; e < p * p; ++e) d.ds[kDsBi + e] = Bi[e];
+    for (int a = 0; a < kMaxP; ++a) {
+      d.ds[kDsH + a] = a < p ? hcoef[a] : 0.0;
+      d.ds[kDsHc + a] = a < p ? hcoef[a] : 0.0;
+    }
   }
   __syncthreads();
   for (int i = tid; i < n; i += 256) {
-    double a = X[i + (int64_t)p * ldt];
+    double a = d.X[i + (int64_t)p * ldt];
     if (method != SPK_M_SK)
-      for (int k = 0; k < p; ++k) a -= X[i + (int64_t)k * ldt] * hcoef[k];
-    alpha[i] = a;
-    w[i] = X[i];
+      for (int k = 0; k < p; ++k) a -= d.X[i + (int64_t)k * ldt] * hcoef[k];
+    d.alpha[i] = a;
+    d.w[i] = d.X[i];
+  }
+}
+
+// ---- iterative refinement with FP64 residuals ------------------------------
+// r1 = z - C alpha - F h, rw = 1 - C w, with C the covariances the
+// Cholesky factorised: c0 - gamma (OK, EDK) or the covariance sum (SK),
+// recomputed from the re-centred coordinates with the same arithmetic.
+// Two steps: the first removes the error of the explicit-inverse solve,
+// the second's correction is of the size of the error left, which the
+// cell kernel turns into each cell's error indicator.
+// One row per thread; grid (ceil(n_max / 256), steps).
+__global__ __launch_bounds__(256) void k_dual_resid(
+    const StepSc *__restrict__ sc, const double *__restrict__ zact,
+    int64_t ldz, const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ stn_drift, int m, int method,
+    double *__restrict__ T0, int64_t ldt, double *__restrict__ V, int64_t ldv,
+    double *__restrict__ DS, const spk_vg *__restrict__ vgs, double min_vg_val,
+    const double *__restrict__ sx, const double *__restrict__ sy) {
+  const int s = blockIdx.y;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const int i0 = blockIdx.x * 256;
+  if (i0 >= n) return;
+  const DualPtrs d = dual_ptrs(T0, ldt, V, ldv, DS, s);
+  const double *z = zact + (int64_t)s * ldz;
+  const int32_t *my = idx + (int64_t)s * N;
+  const int p = (method == SPK_M_EDK) ? 1 + m : 1;
+  const int tid = threadIdx.x;
+  __shared__ double stx[256], sty[256], sal[256], sw[256];
+  __shared__ spk_vg vg;
+  if (tid == 0) vg = vgs[s];
+  const bool sk = method == SPK_M_SK;
+  const double c0 = sc[s].c0;
+  const int i = i0 + tid;
+  double xi = 0.0, yi = 0.0;
+  if (i < n) {
+    xi = sx[my[i]];
+    yi = sy[my[i]];
+  }
+  double ca = 0.0, cw = 0.0;
+  for (int j0 = 0; j0 < n; j0 += 256) {
+    __syncthreads();
+    if (j0 + tid < n) {
+      const int g = my[j0 + tid];
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+      sal[tid] = d.alpha[j0 + tid];
+      sw[tid] = d.w[j0 + tid];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - j0);
+    if (i < n)
+      for (int j = 0; j < cnt; ++j) {
+        const double dd = dist2(xi, yi, stx[j], sty[j]);
+        const double e = vg_entry(vg, dd, sk ? 1 : 0, min_vg_val);
+        const double c = sk ? e : c0 - e;
+        ca += c * sal[j];
+        cw += c * sw[j];
+      }
+  }
+  if (i < n) {
+    double r1 = z[i] - ca;
+    if (!sk) {
+      r1 -= d.ds[kDsHc + 0];
+      for (int k = 1; k < p; ++k)
+        r1 -= stn_drift[(int64_t)my[i] * m + (k - 1)] * d.ds[kDsHc + k];
+    }
+    d.R2[i] = r1;
+    d.R2[i + ldt] = 1.0 - cw;
+  }
+}
+
+// gF[a] = U_F[:, a] . (Li r1) = F^T C^-1 r1, r2[a] = -F[:, a] . alpha, the
+// corrections dh = Bi (gF - r2), alpha += Z2[:, 0] - C^-1 F dh,
+// w += Z2[:, 1]; grid (steps)
+__global__ __launch_bounds__(256) void k_dual_upd(
+    const StepSc *__restrict__ sc, const int32_t *__restrict__ idx, int64_t N,
+    const double *__restrict__ stn_drift, int m, int method,
+    double *__restrict__ T0, int64_t ldt, double *__restrict__ V, int64_t ldv,
+    double *__restrict__ DS) {
+  const int s = blockIdx.x;
+  if (sc[s].status) return;
+  const int n = sc[s].n;
+  const DualPtrs d = dual_ptrs(T0, ldt, V, ldv, DS, s);
+  const int32_t *my = idx + (int64_t)s * N;
+  const int p = (method == SPK_M_EDK) ? 1 + m : 1;
+  const int tid = threadIdx.x;
+  const bool sk = method == SPK_M_SK;
+  __shared__ double gF[kMaxP], r2s[kMaxP], dhs[kMaxP];
+  if (!sk) {
+    for (int a = tid >> 6; a < 2 * p; a += 4) {
+      double acc = 0.0;
+      const int col = a < p ? a : a - p;
+      for (int i = (tid & 63); i < n; i += 64) {
+        if (a < p) {
+          acc += d.U[i + (int64_t)col * ldt] * d.Y2[i];
+        } else {
+          const double f =
+              col == 0 ? 1.0 : stn_drift[(int64_t)my[i] * m + (col - 1)];
+          acc -= f * d.alpha[i];
+        }
+      }
+      acc = wave_sum(acc);
+      if ((tid & 63) == 0) {
+        if (a < p)
+          gF[col] = acc;
+        else
+          r2s[col] = acc;
+      }
+    }
   }
   __syncthreads();
-
-  // ---- iterative refinement with FP64 residuals ------------------------------
-  // r1 = z - C alpha - F h, rw = 1 - C w, with C the covariances the
-  // Cholesky factorised: c0 - gamma (OK, EDK) or the covariance sum (SK),
-  // recomputed from the re-centred coordinates with the same arithmetic.
-  // Two steps: the first removes the error of the explicit-inverse solve,
-  // the second's correction is of the size of the error left, which the
-  // cell kernel turns into each cell's error indicator.
-  {
-    __shared__ double stx[256], sty[256], sal[256], sw[256];
-    __shared__ spk_vg vg;
-    __shared__ double hcur[kMaxP];
-    __shared__ double gF[kMaxP], r2s[kMaxP], dhs[kMaxP];
-    if (tid == 0) vg = vgs[s];
-    if (tid < kMaxP) hcur[tid] = tid < p ? hcoef[tid] : 0.0;
-    const bool sk = method == SPK_M_SK;
-    const double c0 = sc[s].c0;
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int i0 = 0; i0 < n; i0 += 256) {
-        const int i = i0 + tid;
-        double xi = 0.0, yi = 0.0;
-        if (i < n) {
-          xi = sx[my[i]];
-          yi = sy[my[i]];
-        }
-        double ca = 0.0, cw = 0.0;
-        for (int j0 = 0; j0 < n; j0 += 256) {
-          __syncthreads();
-          if (j0 + tid < n) {
-            const int g = my[j0 + tid];
-            stx[tid] = sx[g];
-            sty[tid] = sy[g];
-            sal[tid] = alpha[j0 + tid];
-            sw[tid] = w[j0 + tid];
-          }
-          __syncthreads();
-          const int cnt = min(256, n - j0);
-          if (i < n)
-            for (int j = 0; j < cnt; ++j) {
-              const double d = dist2(xi, yi, stx[j], sty[j]);
-              const double e = vg_entry(vg, d, sk ? 1 : 0, min_vg_val);
-              const double c = sk ? e : c0 - e;
-              ca += c * sal[j];
-              cw += c * sw[j];
-            }
-        }
-        if (i < n) {
-          double r1 = z[i] - ca;
-          if (!sk) {
-            r1 -= hcur[0];
-            for (int k = 1; k < p; ++k)
-              r1 -= stn_drift[(int64_t)my[i] * m + (k - 1)] * hcur[k];
-          }
-          R2[i] = r1;
-          R2[i + ldt] = 1.0 - cw;
-        }
-      }
-      __syncthreads();
-      tri_lower_mv(L, ld, n, R2, Y2, 2);  // Li [r1 rw]
-      __syncthreads();
-      tri_lower_tmv(L, ld, n, Y2, Z2, 2);  // C^-1 [r1 rw]
-      __syncthreads();
-      // gF[a] = U_F[:, a] . (Li r1) = F^T C^-1 r1, r2[a] = -F[:, a] . alpha
-      if (!sk) {
-        for (int a = tid >> 6; a < 2 * p; a += 4) {
-          double acc = 0.0;
-          const int col = a < p ? a : a - p;
-          for (int i = (tid & 63); i < n; i += 64) {
-            if (a < p) {
-              acc += U[i + (int64_t)col * ldt] * Y2[i];
-            } else {
-              const double f =
-                  col == 0 ? 1.0 : stn_drift[(int64_t)my[i] * m + (col - 1)];
-              acc -= f * alpha[i];
-            }
-          }
-          acc = wave_sum(acc);
-          if ((tid & 63) == 0) {
-            if (a < p)
-              gF[col] = acc;
-            else
-              r2s[col] = acc;
-          }
-        }
-      }
-      __syncthreads();
-      if (tid == 0) {
-        for (int a = 0; a < kMaxP; ++a) dhs[a] = 0.0;
-        if (!sk)
-          for (int a = 0; a < p; ++a) {
-            double v = 0.0;
-            for (int b = 0; b < p; ++b) v += Bi[a * p + b] * (gF[b] - r2s[b]);
-            dhs[a] = v;
-          }
-      }
-      __syncthreads();
-      for (int i = tid; i < n; i += 256) {
-        double da = Z2[i];
-        if (!sk)
-          for (int k = 0; k < p; ++k) da -= X[i + (int64_t)k * ldt] * dhs[k];
-        const double dwi = Z2[i + ldt];
-        alpha[i] += da;
-        w[i] += dwi;
-        dalpha[i] = da;
-        dw[i] = dwi;
-      }
-      if (tid < p) hcur[tid] += dhs[tid];
-      __syncthreads();
-    }
-    if (tid == 0) {
-      StepSc &o = sc[s];
-      o.b = G[0];  // 1^T C^-1 1
+  if (tid == 0) {
+    for (int a = 0; a < kMaxP; ++a) dhs[a] = 0.0;
+    if (!sk)
       for (int a = 0; a < p; ++a) {
-        o.h[a] = hcur[a];
-        o.dh[a] = dhs[a];
+        double v = 0.0;
+        for (int b = 0; b < p; ++b)
+          v += d.ds[kDsBi + a * p + b] * (gF[b] - r2s[b]);
+        dhs[a] = v;
       }
-      o.a0 = sk ? 0.0 : hcur[0];
-    }
   }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    double da = d.Z2[i];
+    if (!sk)
+      for (int k = 0; k < p; ++k) da -= d.X[i + (int64_t)k * ldt] * dhs[k];
+    const double dwi = d.Z2[i + ldt];
+    d.alpha[i] += da;
+    d.w[i] += dwi;
+    d.dalpha[i] = da;
+    d.dw[i] = dwi;
+  }
+  if (tid < p) d.ds[kDsHc + tid] += dhs[tid];
+  if (tid < kMaxP) d.ds[kDsDh + tid] = dhs[tid];
+}
+
+// the step record's dual-form terms; grid (steps)
+__global__ void k_dual_fin(StepSc *__restrict__ sc, int m, int method,
+                           const double *__restrict__ DS) {
+  const int s = blockIdx.x;
+  if (threadIdx.x != 0 || sc[s].status) return;
+  const double *ds = DS + (int64_t)s * kDualSc;
+  const int p = (method == SPK_M_EDK) ? 1 + m : 1;
+  const bool sk = method == SPK_M_SK;
+  StepSc &o = sc[s];
+  o.b = ds[kDsG + 0];  // 1^T C^-1 1
+  for (int a = 0; a < p; ++a) {
+    o.h[a] = ds[kDsHc + a];
+    o.dh[a] = ds[kDsDh + a];
+  }
+  o.a0 = sk ? 0.0 : ds[kDsHc + 0];
 }
 
 // ---- explicit K^-1 for the plain path -------------------------------------------

@@ -152,15 +152,33 @@ __global__ void k_kap_umv(const double *T, int64_t ld, int64_t stride,
 __global__ void k_kap_flags(StepSc *sc, const double *lam, const spk_vg *vgs,
                             double fp64var_thr, double fp64var_thr_gau,
                             double int8var_thr);
-__global__ void k_dual(const double *Lc, const double *Li, int64_t ld,
-                       int64_t stride,
-                       StepSc *sc, const double *zact, int64_t ldz,
-                       const int32_t *idx, int64_t N, const double *stn_drift,
-                       int m, int method, double *V, int64_t ldv, double *T0,
-                       const spk_vg *vgs, double min_vg_val,
-                       double fp64var_thr, double fp64var_thr_gau,
-                       double int8var_thr, const double *sx,
-                       const double *sy);
+// dual vectors and refinement (spk_factor.hip): k_dual_rhs, k_dual_lmv,
+// k_dual_umv, k_dual_mid, then twice k_dual_resid, k_dual_lmv, k_dual_umv,
+// k_dual_upd, then k_dual_fin.  DS: kDualSc doubles per step.
+constexpr int kDualSc = 256;
+__global__ void k_dual_rhs(const StepSc *sc, const double *zact, int64_t ldz,
+                           const int32_t *idx, int64_t N,
+                           const double *stn_drift, int m, int method,
+                           double *T0, int64_t ldt);
+__global__ void k_dual_lmv(const double *Li, int64_t ld, int64_t stride,
+                           const StepSc *sc, double *T0, int64_t ldt, int src,
+                           int dst, int nr);
+__global__ void k_dual_umv(const double *Li, int64_t ld, int64_t stride,
+                           const StepSc *sc, double *T0, int64_t ldt, int src,
+                           int dst, int nr);
+__global__ void k_dual_mid(const StepSc *sc, int m, int method, double *T0,
+                           int64_t ldt, double *V, int64_t ldv, double *DS);
+__global__ void k_dual_resid(const StepSc *sc, const double *zact,
+                             int64_t ldz, const int32_t *idx, int64_t N,
+                             const double *stn_drift, int m, int method,
+                             double *T0, int64_t ldt, double *V, int64_t ldv,
+                             double *DS, const spk_vg *vgs, double min_vg_val,
+                             const double *sx, const double *sy);
+__global__ void k_dual_upd(const StepSc *sc, const int32_t *idx, int64_t N,
+                           const double *stn_drift, int m, int method,
+                           double *T0, int64_t ldt, double *V, int64_t ldv,
+                           double *DS);
+__global__ void k_dual_fin(StepSc *sc, int m, int method, const double *DS);
 __global__ void k_minv(const double *Li, int64_t ld, int64_t stride,
                        StepSc *sc, const int32_t *idx, int64_t N,
                        const double *stn_drift, int m, int method, double *M,
