@@ -940,6 +940,11 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
       qloc = 2.0 * bk - tt;
     }
   }
+  // slots of the local fit for the delta update; an unused one (slt -1)
+  // has beta 0 and reads slot 0, which subtracts an exact zero
+  int slq[kJ];
+#pragma unroll
+  for (int q = 0; q < kJ; ++q) slq[q] = slt[q] >= 0 ? slt[q] : 0;
   double dm = 0.0;  // max |delta| of the cell (int8 plane scaling)
   VgOne vo;
   const int one = vg_one_of(vf, vo);
@@ -1074,7 +1079,7 @@ __global__ __launch_bounds__(256, 4) void k_kgen(
             double delta = k;
 #pragma unroll
             for (int q = 0; q < kJ; ++q)
-              if (slt[q] >= 0) delta -= sG[slt[q]][j] * beta[q];
+              delta -= sG[slq[q]][j] * beta[q];
             dm = fmax(dm, fabs(delta));
             if (delta != 0.0) nzf |= 1u << (j >> 4);
             return delta;
