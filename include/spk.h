@@ -43,6 +43,38 @@ int spk_ctx_sync(spk_ctx *ctx);
 /* Release cached device workspaces. */
 int spk_ctx_trim(spk_ctx *ctx);
 
+/* Captured launch sequences: the batched path runs the launch sequences
+ * of its factorisation (Cholesky panels, condition estimate, dual vectors
+ * and refinement: about 300 launches per batch of 2 000-row systems) as
+ * hipGraphs, captured at their first use on a context and replayed for
+ * every later batch with the same buffers and sizes (on = 1, the
+ * default); 0 launches them one by one.  Same kernels, same results.
+ * Replaces the per-step loop of interp/steps.py:315-401. */
+int spk_ctx_set_graphs(spk_ctx *ctx, int on);
+
+/* ---- tile autotuner --------------------------------------------------
+ * Kernel families with several tile variants that write identical
+ * results: "gemm_plain_f64" (the plain FP64 contraction lambda = K^-1 r
+ * of interp/steps.py:416, k depth 8, 16 or 32 per LDS stage: variants
+ * 0, 1, 2) and "gemm_tri_bf16" (the split-BF16 variance contraction, the
+ * next chunk of delta loaded after (0) or while (1) a chunk is
+ * multiplied).  With SPK_TUNE_AUTO a context times every variant on the
+ * first launch of each size class (the system's rows in classes of 1024)
+ * on the real operands and keeps the fastest; SPK_TUNE_OFF launches the
+ * library's defaults (1 and 0).  The reference's per-step loop
+ * (interp/steps.py:315-401) has no counterpart: its tiles are numpy's. */
+#define SPK_TUNE_OFF   0
+#define SPK_TUNE_AUTO  1   /* default */
+int spk_ctx_set_tuning(spk_ctx *ctx, int mode);
+/* Launch variant v of a family on this context (v < 0: tune or default
+ * again).  SPK_EARG for an unknown family or variant. */
+int spk_ctx_force_variant(spk_ctx *ctx, const char *family, int v);
+/* The picked variants as text, one line per (family, size class):
+ * "family class variant ms_0 ms_1 ...", NUL-terminated within cap bytes.
+ * *len (optional) receives the full length. */
+int spk_ctx_tuning_report(spk_ctx *ctx, char *buf, int64_t cap,
+                          int64_t *len);
+
 /* ---- variogram programs (pyx:86-95, pyx:174-184) -------------------- */
 #define SPK_VG_MAX 8
 enum { SPK_VG_RNG = 0, SPK_VG_NUG, SPK_VG_SPH, SPK_VG_EXP, SPK_VG_LIN,

@@ -24,6 +24,7 @@ VG_MAX = 8
 M_OK, M_SK, M_EDK, M_IDW, M_NNB = range(5)
 METHODS = {'OK': M_OK, 'SK': M_SK, 'EDK': M_EDK, 'IDW': M_IDW, 'NNB': M_NNB}
 P_FP64, P_MIXED, P_FP64B, P_MIXED_F32 = range(4)
+TUNE_OFF, TUNE_AUTO = 0, 1
 PRECISIONS = {'fp64': P_FP64, 'mixed': P_MIXED, 'fp64b': P_FP64B,
               'mixed_f32': P_MIXED_F32}
 
@@ -81,6 +82,10 @@ SIGNATURES = {
     'spk_ctx_set_stream': (c_int, [c_void_p, c_void_p]),
     'spk_ctx_sync': (c_int, [c_void_p]),
     'spk_ctx_trim': (c_int, [c_void_p]),
+    'spk_ctx_set_graphs': (c_int, [c_void_p, c_int]),
+    'spk_ctx_set_tuning': (c_int, [c_void_p, c_int]),
+    'spk_ctx_force_variant': (c_int, [c_void_p, c_char_p, c_int]),
+    'spk_ctx_tuning_report': (c_int, [c_void_p, c_char_p, c_int64, _PI64]),
     'spk_vg_parse': (c_int, [c_char_p, POINTER(SpkVg)]),
     'spk_vg_full_nuggetness': (c_int, [c_char_p, c_double, POINTER(c_int)]),
     'spk_fill_dists_2d_mat': (c_int, [c_void_p, _PD, _PD, c_int64, _PD, _PD,
@@ -228,6 +233,33 @@ class Context:
 
     def trim(self):
         check(lib().spk_ctx_trim(self.h))
+
+    def set_graphs(self, on):
+        """Run the batched factorisation's launch sequences as captured
+        hipGraphs (default) or launch by launch."""
+        check(lib().spk_ctx_set_graphs(self.h, 1 if on else 0))
+
+    def set_tuning(self, auto):
+        """Tile autotuning on (time each kernel variant at the first launch
+        of a size class and keep the fastest) or off (library defaults)."""
+        check(lib().spk_ctx_set_tuning(self.h, TUNE_AUTO if auto else TUNE_OFF))
+
+    def force_variant(self, family, v):
+        """Launch variant v of a kernel family ('gemm_plain_f64' 0-2,
+        'gemm_tri_bf16' 0-1); v = -1 clears."""
+        check(lib().spk_ctx_force_variant(self.h, family.encode(), int(v)))
+
+    def tuning_report(self):
+        """[(family, size class, variant, [ms per variant])] picked so far."""
+        n = c_int64(0)
+        check(lib().spk_ctx_tuning_report(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        check(lib().spk_ctx_tuning_report(self.h, buf, n.value + 1, None))
+        out = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            out.append((f[0], int(f[1]), int(f[2]), [float(x) for x in f[3:]]))
+        return out
 
     def close(self):
         if self.h:

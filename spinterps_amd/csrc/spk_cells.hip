@@ -1714,6 +1714,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     int8_t *L8;  // int8 planes of L^-1 rows (mixed path)
     int *ea;     // their row exponents
     StepSc *sc;
+    // the batch's variogram records, copied so that the captured
+    // factorisation sequences read a buffer whose address is the same
+    // for every batch of the set
+    spk_vg *vg;
     // steps of the batch whose Cholesky failed: solved by the
     // pseudo-inverse fallback (spk_pinv.hip) and the plain contraction
     std::vector<int> pinv;
@@ -1747,6 +1751,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(get("idx", S * N * sizeof(int32_t), (void **)&q.idx));
     SPK_TRY(get("zact", S * ldz * sizeof(double), (void **)&q.zact));
     SPK_TRY(get("sc", S * sizeof(StepSc), (void **)&q.sc));
+    SPK_TRY(get("vgb", S * sizeof(spk_vg), (void **)&q.vg));
     SPK_TRY(get("A", S * strideM * 8, (void **)&q.A));
     SPK_TRY(get("Li", S * strideM * 8, (void **)&q.Li));
     SPK_TRY(get("T0", S * ld * kT0Cols * 8, (void **)&q.T0));
@@ -1919,7 +1924,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   for (int k = 0; k < 4; ++k)
     if (!ctx->ev[k])
       SPK_HIP(hipEventCreateWithFlags(&ctx->ev[k], hipEventDisableTiming));
-  hipStream_t sf = nsets > 1 ? ctx->aux_stream : sm;
+  // the factorisations run on the second stream also for a single batch:
+  // their graphs are captured there, never on the caller's stream (which
+  // may be the null stream)
+  hipStream_t sf = ctx->aux_stream;
   hipEvent_t ev_fact[2] = {ctx->ev[0], ctx->ev[1]};
   hipEvent_t ev_done[2] = {ctx->ev[2], ctx->ev[3]};
   // the factor stream must see the re-centred coordinates
@@ -1930,6 +1938,8 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
 
   auto factorise = [&](int64_t t0, int Sb, Set &q) -> int {
     const spk_vg *gb = vgs + t0;
+    SPK_HIP(hipMemcpyAsync(q.vg, gb, (size_t)Sb * sizeof(spk_vg),
+                           hipMemcpyDeviceToDevice, sf));
     hipLaunchKernelGGL(k_compact, dim3(Sb), dim3(256), 0, sf, vals + t0 * N,
                        N, q.idx, q.zact, ldz, q.sc, gb, method, thr, mvg,
                        (const int32_t *)sorder);
@@ -1938,32 +1948,40 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     {
       ProfScope ps(ctx, "assemble", sf);
       hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles, Sb), dim3(256), 0, sf,
-                         sx, sy, q.idx, N, q.sc, gb,
+                         sx, sy, q.idx, N, q.sc, (const spk_vg *)q.vg,
                          method == SPK_M_SK ? 1 : 0, mvg, q.A, ld, strideM,
                          q.Gm);
     }
     SPK_TRY(check_launch("k_assemble"));
+    // The factorisation's launch sequences (about 190 launches for the
+    // Cholesky panels of 2 000 rows, 100 for the condition estimate) run
+    // as captured graphs, one per sequence and set of buffers.
+    GraphKey gk;
+    gk << q.A << q.Li << q.sc << q.vg << q.T0 << ld << strideM << N << Sb
+       << method;
     {
       ProfScope ps(ctx, "potrf", sf);
-      // the systems have at most N rows: the panels of the largest
-      if (method != SPK_M_SK)
-        hipLaunchKernelGGL(k_chol_shift, dim3(Sb), dim3(256), 0, sf, q.A, ld,
-                           strideM, q.sc, gb);
-      for (int k0 = 0; k0 < N; k0 += 32) {
-        hipLaunchKernelGGL(k_chol_diag, dim3(1, Sb), dim3(256), 0, sf, q.A,
-                           ld, strideM, q.sc, k0);
-        const int64_t below = N - (k0 + 32);
-        if (below <= 0) break;
-        hipLaunchKernelGGL(k_chol_solve,
-                           dim3((unsigned)ceil_div(below, 256), Sb),
-                           dim3(256), 0, sf, q.A, ld, strideM, q.sc, k0);
-        const int64_t nt = ceil_div(below, 64);
-        hipLaunchKernelGGL(k_chol_trail,
-                           dim3((unsigned)(nt * (nt + 1) / 2), Sb), dim3(256),
-                           0, sf, q.A, ld, strideM, q.sc, k0);
-      }
+      SPK_TRY(graph_run(ctx, "potrf", gk.v, sf, [&]() -> int {
+        // the systems have at most N rows: the panels of the largest
+        if (method != SPK_M_SK)
+          hipLaunchKernelGGL(k_chol_shift, dim3(Sb), dim3(256), 0, sf, q.A,
+                             ld, strideM, q.sc, (const spk_vg *)q.vg);
+        for (int k0 = 0; k0 < N; k0 += 32) {
+          hipLaunchKernelGGL(k_chol_diag, dim3(1, Sb), dim3(256), 0, sf, q.A,
+                             ld, strideM, q.sc, k0);
+          const int64_t below = N - (k0 + 32);
+          if (below <= 0) break;
+          hipLaunchKernelGGL(k_chol_solve,
+                             dim3((unsigned)ceil_div(below, 256), Sb),
+                             dim3(256), 0, sf, q.A, ld, strideM, q.sc, k0);
+          const int64_t nt = ceil_div(below, 64);
+          hipLaunchKernelGGL(k_chol_trail,
+                             dim3((unsigned)(nt * (nt + 1) / 2), Sb),
+                             dim3(256), 0, sf, q.A, ld, strideM, q.sc, k0);
+        }
+        return check_launch("k_potrf");
+      }));
     }
-    SPK_TRY(check_launch("k_potrf"));
     // which steps failed the Cholesky (the host waits for this batch's
     // factorisation while the previous batch's cells contract)
     SPK_HIP(hipMemcpyAsync(h_sc, q.sc, (size_t)Sb * sizeof(StepSc),
@@ -1992,65 +2010,76 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         ProfScope ps(ctx, "kappa", sf);
         const unsigned gl = (unsigned)ceil_div(N, 256);
         const unsigned gu = (unsigned)ceil_div(N, 4);
-        for (int which = 0; which < 2; ++which) {
-          const double *Tm = which == 0 ? q.A : q.Li;
-          hipLaunchKernelGGL(k_kap_start, dim3(Sb), dim3(256), 0, sf, q.sc,
-                             q.T0, ld, which);
-          for (int it = 0; it < 12; ++it) {
-            hipLaunchKernelGGL(k_kap_norm, dim3(Sb), dim3(256), 0, sf, q.sc,
-                               q.T0, ld);
-            if (which == 0)
-              hipLaunchKernelGGL(k_kap_umv, dim3(gu, Sb), dim3(256), 0, sf, Tm,
-                                 ld, strideM, q.sc, q.T0, ld, 1);
-            else
-              hipLaunchKernelGGL(k_kap_lmv, dim3(gl, Sb), dim3(256), 0, sf, Tm,
-                                 ld, strideM, q.sc, q.T0, ld, 1);
-            hipLaunchKernelGGL(k_kap_rq, dim3(Sb), dim3(256), 0, sf, q.sc,
-                               q.T0, ld, kap_lam, which);
-            if (which == 0)
-              hipLaunchKernelGGL(k_kap_lmv, dim3(gl, Sb), dim3(256), 0, sf, Tm,
-                                 ld, strideM, q.sc, q.T0, ld, 0);
-            else
-              hipLaunchKernelGGL(k_kap_umv, dim3(gu, Sb), dim3(256), 0, sf, Tm,
-                                 ld, strideM, q.sc, q.T0, ld, 0);
+        GraphKey kk = gk;
+        kk << kap_lam << fp64var_thr << fp64var_thr_gau << int8var_thr;
+        SPK_TRY(graph_run(ctx, "kappa", kk.v, sf, [&]() -> int {
+          for (int which = 0; which < 2; ++which) {
+            const double *Tm = which == 0 ? q.A : q.Li;
+            hipLaunchKernelGGL(k_kap_start, dim3(Sb), dim3(256), 0, sf, q.sc,
+                               q.T0, ld, which);
+            for (int it = 0; it < 12; ++it) {
+              hipLaunchKernelGGL(k_kap_norm, dim3(Sb), dim3(256), 0, sf, q.sc,
+                                 q.T0, ld);
+              if (which == 0)
+                hipLaunchKernelGGL(k_kap_umv, dim3(gu, Sb), dim3(256), 0, sf, Tm,
+                                   ld, strideM, q.sc, q.T0, ld, 1);
+              else
+                hipLaunchKernelGGL(k_kap_lmv, dim3(gl, Sb), dim3(256), 0, sf, Tm,
+                                   ld, strideM, q.sc, q.T0, ld, 1);
+              hipLaunchKernelGGL(k_kap_rq, dim3(Sb), dim3(256), 0, sf, q.sc,
+                                 q.T0, ld, kap_lam, which);
+              if (which == 0)
+                hipLaunchKernelGGL(k_kap_lmv, dim3(gl, Sb), dim3(256), 0, sf, Tm,
+                                   ld, strideM, q.sc, q.T0, ld, 0);
+              else
+                hipLaunchKernelGGL(k_kap_umv, dim3(gu, Sb), dim3(256), 0, sf, Tm,
+                                   ld, strideM, q.sc, q.T0, ld, 0);
+            }
           }
-        }
-        hipLaunchKernelGGL(k_kap_flags, dim3(Sb), dim3(64), 0, sf, q.sc,
-                           (const double *)kap_lam, gb, fp64var_thr,
-                           fp64var_thr_gau, int8var_thr);
+          hipLaunchKernelGGL(k_kap_flags, dim3(Sb), dim3(64), 0, sf, q.sc,
+                             (const double *)kap_lam, (const spk_vg *)q.vg,
+                             fp64var_thr, fp64var_thr_gau, int8var_thr);
+          return check_launch("k_kap");
+        }));
       }
-      SPK_TRY(check_launch("k_kap"));
       {
         ProfScope ps(ctx, "dual", sf);
         const unsigned gl = (unsigned)ceil_div(N, 256);
         const unsigned gu = (unsigned)ceil_div(N, 4);
         const int pq = method == SPK_M_EDK ? 1 + m : 1;
-        hipLaunchKernelGGL(k_dual_rhs, dim3(Sb), dim3(256), 0, sf, q.sc,
-                           q.zact, ldz, q.idx, N, prob->stn_drift, m, method,
-                           q.T0, ld);
-        // U = Li [F z], X = C^-1 [F z]
-        hipLaunchKernelGGL(k_dual_lmv, dim3(gl, Sb), dim3(256), 0, sf, q.Li, ld,
-                           strideM, q.sc, q.T0, ld, 0, 1, pq + 1);
-        hipLaunchKernelGGL(k_dual_umv, dim3(gu, Sb), dim3(256), 0, sf, q.Li, ld,
-                           strideM, q.sc, q.T0, ld, 1, 0, pq + 1);
-        hipLaunchKernelGGL(k_dual_mid, dim3(Sb), dim3(256), 0, sf, q.sc, m,
-                           method, q.T0, ld, q.V, 4 * ld, dual_ds);
-        for (int pass = 0; pass < 2; ++pass) {
-          hipLaunchKernelGGL(k_dual_resid, dim3(gl, Sb), dim3(256), 0, sf,
-                             q.sc, q.zact, ldz, q.idx, N, prob->stn_drift, m,
-                             method, q.T0, ld, q.V, 4 * ld, dual_ds, gb, mvg,
-                             (const double *)sx, (const double *)sy);
-          // Y2 = Li [r1 rw], Z2 = C^-1 [r1 rw]
-          hipLaunchKernelGGL(k_dual_lmv, dim3(gl, Sb), dim3(256), 0, sf, q.Li,
-                             ld, strideM, q.sc, q.T0, ld, 2, 3, 2);
-          hipLaunchKernelGGL(k_dual_umv, dim3(gu, Sb), dim3(256), 0, sf, q.Li,
-                             ld, strideM, q.sc, q.T0, ld, 3, 4, 2);
-          hipLaunchKernelGGL(k_dual_upd, dim3(Sb), dim3(256), 0, sf, q.sc,
-                             q.idx, N, prob->stn_drift, m, method, q.T0, ld,
-                             q.V, 4 * ld, dual_ds);
-        }
-        hipLaunchKernelGGL(k_dual_fin, dim3(Sb), dim3(64), 0, sf, q.sc, m,
-                           method, (const double *)dual_ds);
+        GraphKey dk = gk;
+        dk << q.zact << ldz << q.idx << prob->stn_drift << m << q.V
+           << dual_ds << mvg << sx << sy;
+        SPK_TRY(graph_run(ctx, "dual", dk.v, sf, [&]() -> int {
+          hipLaunchKernelGGL(k_dual_rhs, dim3(Sb), dim3(256), 0, sf, q.sc,
+                             q.zact, ldz, q.idx, N, prob->stn_drift, m, method,
+                             q.T0, ld);
+          // U = Li [F z], X = C^-1 [F z]
+          hipLaunchKernelGGL(k_dual_lmv, dim3(gl, Sb), dim3(256), 0, sf, q.Li, ld,
+                             strideM, q.sc, q.T0, ld, 0, 1, pq + 1);
+          hipLaunchKernelGGL(k_dual_umv, dim3(gu, Sb), dim3(256), 0, sf, q.Li, ld,
+                             strideM, q.sc, q.T0, ld, 1, 0, pq + 1);
+          hipLaunchKernelGGL(k_dual_mid, dim3(Sb), dim3(256), 0, sf, q.sc, m,
+                             method, q.T0, ld, q.V, 4 * ld, dual_ds);
+          for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(k_dual_resid, dim3(gl, Sb), dim3(256), 0, sf,
+                               q.sc, q.zact, ldz, q.idx, N, prob->stn_drift, m,
+                               method, q.T0, ld, q.V, 4 * ld, dual_ds,
+                               (const spk_vg *)q.vg, mvg, (const double *)sx,
+                               (const double *)sy);
+            // Y2 = Li [r1 rw], Z2 = C^-1 [r1 rw]
+            hipLaunchKernelGGL(k_dual_lmv, dim3(gl, Sb), dim3(256), 0, sf, q.Li,
+                               ld, strideM, q.sc, q.T0, ld, 2, 3, 2);
+            hipLaunchKernelGGL(k_dual_umv, dim3(gu, Sb), dim3(256), 0, sf, q.Li,
+                               ld, strideM, q.sc, q.T0, ld, 3, 4, 2);
+            hipLaunchKernelGGL(k_dual_upd, dim3(Sb), dim3(256), 0, sf, q.sc,
+                               q.idx, N, prob->stn_drift, m, method, q.T0, ld,
+                               q.V, 4 * ld, dual_ds);
+          }
+          hipLaunchKernelGGL(k_dual_fin, dim3(Sb), dim3(64), 0, sf, q.sc, m,
+                             method, (const double *)dual_ds);
+          return check_launch("k_dual");
+        }));
       }
       // Steps whose Cholesky succeeded with a condition estimate above
       // kPinvKappa take the pseudo-inverse route too: their solve is
@@ -2142,20 +2171,36 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           SPK_TRY(check_launch("k_rhs_plain"));
           {
             ProfScope ps(ctx, "gemm_plain_f64");
-            hipLaunchKernelGGL((k_gemm<double, true, 16>),
-                               dim3((unsigned)ceil_div(cn, 128), 1), dim3(256),
-                               0, sm, q.M + s * strideM, ld, (int64_t)0, R, ldk,
-                               (int64_t)0, q.sc + s, p, method,
-                               want_var ? 1 : 0, skm, q.zact + s * ldz, ldz,
-                               nnb,
-                               ldk, (const double *)nullptr, (int64_t)0, cn,
-                               eb + s * C + cb,
-                               vbo ? vbo + s * C + cb : nullptr, C, 0,
-                               (double *)nullptr, 0, (const uint8_t *)nullptr,
-                               (int64_t)0, (unsigned long long *)nullptr,
-                               (const int *)nullptr, (const int32_t *)nullptr);
+            // k depth per LDS stage 8, 16 or 32 (the same products in the
+            // same order); a tuning run takes the first 1 024 cell tiles,
+            // whole waves of blocks at one, two or four blocks per CU
+            auto run = [&](int v, bool timing) -> int {
+              const int64_t cr =
+                  timing ? std::min<int64_t>(cn, 1024 * 128) : cn;
+              auto k = v == 0   ? k_gemm<double, true, 8>
+                       : v == 2 ? k_gemm<double, true, 32>
+                                : k_gemm<double, true, 16>;
+              hipLaunchKernelGGL(
+                  k, dim3((unsigned)ceil_div(cr, 128), 1), dim3(256), 0, sm,
+                  q.M + s * strideM, ld, (int64_t)0, R, ldk, (int64_t)0,
+                  q.sc + s, p, method, want_var ? 1 : 0, skm,
+                  q.zact + s * ldz, ldz, nnb, ldk, (const double *)nullptr,
+                  (int64_t)0, cr, eb + s * C + cb,
+                  vbo ? vbo + s * C + cb : nullptr, C, 0, (double *)nullptr,
+                  0, (const uint8_t *)nullptr, (int64_t)0,
+                  (unsigned long long *)nullptr, (const int *)nullptr,
+                  (const int32_t *)nullptr);
+              return check_launch("k_gemm<double,plain>");
+            };
+            // steps the kernel skips (no system) are not timed
+            const int hs = h_sc[s].status;
+            if (hs != 0 && hs != SPK_ST_FAILED)
+              SPK_TRY(run(1, false));
+            else
+              SPK_TRY(tune_launch(ctx, "gemm_plain_f64",
+                                  (int)ceil_div(h_sc[s].n + p, 1024), sm, 3,
+                                  1, run));
           }
-          SPK_TRY(check_launch("k_gemm<double,plain>"));
         }
       }
     } else {
@@ -2232,14 +2277,23 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         const dim3 grid((unsigned)nrb, (unsigned)ceil_div(cn, 128), Sb);
         if (bfop) {
           ProfScope ps(ctx, "gemm_tri_bf16");
-          if (q.lst32.cnt > 0)
+          // delta of the next chunk loaded after (0) or while (1) a chunk
+          // is multiplied: the same products in the same order
+          auto run = [&](int v, bool timing) -> int {
+            unsigned long long *w =
+                prof_of(ctx).on && !timing ? prof_of(ctx).work + 2 : nullptr;
+            auto k = v == 1 ? k_tri_bf<1, kBfA32> : k_tri_bf<0, kBfA32>;
             hipLaunchKernelGGL(
-                (k_tri_bf<0, kBfA32>),
-                dim3((unsigned)ceil_div(q.n32, 64), (unsigned)ceil_div(cn, 128),
+                k, dim3((unsigned)ceil_div(q.n32, 64), (unsigned)ceil_div(cn, 128),
                      (unsigned)q.lst32.cnt),
                 dim3(256), 0, sm, (const uint4 *)q.Lb, ld, (const uint4 *)Kb,
                 ldk, q.sc, q.lst32, cn, qpart, nrb, (const uint8_t *)nzb, nzk,
-                prof_of(ctx).on ? prof_of(ctx).work + 2 : nullptr);
+                w);
+            return check_launch("k_tri_bf");
+          };
+          if (q.lst32.cnt > 0)
+            SPK_TRY(tune_launch(ctx, "gemm_tri_bf16",
+                                (int)ceil_div(q.n32, 1024), sm, 2, 0, run));
         } else if (mixed) {
           ProfScope ps(ctx, "gemm_tri_f32");
           // k chunks of 16: FP32 sums of 32 products, 3 waves per SIMD

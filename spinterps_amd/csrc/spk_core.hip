@@ -151,6 +151,107 @@ int mark_collect(spk_ctx *ctx) {
   return SPK_OK;
 }
 
+// ---- tile autotuner ----------------------------------------------------------
+// The kernel families with tile variants, and how many each has.
+struct TuneFamily {
+  const char *name;
+  int variants;
+};
+static const TuneFamily kTuneFamilies[] = {{"gemm_plain_f64", 3},
+                                           {"gemm_tri_bf16", 2}};
+
+static int tune_family_variants(const char *name) {
+  for (const auto &f : kTuneFamilies)
+    if (!strcmp(f.name, name)) return f.variants;
+  return 0;
+}
+
+int tune_launch(spk_ctx *ctx, const char *kernel, int cls, hipStream_t st,
+                int ncand, int dflt, const std::function<int(int, bool)> &run) {
+  for (const auto &f : ctx->forced)
+    if (f.first == kernel) return run(f.second, false);
+  if (ctx->tune_mode == SPK_TUNE_OFF || ncand < 2) return run(dflt, false);
+  for (const auto &e : ctx->tuned)
+    if (e.kernel == kernel && e.cls == cls) return run(e.choice, false);
+  hipEvent_t ev[3];
+  for (auto &e : ev) SPK_HIP(hipEventCreate(&e));
+  spk_ctx::TuneEntry ent;
+  ent.kernel = kernel;
+  ent.cls = cls;
+  int rc = SPK_OK;
+  float best = INFINITY;
+  for (int v = 0; v < ncand && rc == SPK_OK; ++v) {
+    // the first run warms caches (and code objects), the second is timed
+    rc = hipEventRecord(ev[0], st) == hipSuccess ? SPK_OK : SPK_EHIP;
+    if (rc == SPK_OK) rc = run(v, true);
+    if (rc == SPK_OK && hipEventRecord(ev[1], st) != hipSuccess) rc = SPK_EHIP;
+    if (rc == SPK_OK) rc = run(v, true);
+    if (rc == SPK_OK && hipEventRecord(ev[2], st) != hipSuccess) rc = SPK_EHIP;
+    float ms = 0.f;
+    if (rc == SPK_OK && (hipEventSynchronize(ev[2]) != hipSuccess ||
+                         hipEventElapsedTime(&ms, ev[1], ev[2]) != hipSuccess))
+      rc = SPK_EHIP;
+    if (rc != SPK_OK) break;
+    ent.ms.push_back(ms);
+    if (ms < best) {
+      best = ms;
+      ent.choice = v;
+    }
+  }
+  for (auto &e : ev) hipEventDestroy(e);
+  if (rc == SPK_EHIP)
+    return set_error(SPK_EHIP, "tuning %s: %s", kernel,
+                     hipGetErrorString(hipGetLastError()));
+  SPK_TRY(rc);
+  ctx->tuned.push_back(ent);
+  return run(ent.choice, false);
+}
+
+// ---- captured launch sequences ------------------------------------------
+// At most this many graphs per context; the cache is emptied when full.
+constexpr size_t kMaxGraphs = 32;
+
+int graphs_clear(spk_ctx *ctx) {
+  if (ctx->graphs.empty()) return SPK_OK;
+  // a replay may still run on either stream
+  SPK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
+  for (auto &g : ctx->graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
+  return SPK_OK;
+}
+
+int graph_run(spk_ctx *ctx, const char *tag, const std::vector<uint64_t> &key,
+              hipStream_t st, const std::function<int()> &fn) {
+  // the null stream cannot be captured
+  if (!ctx->graphs_on || st == nullptr) return fn();
+  for (const auto &g : ctx->graphs)
+    if (g.tag == tag && g.key == key) {
+      SPK_HIP(hipGraphLaunch(g.exec, st));
+      return SPK_OK;
+    }
+  if (ctx->graphs.size() >= kMaxGraphs) SPK_TRY(graphs_clear(ctx));
+  SPK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = fn();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != SPK_OK || e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    if (rc != SPK_OK) return rc;
+    return set_error(SPK_EHIP, "capture of %s: %s", tag, hipGetErrorString(e));
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess)
+    return set_error(SPK_EHIP, "instantiating %s: %s", tag,
+                     hipGetErrorString(ei));
+  ctx->graphs.push_back({tag, key, ex});
+  SPK_HIP(hipGraphLaunch(ex, st));
+  return SPK_OK;
+}
+
 // ---- variogram string parsing (pyx:174-184) ------------------------------
 
 static std::string strip(const std::string &s) {
@@ -393,6 +494,8 @@ int spk_ctx_trim(spk_ctx *ctx) {
   SPK_HIP(hipSetDevice(ctx->device));
   SPK_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
+  // the graphs hold the buffers' addresses
+  SPK_TRY(graphs_clear(ctx));
   for (auto &kv : ctx->bufs)
     if (kv.second.ptr) SPK_HIP(hipFree(kv.second.ptr));
   ctx->bufs.clear();
@@ -426,6 +529,62 @@ int spk_ctx_set_stream(spk_ctx *ctx, void *s) {
 int spk_ctx_sync(spk_ctx *ctx) {
   if (!ctx) return set_error(SPK_EARG, "null ctx");
   return sync(ctx);
+}
+
+int spk_ctx_set_graphs(spk_ctx *ctx, int on) {
+  if (!ctx) return set_error(SPK_EARG, "null ctx");
+  if (on != 0 && on != 1) return set_error(SPK_EARG, "graphs %d (0 or 1)", on);
+  ctx->graphs_on = on != 0;
+  return SPK_OK;
+}
+
+int spk_ctx_set_tuning(spk_ctx *ctx, int mode) {
+  if (!ctx) return set_error(SPK_EARG, "null ctx");
+  if (mode != SPK_TUNE_OFF && mode != SPK_TUNE_AUTO)
+    return set_error(SPK_EARG, "tuning mode %d (SPK_TUNE_OFF or _AUTO)", mode);
+  ctx->tune_mode = mode;
+  return SPK_OK;
+}
+
+int spk_ctx_force_variant(spk_ctx *ctx, const char *family, int v) {
+  if (!ctx || !family) return set_error(SPK_EARG, "null argument");
+  const int nv = tune_family_variants(family);
+  if (nv == 0) return set_error(SPK_EARG, "no kernel family '%s'", family);
+  if (v >= nv)
+    return set_error(SPK_EARG, "%s has variants 0..%d, not %d", family,
+                     nv - 1, v);
+  auto &f = ctx->forced;
+  f.erase(std::remove_if(f.begin(), f.end(),
+                         [&](const std::pair<std::string, int> &e) {
+                           return e.first == family;
+                         }),
+          f.end());
+  if (v >= 0) f.emplace_back(family, v);
+  return SPK_OK;
+}
+
+int spk_ctx_tuning_report(spk_ctx *ctx, char *buf, int64_t cap,
+                          int64_t *len) {
+  if (!ctx) return set_error(SPK_EARG, "null ctx");
+  std::string s;
+  char tmp[64];
+  for (const auto &e : ctx->tuned) {
+    s += e.kernel;
+    snprintf(tmp, sizeof tmp, " %d %d", e.cls, e.choice);
+    s += tmp;
+    for (float ms : e.ms) {
+      snprintf(tmp, sizeof tmp, " %.4f", ms);
+      s += tmp;
+    }
+    s += '\n';
+  }
+  if (len) *len = (int64_t)s.size();
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(s.size(), (size_t)cap - 1);
+    memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+  }
+  return SPK_OK;
 }
 
 int spk_vg_parse(const char *vg_str, spk_vg *out) {

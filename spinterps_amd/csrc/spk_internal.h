@@ -7,6 +7,8 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -63,6 +65,27 @@ struct spk_ctx {
   void *blas = nullptr;
   void *h_sc = nullptr;
   size_t h_sc_bytes = 0;
+  // tile autotuner: mode (SPK_TUNE_*), the variant picked per (kernel
+  // family, size class) with the timings it was picked on, and variants
+  // forced by the caller
+  int tune_mode = SPK_TUNE_AUTO;
+  struct TuneEntry {
+    std::string kernel;
+    int cls = 0;
+    int choice = 0;
+    std::vector<float> ms;
+  };
+  std::vector<TuneEntry> tuned;
+  std::vector<std::pair<std::string, int>> forced;
+  // captured launch sequences (hipGraph) of the batched factorisation,
+  // each keyed by every argument and grid of its launches
+  bool graphs_on = true;
+  struct GraphEntry {
+    std::string tag;
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<GraphEntry> graphs;
 };
 
 namespace spk {
@@ -78,6 +101,38 @@ int check_launch(const char *what);
 void mark_reset(spk_ctx *ctx);
 int mark(spk_ctx *ctx, const char *name);
 int mark_collect(spk_ctx *ctx);
+
+// Tile autotuner.  run(v, timing) launches variant v of a kernel family
+// (every variant writes the same results); with timing set it may launch
+// a representative part of the grid.  The variant is the caller's forced
+// one, the one already picked for (kernel, cls), or, the first time the
+// class is seen with tuning on, each variant is launched twice on st
+// between events, the fastest second run is kept, and the picked variant
+// is then launched in full.  With tuning off the variant is dflt.
+int tune_launch(spk_ctx *ctx, const char *kernel, int cls, hipStream_t st,
+                int ncand, int dflt, const std::function<int(int, bool)> &run);
+
+// Launches fn's kernels on st through a hipGraph: captured from fn the
+// first time (tag, key) is seen on the context, replayed afterwards.  fn
+// may only launch kernels on st (no copies, memsets, allocations or host
+// waits), and key must hold every argument and grid size of those
+// launches.  With graphs off (spk_ctx_set_graphs) fn runs directly.
+int graph_run(spk_ctx *ctx, const char *tag, const std::vector<uint64_t> &key,
+              hipStream_t st, const std::function<int()> &fn);
+// Destroys the context's graphs (both streams are synchronised first).
+int graphs_clear(spk_ctx *ctx);
+// Appends values (pointers, integers, doubles) to a graph key.
+struct GraphKey {
+  std::vector<uint64_t> v;
+  template <typename T>
+  GraphKey &operator<<(const T &x) {
+    static_assert(sizeof(T) <= 8, "key fields are at most 8 bytes");
+    uint64_t u = 0;
+    memcpy(&u, &x, sizeof(T));
+    v.push_back(u);
+    return *this;
+  }
+};
 
 // Host-side parse of a variogram string (pyx:174-184).
 int parse_vg(const char *s, spk_vg *out);
