@@ -631,51 +631,6 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
 // kChunk: stations staged per pass of k_kgen's main loop
 constexpr int kJ = 4, kSlots = 16, kChunk = 128;
 
-// One bounded model (spherical, exponential or Gaussian) with or without a
-// nugget, the common form of fitted variograms, so that k_kgen's station
-// loop can be specialised on the model with no component loop inside.  The
-// operations are those of vg_simple_entry for these components (its sum of
-// two terms commutes exactly), and neither contracts a product and a sum
-// into one rounding, so both give the same values.
-struct VgOne {
-  double nug, s, r, c1, c2;
-};
-__device__ inline int vg_one_of(const VgFast &f, VgOne &o) {
-  int model = -1, nugs = 0;
-  o.nug = 0.0;
-  for (int k = 0; k < f.n; ++k) {
-    const int t = f.type[k];
-    if (t == SPK_VG_NUG) {
-      ++nugs;
-      o.nug = f.s[k];
-    } else if (model < 0 &&
-               (t == SPK_VG_SPH || t == SPK_VG_EXP || t == SPK_VG_GAU)) {
-      model = k;
-    } else {
-      return -1;
-    }
-  }
-  if (model < 0 || nugs > 1) return -1;
-  o.s = f.s[model];
-  o.r = f.r[model];
-  o.c1 = f.c1[model];
-  o.c2 = f.c2[model];
-  return f.type[model];
-}
-template <int MT>
-__device__ inline double vg_one(const VgOne &o, double h, double min_vg_val) {
-#pragma clang fp contract(off)
-  double g;
-  if constexpr (MT == SPK_VG_SPH)
-    g = (h >= o.r) ? o.s : o.s * (h * o.c1 - (h * h * h) * o.c2);
-  else if constexpr (MT == SPK_VG_EXP)
-    g = o.s * (1.0 - exp(h * o.c1));
-  else
-    g = o.s * (1.0 - exp((h * h) * o.c1));
-  double acc = o.nug + g;
-  if (acc <= min_vg_val) acc = 0.0;
-  return acc;
-}
 template <bool GEN>
 __device__ inline double kg_vg(const VgFast &f, double h, int covar_flag,
                                double min_vg_val) {

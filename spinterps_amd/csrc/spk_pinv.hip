@@ -158,11 +158,42 @@ __global__ __launch_bounds__(256) void k_pinv_vec(
 // np.isclose(x, 1.0): rtol 1e-5, atol 1e-8 (interp/steps.py:418)
 __device__ inline bool close1(double x) { return fabs(x - 1.0) <= 1e-8 + 1e-5; }
 
+// One staged chunk of stations for k_pinv_cells: the nearest station and
+// the sums r.beta, r.omega.  MT: the step's single bounded model (VgOne),
+// or 0 for the general form; both give vg_fast_entry's values.
+template <int MT>
+__device__ inline void pinv_chunk(const VgFast &vf, const VgOne &vo,
+                                  int cflag, double min_vg_val, double x,
+                                  double y, const double *stx,
+                                  const double *sty, const double *sb,
+                                  const double *sw,
+                                  const int32_t *__restrict__ idx, int base,
+                                  int cnt, double &e, double &a, double &best,
+                                  int &bi) {
+  for (int j = 0; j < cnt; ++j) {
+    const double d = dist2(x, y, stx[j], sty[j]);
+    if (d < best || (d == best && idx[base + j] < idx[bi])) {
+      best = d;
+      bi = base + j;
+    }
+    double g;
+    if constexpr (MT != 0)
+      g = vg_one<MT>(vo, d, min_vg_val);
+    else
+      g = vg_fast_entry(vf, d, cflag, min_vg_val);
+    e += g * sb[j];
+    a += g * sw[j];
+  }
+}
+
 // Per cell of a stripe: the estimate and the sum test from beta / omega.
 // Cells that fail take the nearest active station (ties to the lower
 // station index) and variance 0; OK cells that pass are appended to list
 // (their variance, and their estimate in the reference's arithmetic, come
-// from the plain contraction) when want_list is set.
+// from the plain contraction) when want_list is set.  The right-hand side
+// takes the batched path's variogram values (vg_fast_entry: within a few
+// ulp of fill_vg_var_arr's), with the station loop specialised on a single
+// bounded model.
 __global__ __launch_bounds__(256) void k_pinv_cells(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, const StepSc *__restrict__ sc,
@@ -176,13 +207,20 @@ __global__ __launch_bounds__(256) void k_pinv_cells(
   if (sc->status) return;
   const int n = sc->n;
   __shared__ double stx[256], sty[256], sb[256], sw[256];
-  __shared__ spk_vg vg;
+  __shared__ VgFast vf;
+  __shared__ VgOne vo;
+  __shared__ int mt;
   const int tid = threadIdx.x;
   const int64_t c = blockIdx.x * (int64_t)256 + tid;
   const bool valid = c < Cs;
   const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
-  if (tid == 0) vg = *vgp;
   const int cflag = method == SPK_M_SK ? 1 : 0;
+  if (tid == 0) {
+    vg_fast_init(*vgp, vf);
+    // the single-model form is the semivariogram (OK / EDK)
+    const int t = vg_one_of(vf, vo);
+    mt = (cflag == 0 && t > 0) ? t : 0;
+  }
   double e = 0.0, a = 0.0, best = INFINITY;
   int bi = 0;
   for (int base = 0; base < n; base += 256) {
@@ -195,17 +233,20 @@ __global__ __launch_bounds__(256) void k_pinv_cells(
     }
     __syncthreads();
     const int cnt2 = min(256, n - base);
-    if (valid)
-      for (int j = 0; j < cnt2; ++j) {
-        const double d = dist2(x, y, stx[j], sty[j]);
-        if (d < best || (d == best && idx[base + j] < idx[bi])) {
-          best = d;
-          bi = base + j;
-        }
-        const double g = vg_entry(vg, d, cflag, min_vg_val);
-        e += g * sb[j];
-        a += g * sw[j];
-      }
+    if (valid) {
+      if (mt == SPK_VG_GAU)
+        pinv_chunk<SPK_VG_GAU>(vf, vo, cflag, min_vg_val, x, y, stx, sty, sb,
+                               sw, idx, base, cnt2, e, a, best, bi);
+      else if (mt == SPK_VG_EXP)
+        pinv_chunk<SPK_VG_EXP>(vf, vo, cflag, min_vg_val, x, y, stx, sty, sb,
+                               sw, idx, base, cnt2, e, a, best, bi);
+      else if (mt == SPK_VG_SPH)
+        pinv_chunk<SPK_VG_SPH>(vf, vo, cflag, min_vg_val, x, y, stx, sty, sb,
+                               sw, idx, base, cnt2, e, a, best, bi);
+      else
+        pinv_chunk<0>(vf, vo, cflag, min_vg_val, x, y, stx, sty, sb, sw, idx,
+                      base, cnt2, e, a, best, bi);
+    }
   }
   if (!valid) return;
   if (method != SPK_M_SK) {
