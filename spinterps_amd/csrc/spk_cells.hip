@@ -646,18 +646,110 @@ __global__ void k_vg_kinds(const spk_vg *__restrict__ vgs, int64_t T,
     simple[t] = vg_simple(vgs[t]) ? 1 : 0;
 }
 
+// Squared cell-station distance of the nearest-station search, one
+// rounding per operation (no fused multiply-add), so that every kernel
+// that orders stations by it orders them alike.
+__device__ inline double near_d2(double x, double y, double sx, double sy) {
+#pragma clang fp contract(off)
+  const double dx = x - sx, dy = y - sy;
+  return (dx * dx) + (dy * dy);
+}
+
+// The kNear stations nearest to each cell over all N stations, in
+// ascending (near_d2, rank) order, rank being the station's position in
+// sorder (the order in which k_compact lists a step's active stations), as
+// global station indices list[j * ldl + c] (-1 past N).  Once per call:
+// for a step, the first listed station that is active is the cell's
+// nearest active station with ties to the lower compacted index, unless
+// none of the listed ones is active.  Stations are staged in rank order
+// and inserted with a strict comparison, so equal distances keep rank
+// order.  Grid ceil(C / 256), 256 threads.
+constexpr int kNear = 32;
+__global__ __launch_bounds__(256) void k_near_list(
+    const double *__restrict__ sx, const double *__restrict__ sy,
+    const int32_t *__restrict__ sorder, int64_t N,
+    const double *__restrict__ cx, const double *__restrict__ cy, int64_t C,
+    int32_t *__restrict__ list, int64_t ldl) {
+  __shared__ double stx[256], sty[256];
+  __shared__ int32_t sid[256];
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x * (int64_t)256 + tid;
+  const bool valid = c < C;
+  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+  double d[kNear];
+  int id[kNear];
+#pragma unroll
+  for (int k = 0; k < kNear; ++k) {
+    d[k] = INFINITY;
+    id[k] = -1;
+  }
+  for (int64_t base = 0; base < N; base += 256) {
+    __syncthreads();
+    if (base + tid < N) {
+      const int g = sorder ? sorder[base + tid] : (int)(base + tid);
+      stx[tid] = sx[g];
+      sty[tid] = sy[g];
+      sid[tid] = g;
+    }
+    __syncthreads();
+    const int cnt = (int)min((int64_t)256, N - base);
+    if (valid)
+      for (int j = 0; j < cnt; ++j) {
+        const double d2 = near_d2(x, y, stx[j], sty[j]);
+        if (!(d2 < d[kNear - 1])) continue;
+        const int g = sid[j];
+        // sorted insert: slot k takes slot k - 1's entry when the new one
+        // goes above it, else the new one when it goes above slot k
+#pragma unroll
+        for (int k = kNear - 1; k > 0; --k) {
+          const double dk = d[k], dkm = d[k - 1];
+          const int ik = id[k], ikm = id[k - 1];
+          const bool up = d2 < dkm;
+          d[k] = up ? dkm : (d2 < dk ? d2 : dk);
+          id[k] = up ? ikm : (d2 < dk ? g : ik);
+        }
+        if (d2 < d[0]) {
+          d[0] = d2;
+          id[0] = g;
+        }
+      }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int k = 0; k < kNear; ++k) list[(int64_t)k * ldl + c] = id[k];
+}
+
+// pos[step][g]: the compacted index of station g in the step (idx), -1
+// where it is not active.  Grid (steps), 256 threads.
+__global__ __launch_bounds__(256) void k_stn_pos(
+    const int32_t *__restrict__ idx, int64_t N,
+    const StepSc *__restrict__ sc, int32_t *__restrict__ pos) {
+  const int s = blockIdx.x;
+  int32_t *ps = pos + (int64_t)s * N;
+  const int32_t *my = idx + (int64_t)s * N;
+  for (int64_t g = threadIdx.x; g < N; g += 256) ps[g] = -1;
+  __syncthreads();
+  const int n = sc[s].status == SPK_ST_NOSTNS ? 0 : sc[s].n;
+  for (int q = threadIdx.x; q < n; q += 256) ps[my[q]] = q;
+}
+
 // Nearest active station of every cell of a stripe (ties to the lower
-// index), for the local fit of k_kgen.  With promote set (mixed path) a
-// step is moved to the FP64 contraction (SPK_ST_FP64VAR) when any block of
-// 256 cells has more distinct nearest stations than k_kgen stages
-// (kSlots): cells whose nearest station is not staged get a poorer local
-// fit, a larger residual delta and so a larger FP32 rounding term.  The
-// block partition is the one k_kgen uses.
+// index), for the local fit of k_kgen: the first active station of the
+// cell's list from k_near_list (list, pos of k_stn_pos), or, for the
+// blocks where a cell has no active station among its listed ones, a scan
+// of the step's stations.  With promote set (mixed path) a step is moved
+// to the FP64 contraction (SPK_ST_FP64VAR) when any block of 256 cells has
+// more distinct nearest stations than k_kgen stages (kSlots): cells whose
+// nearest station is not staged get a poorer local fit, a larger residual
+// delta and so a larger FP32 rounding term.  The block partition is the
+// one k_kgen uses.
 __global__ __launch_bounds__(256) void k_nearest(
     const double *__restrict__ sx, const double *__restrict__ sy,
     const int32_t *__restrict__ idx, int64_t N, StepSc *__restrict__ sc,
     const double *__restrict__ cx, const double *__restrict__ cy, int64_t Cs,
-    int32_t *__restrict__ nrst, int64_t ldn, int promote) {
+    int32_t *__restrict__ nrst, int64_t ldn, int promote,
+    const int32_t *__restrict__ list, int64_t ldl,
+    const int32_t *__restrict__ pos) {
   const int sl = blockIdx.y;
   StepSc *stp = sc + sl;
   if (stp->status) return;
@@ -669,29 +761,43 @@ __global__ __launch_bounds__(256) void k_nearest(
   const int tid = threadIdx.x;
   const int64_t c = blockIdx.x * (int64_t)256 + tid;
   const bool valid = c < Cs;
-  const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
   if (tid < kSlots) slot_j[tid] = -1;
   if (tid == 0) ovf = 0;
-  double bd = INFINITY;
   int bj = -1;
-  for (int base = 0; base < n; base += 256) {
-    __syncthreads();
-    if (base + tid < n) {
-      const int g = my[base + tid];
-      stx[tid] = sx[g];
-      sty[tid] = sy[g];
-    }
-    __syncthreads();
-    const int cnt = min(256, n - base);
-    if (valid)
-      for (int j = 0; j < cnt; ++j) {
-        const double dx = x - stx[j], dy = y - sty[j];
-        const double d2 = dx * dx + dy * dy;
-        if (d2 < bd) {
-          bd = d2;
-          bj = base + j;
-        }
+  if (valid && list) {
+    const int32_t *ps = pos + (int64_t)sl * N;
+    for (int k = 0; k < kNear; ++k) {
+      const int g = list[(int64_t)k * ldl + c];
+      if (g < 0) break;
+      const int q = ps[g];
+      if (q >= 0) {
+        bj = q;
+        break;
       }
+    }
+  }
+  if (__syncthreads_or(valid && bj < 0 ? 1 : 0)) {
+    const bool scan = valid && bj < 0;
+    const double x = valid ? cx[c] : 0.0, y = valid ? cy[c] : 0.0;
+    double bd = INFINITY;
+    for (int base = 0; base < n; base += 256) {
+      __syncthreads();
+      if (base + tid < n) {
+        const int g = my[base + tid];
+        stx[tid] = sx[g];
+        sty[tid] = sy[g];
+      }
+      __syncthreads();
+      const int cnt = min(256, n - base);
+      if (scan)
+        for (int j = 0; j < cnt; ++j) {
+          const double d2 = near_d2(x, y, stx[j], sty[j]);
+          if (d2 < bd) {
+            bd = d2;
+            bj = base + j;
+          }
+        }
+    }
   }
   if (valid) nrst[(int64_t)sl * ldn + c] = bj;
   if (!promote) return;
@@ -1673,6 +1779,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     // factorisation sequences read a buffer whose address is the same
     // for every batch of the set
     spk_vg *vg;
+    // compacted index of every station in each step, -1 where inactive
+    // (k_stn_pos; read by k_nearest)
+    int32_t *spos;
     // steps of the batch whose Cholesky failed: solved by the
     // pseudo-inverse fallback (spk_pinv.hip) and the plain contraction
     std::vector<int> pinv;
@@ -1707,6 +1816,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(get("zact", S * ldz * sizeof(double), (void **)&q.zact));
     SPK_TRY(get("sc", S * sizeof(StepSc), (void **)&q.sc));
     SPK_TRY(get("vgb", S * sizeof(spk_vg), (void **)&q.vg));
+    q.spos = nullptr;
+    if (batched && want_var && method == SPK_M_OK)
+      SPK_TRY(get("spos", S * N * sizeof(int32_t), (void **)&q.spos));
     SPK_TRY(get("A", S * strideM * 8, (void **)&q.A));
     SPK_TRY(get("Li", S * strideM * 8, (void **)&q.Li));
     SPK_TRY(get("T0", S * ld * kT0Cols * 8, (void **)&q.T0));
@@ -1872,6 +1984,17 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(ws_get(ctx, "stn_pxy", 2 * N * sizeof(double), (void **)&psx));
     SPK_TRY(cell_order(ctx, sx, sy, N, sorder, psx, psx + N));
   }
+  // every cell's nearest stations over all of them, for k_nearest
+  int32_t *near_list = nullptr;
+  if (nrst) {
+    SPK_TRY(ws_get(ctx, "near_list", (size_t)kNear * C * sizeof(int32_t),
+                   (void **)&near_list));
+    hipLaunchKernelGGL(k_near_list, dim3((unsigned)ceil_div(C, 256)),
+                       dim3(256), 0, sm, sx, sy, (const int32_t *)sorder, N,
+                       (const double *)(perm ? pcx : cx),
+                       (const double *)(perm ? pcy : cy), C, near_list, C);
+    SPK_TRY(check_launch("k_near_list"));
+  }
 
   // second stream + events for the overlap
   if (!ctx->aux_stream)
@@ -1899,6 +2022,11 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                        N, q.idx, q.zact, ldz, q.sc, gb, method, thr, mvg,
                        (const int32_t *)sorder);
     SPK_TRY(check_launch("k_compact"));
+    if (q.spos) {
+      hipLaunchKernelGGL(k_stn_pos, dim3(Sb), dim3(256), 0, sf, q.idx, N,
+                         q.sc, q.spos);
+      SPK_TRY(check_launch("k_stn_pos"));
+    }
     const int tiles = (int)ceil_div(N, 64);
     {
       ProfScope ps(ctx, "assemble", sf);
@@ -2172,7 +2300,10 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
           hipLaunchKernelGGL(k_nearest, dim3((unsigned)ceil_div(cn, 256), Sb),
                              dim3(256), 0, sm, sx, sy, q.idx, N, q.sc,
                              (perm ? pcx : cx) + cb, (perm ? pcy : cy) + cb,
-                             cn, nrst, ldk, mixed ? 1 : 0);
+                             cn, nrst, ldk, mixed ? 1 : 0,
+                             q.spos ? (const int32_t *)near_list + cb
+                                    : (const int32_t *)nullptr,
+                             C, (const int32_t *)q.spos);
           SPK_TRY(check_launch("k_nearest"));
         }
         const int64_t nzt = ceil_div(cn, 128);
