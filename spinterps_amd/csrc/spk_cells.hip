@@ -1999,7 +1999,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // second stream + events for the overlap
   if (!ctx->aux_stream)
     SPK_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 6; ++k)
     if (!ctx->ev[k])
       SPK_HIP(hipEventCreateWithFlags(&ctx->ev[k], hipEventDisableTiming));
   // the factorisations run on the second stream also for a single batch:
@@ -2008,13 +2008,16 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   hipStream_t sf = ctx->aux_stream;
   hipEvent_t ev_fact[2] = {ctx->ev[0], ctx->ev[1]};
   hipEvent_t ev_done[2] = {ctx->ev[2], ctx->ev[3]};
+  // batched path: a batch's factorisation up to the pseudo-inverse
+  // fallback, which only the contraction of the failed steps waits for
+  hipEvent_t ev_pre[2] = {ctx->ev[4], ctx->ev[5]};
   // the factor stream must see the re-centred coordinates
   if (sf != sm) {
     SPK_HIP(hipEventRecord(ev_done[0], sm));
     SPK_HIP(hipStreamWaitEvent(sf, ev_done[0], 0));
   }
 
-  auto factorise = [&](int64_t t0, int Sb, Set &q) -> int {
+  auto factorise = [&](int64_t t0, int Sb, Set &q, hipEvent_t pre) -> int {
     const spk_vg *gb = vgs + t0;
     SPK_HIP(hipMemcpyAsync(q.vg, gb, (size_t)Sb * sizeof(spk_vg),
                            hipMemcpyDeviceToDevice, sf));
@@ -2218,6 +2221,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       }
     }
     SPK_TRY(check_launch("factorise"));
+    if (sf != sm) SPK_HIP(hipEventRecord(pre, sf));
     // pseudo-inverses of the failed steps' bordered systems, written where
     // the plain contraction reads K^-1: the M slab (plain path) or the
     // step's L^-1 slab, which a failed step leaves unused (batched path)
@@ -2230,7 +2234,7 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     return SPK_OK;
   };
 
-  auto contract = [&](int64_t t0, int Sb, Set &q) -> int {
+  auto contract = [&](int64_t t0, int Sb, Set &q, hipEvent_t fact) -> int {
     const spk_vg *gb = vgs + t0;
     bool simple = batched;
     for (int s = 0; batched && s < Sb; ++s) simple = simple && vg_simple_h[t0 + s];
@@ -2410,6 +2414,9 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
         }
         SPK_TRY(check_launch("k_var_reduce"));
       }
+      // the cell kernels above skip the failed steps and started once the
+      // factorisation was done; the rest waits for the pseudo-inverses
+      if (sf != sm) SPK_HIP(hipStreamWaitEvent(sm, fact, 0));
       // steps solved by the pseudo-inverse fallback: estimates and the
       // reference's per-cell sum test from the dual vectors of pinv(K)
       // (O(n) per cell); the plain contraction then gives the OK variance
@@ -2496,18 +2503,25 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // Batch b's contraction is queued before batch b + 1's factorisation:
   // the host reads each batch's Cholesky outcome (for the pseudo-inverse
   // fallback) while the previous batch's cells contract.
-  SPK_TRY(factorise(0, bsize(0), set[0]));
+  // The batched path's cell kernels start after the factorisation proper
+  // (ev_pre); the pseudo-inverses of the failed steps, which those kernels
+  // skip, run beside them on the factor stream until the contraction of
+  // the failed steps waits for them (ev_fact).  The plain path contracts
+  // every step with its K^-1 and waits for all of it.
+  SPK_TRY(factorise(0, bsize(0), set[0], ev_pre[0]));
   if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[0], sf));
   for (int64_t b = 0; b < nb; ++b) {
     const int k = (int)(b & 1);
-    if (sf != sm) SPK_HIP(hipStreamWaitEvent(sm, ev_fact[k], 0));
-    SPK_TRY(contract(b * S, bsize(b), set[nsets > 1 ? k : 0]));
+    if (sf != sm)
+      SPK_HIP(hipStreamWaitEvent(sm, batched ? ev_pre[k] : ev_fact[k], 0));
+    SPK_TRY(contract(b * S, bsize(b), set[nsets > 1 ? k : 0], ev_fact[k]));
     if (sf != sm) SPK_HIP(hipEventRecord(ev_done[k], sm));
     if (b + 1 < nb) {
       const int k2 = k ^ 1;
       // set k2 was last read by the contraction of batch b - 1
       if (b >= 1 && sf != sm) SPK_HIP(hipStreamWaitEvent(sf, ev_done[k2], 0));
-      SPK_TRY(factorise((b + 1) * S, bsize(b + 1), set[nsets > 1 ? k2 : 0]));
+      SPK_TRY(factorise((b + 1) * S, bsize(b + 1), set[nsets > 1 ? k2 : 0],
+                        ev_pre[k2]));
       if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[k2], sf));
     }
   }

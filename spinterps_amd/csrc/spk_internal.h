@@ -48,7 +48,7 @@ struct spk_ctx {
   // second stream (default priority) for factorisations that overlap the
   // cell contractions, and the events that order the two
   hipStream_t aux_stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // named scratch buffers, grown on demand, freed by spk_ctx_trim/destroy
   std::vector<std::pair<std::string, spk::Buffer>> bufs;
   std::vector<hipEvent_t> events;
@@ -106,8 +106,9 @@ int mark_collect(spk_ctx *ctx);
 // (every variant writes the same results); with timing set it may launch
 // a representative part of the grid.  The variant is the caller's forced
 // one, the one already picked for (kernel, cls), or, the first time the
-// class is seen with tuning on, each variant is launched twice on st
-// between events, the fastest second run is kept, and the picked variant
+// class is seen with tuning on, each variant is launched three times on
+// st (the context's other stream drained first), the faster of the last
+// two runs (timed by events) counts, the fastest variant is kept, and it
 // is then launched in full.  With tuning off the variant is dflt.
 int tune_launch(spk_ctx *ctx, const char *kernel, int cls, hipStream_t st,
                 int ncand, int dflt, const std::function<int(int, bool)> &run);
