@@ -175,7 +175,7 @@ def test_ok_small_all_precisions(gpu, precision):
     err = _rel(v, vref, vfloor)
     print(f'{precision}: est max rel {_rel(e, f64["OK"], 1e-12 * scale):.2e} '
           f'var max rel (floor {vfloor:.3g}) {err:.2e}')
-    assert err <= (1e-9 if precision != 'mixed' else 1e-4)
+    assert err <= (1e-9 if precision != 'mixed' else 1e-6)
 
 
 @pytest.mark.parametrize('method', ['SK', 'SKM', 'EDK'])
@@ -231,7 +231,8 @@ def test_special_steps(gpu):
         e, v = est.cpu().numpy(), var.cpu().numpy()
         np.testing.assert_array_equal(info.cpu().numpy()[:4], [1, 8, 2, 2])
         np.testing.assert_allclose(e, f64['OK'], rtol=1e-9, equal_nan=True)
-        np.testing.assert_allclose(v[1:], f64['EST_VARS_OK'][1:], rtol=1e-4,
+        np.testing.assert_allclose(v[1:], f64['EST_VARS_OK'][1:],
+                                   rtol=1e-9 if prec == 'fp64' else 1e-6,
                                    atol=1e-9, equal_nan=True)
         assert np.isnan(e[0]).all() and np.isnan(v[0]).all()
         assert prb == [0]
