@@ -268,6 +268,33 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
                          const double *vals, const spk_vg *vgs, double *est,
                          double *var, int32_t *info, double *kappa);
 
+/* The fields as the reference keeps them, in host memory: interp/steps.py
+ * clips each interpolated field to [min_var_cut, max_var_cut]
+ * (_mod_min_max, interp/steps.py:466-476) and stores it as float32.
+ * est32 [n_steps x C] (required) takes the clipped estimates (has_min /
+ * has_max select the cuts), var32 [n_steps x C] (NULL: none) the OK
+ * variance cast only.  HOST pointers; with page-locked memory
+ * (hipHostMalloc) each batch's fields are copied while later batches
+ * compute. */
+typedef struct spk_f32_out {
+  float *est32;
+  float *var32;
+  int32_t has_min, has_max;
+  double min_cut, max_cut;
+} spk_f32_out;
+
+/* spk_interp_steps_dev that also delivers the float32 host fields of
+ * out32, batch by batch on a third stream (est / var are still written).
+ * Asynchronous like spk_interp_steps_dev: the host fields are complete
+ * when the context's stream has finished the call (spk_ctx_sync).
+ * Replaces the per-step float32 writes of interp/steps.py:466-476 and
+ * 815-831 into interp_flds. */
+int spk_interp_steps_f32_dev(spk_ctx *ctx, const spk_problem *prob,
+                             const spk_opts *opts, int64_t n_steps,
+                             const double *vals, const spk_vg *vgs,
+                             double *est, double *var, int32_t *info,
+                             double *kappa, const spk_f32_out *out32);
+
 /* ---- neighbourhood interpolation ------------------------------------
  * Each cell uses only n_nebs of the step's active stations: its nearest
  * (SPK_NEB_NRST, interp/grps.py:147-166) or n_nebs taken rank by rank over

@@ -55,6 +55,12 @@ class SpkOpts(ctypes.Structure):
                 ('int8var_kappa', c_double)]
 
 
+class SpkF32Out(ctypes.Structure):
+    _fields_ = [('est32', c_void_p), ('var32', c_void_p),
+                ('has_min', c_int32), ('has_max', c_int32),
+                ('min_cut', c_double), ('max_cut', c_double)]
+
+
 class SpkNeb(ctypes.Structure):
     _fields_ = [('sel', c_int32), ('n_nebs', c_int32), ('n_pies', c_int32),
                 ('pad_', c_int32)]
@@ -114,6 +120,11 @@ SIGNATURES = {
                                      POINTER(SpkOpts), c_int64, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p]),
+    'spk_interp_steps_f32_dev': (c_int, [c_void_p, POINTER(SpkProblem),
+                                         POINTER(SpkOpts), c_int64, c_void_p,
+                                         c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p,
+                                         POINTER(SpkF32Out)]),
     'spk_interp_neb_steps_dev': (c_int, [c_void_p, POINTER(SpkProblem),
                                          POINTER(SpkOpts), POINTER(SpkNeb),
                                          c_int64, c_void_p, c_void_p,

@@ -1675,10 +1675,35 @@ int spk_clip_cast_dev(spk_ctx *ctx, const double *in64, float *out32,
   return check_launch("k_clip_cast");
 }
 
-int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
-                         const spk_opts *opts, int64_t T, const double *vals,
-                         const spk_vg *vgs, double *est, double *var,
-                         int32_t *info, double *kappa) {
+// Float32 host fields of rows t0 .. t0 + rows - 1 (est clipped as
+// interp/steps.py:466-476, var cast): cast on stream st into staging
+// (2 x rows x C floats) and copied to the host arrays on st.
+static int f32_out_rows(spk_ctx *ctx, hipStream_t st, const spk_f32_out *o,
+                        const double *est, const double *var, int64_t C,
+                        int64_t t0, int64_t rows, float *stage) {
+  const int64_t n = rows * C;
+  const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, 256), 65536);
+  hipLaunchKernelGGL(k_clip_cast, dim3(g), dim3(256), 0, st, est + t0 * C,
+                     stage, n, o->has_min, o->min_cut, o->has_max,
+                     o->max_cut);
+  if (o->var32 && var)
+    hipLaunchKernelGGL(k_clip_cast, dim3(g), dim3(256), 0, st, var + t0 * C,
+                       stage + n, n, 0, 0.0, 0, 0.0);
+  SPK_TRY(check_launch("k_clip_cast"));
+  SPK_HIP(hipMemcpyAsync(o->est32 + t0 * C, stage, (size_t)n * sizeof(float),
+                         hipMemcpyDeviceToHost, st));
+  if (o->var32 && var)
+    SPK_HIP(hipMemcpyAsync(o->var32 + t0 * C, stage + n,
+                           (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                           st));
+  return SPK_OK;
+}
+
+static int interp_steps_impl(spk_ctx *ctx, const spk_problem *prob,
+                             const spk_opts *opts, int64_t T,
+                             const double *vals, const spk_vg *vgs,
+                             double *est, double *var, int32_t *info,
+                             double *kappa, const spk_f32_out *o32) {
   if (!ctx || !prob || !opts) return set_error(SPK_EARG, "null argument");
   if (T < 0) return set_error(SPK_ESHAPE, "negative step count");
   if (T == 0) return SPK_OK;
@@ -1746,8 +1771,19 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     SPK_TRY(ws_get(ctx, "idx0", S * N * sizeof(int32_t), (void **)&idx));
     SPK_TRY(ws_get(ctx, "zact0", S * ldz * sizeof(double), (void **)&zact));
     SPK_TRY(ws_get(ctx, "sc0", S * sizeof(StepSc), (void **)&sc));
-    return idw_nnb_steps(ctx, opts, T, N, C, vals, sx, sy, cx, cy, idx, zact,
-                         ldz, sc, S, bad, est, info, kappa);
+    SPK_TRY(idw_nnb_steps(ctx, opts, T, N, C, vals, sx, sy, cx, cy, idx,
+                          zact, ldz, sc, S, bad, est, info, kappa));
+    if (!o32) return SPK_OK;
+    // the float32 fields after the whole call, in slices of the staging
+    const int64_t rows =
+        std::max<int64_t>(1, std::min<int64_t>(T, ((int64_t)256 << 20) / C));
+    float *stage;
+    SPK_TRY(ws_get(ctx, "f32_stage", (size_t)2 * rows * C * sizeof(float),
+                   (void **)&stage));
+    for (int64_t t0 = 0; t0 < T; t0 += rows)
+      SPK_TRY(f32_out_rows(ctx, sm, o32, est, nullptr, C, t0,
+                           std::min<int64_t>(rows, T - t0), stage));
+    return SPK_OK;
   }
 
   // ---- kriging --------------------------------------------------------------
@@ -2508,6 +2544,21 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
   // skip, run beside them on the factor stream until the contraction of
   // the failed steps waits for them (ev_fact).  The plain path contracts
   // every step with its K^-1 and waits for all of it.
+  // float32 host fields: batch b's are cast and copied on a third stream
+  // while later batches compute (staging of at most 512 MB, in slices)
+  hipStream_t scp = nullptr;
+  float *stage = nullptr;
+  int64_t srows = 0;
+  if (o32) {
+    if (!ctx->copy_stream)
+      SPK_HIP(hipStreamCreateWithFlags(&ctx->copy_stream,
+                                       hipStreamNonBlocking));
+    scp = ctx->copy_stream;
+    srows = std::max<int64_t>(
+        1, std::min<int64_t>(S, ((int64_t)256 << 20) / C));
+    SPK_TRY(ws_get(ctx, "f32_stage", (size_t)2 * srows * C * sizeof(float),
+                   (void **)&stage));
+  }
   SPK_TRY(factorise(0, bsize(0), set[0], ev_pre[0]));
   if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[0], sf));
   for (int64_t b = 0; b < nb; ++b) {
@@ -2515,7 +2566,14 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
     if (sf != sm)
       SPK_HIP(hipStreamWaitEvent(sm, batched ? ev_pre[k] : ev_fact[k], 0));
     SPK_TRY(contract(b * S, bsize(b), set[nsets > 1 ? k : 0], ev_fact[k]));
-    if (sf != sm) SPK_HIP(hipEventRecord(ev_done[k], sm));
+    SPK_HIP(hipEventRecord(ev_done[k], sm));
+    if (o32) {
+      SPK_HIP(hipStreamWaitEvent(scp, ev_done[k], 0));
+      for (int64_t r0 = 0; r0 < bsize(b); r0 += srows)
+        SPK_TRY(f32_out_rows(ctx, scp, o32, est, want_var ? var : nullptr, C,
+                             b * S + r0,
+                             std::min<int64_t>(srows, bsize(b) - r0), stage));
+    }
     if (b + 1 < nb) {
       const int k2 = k ^ 1;
       // set k2 was last read by the contraction of batch b - 1
@@ -2525,12 +2583,35 @@ int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
       if (sf != sm) SPK_HIP(hipEventRecord(ev_fact[k2], sf));
     }
   }
-  // nothing may remain queued on the factor stream after return
+  // nothing may remain queued on the factor or copy stream after return
   if (sf != sm) {
     SPK_HIP(hipEventRecord(ev_fact[0], sf));
     SPK_HIP(hipStreamWaitEvent(sm, ev_fact[0], 0));
   }
+  if (o32) {
+    SPK_HIP(hipEventRecord(ev_done[0], scp));
+    SPK_HIP(hipStreamWaitEvent(sm, ev_done[0], 0));
+  }
   return SPK_OK;
+}
+
+int spk_interp_steps_dev(spk_ctx *ctx, const spk_problem *prob,
+                         const spk_opts *opts, int64_t T, const double *vals,
+                         const spk_vg *vgs, double *est, double *var,
+                         int32_t *info, double *kappa) {
+  return interp_steps_impl(ctx, prob, opts, T, vals, vgs, est, var, info,
+                           kappa, nullptr);
+}
+
+int spk_interp_steps_f32_dev(spk_ctx *ctx, const spk_problem *prob,
+                             const spk_opts *opts, int64_t T,
+                             const double *vals, const spk_vg *vgs,
+                             double *est, double *var, int32_t *info,
+                             double *kappa, const spk_f32_out *out32) {
+  if (!out32 || !out32->est32)
+    return set_error(SPK_EARG, "spk_interp_steps_f32_dev needs est32");
+  return interp_steps_impl(ctx, prob, opts, T, vals, vgs, est, var, info,
+                           kappa, out32);
 }
 
 // spk_interp_steps_dev for IDW / nearest neighbour with no allocation:

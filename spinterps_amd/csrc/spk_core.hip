@@ -80,6 +80,7 @@ int ws_get(spk_ctx *ctx, const char *name, size_t bytes, void **out) {
       // The streams may still use the old buffer.
       SPK_HIP(hipStreamSynchronize(ctx->stream));
       if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
+      if (ctx->copy_stream) SPK_HIP(hipStreamSynchronize(ctx->copy_stream));
       SPK_HIP(hipFree(kv.second.ptr));
       kv.second.ptr = nullptr;
       kv.second.bytes = 0;
@@ -502,6 +503,7 @@ int spk_ctx_trim(spk_ctx *ctx) {
   SPK_HIP(hipSetDevice(ctx->device));
   SPK_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->aux_stream) SPK_HIP(hipStreamSynchronize(ctx->aux_stream));
+  if (ctx->copy_stream) SPK_HIP(hipStreamSynchronize(ctx->copy_stream));
   // the graphs hold the buffers' addresses
   SPK_TRY(graphs_clear(ctx));
   for (auto &kv : ctx->bufs)
@@ -519,6 +521,7 @@ int spk_ctx_destroy(spk_ctx *ctx) {
   for (auto ev : ctx->ev)
     if (ev) hipEventDestroy(ev);
   if (ctx->aux_stream) hipStreamDestroy(ctx->aux_stream);
+  if (ctx->copy_stream) hipStreamDestroy(ctx->copy_stream);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return rc;

@@ -48,6 +48,9 @@ struct spk_ctx {
   // second stream (default priority) for factorisations that overlap the
   // cell contractions, and the events that order the two
   hipStream_t aux_stream = nullptr;
+  // third stream (default priority): float32 host copies of finished
+  // batches (spk_interp_steps_f32_dev)
+  hipStream_t copy_stream = nullptr;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // named scratch buffers, grown on demand, freed by spk_ctx_trim/destroy
   std::vector<std::pair<std::string, spk::Buffer>> bufs;

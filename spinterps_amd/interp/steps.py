@@ -15,8 +15,8 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .._lib import (METHODS, NEB_SEL, PRECISIONS, SpkNeb, SpkOpts, SpkProblem,
-                    check, context, lib, vg_array)
+from .._lib import (METHODS, NEB_SEL, PRECISIONS, SpkF32Out, SpkNeb, SpkOpts,
+                    SpkProblem, check, context, lib, vg_array)
 
 
 def _dev(device):
@@ -116,7 +116,8 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
                  precision='fp64', want_var=True, idw_exp=2.0,
                  min_vg_val=0.0, min_var_thr=-np.inf, est=None, var=None,
                  vg_dev=None, kappa=None, fp64var_kappa=0.0,
-                 int8var_kappa=0.0):
+                 int8var_kappa=0.0, out32=None, min_var_cut=None,
+                 max_var_cut=None):
     """Interpolate T steps.  vals [T x N] (NaN = missing station).
 
     method: 'OK', 'SK' (the reference's SK: lambda^T z, nearest station
@@ -140,6 +141,11 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     1e3 for steps with a Gaussian component).  ``int8var_kappa``: up to
     this condition estimate such steps use exact int8 plane products
     (flag 64), above it FP64 (0 = no limit, negative = never).
+    ``out32``: optional (est32, var32) float32 CPU tensors [T x C] (var32
+    may be None; page-locked for overlap) that receive the fields as the
+    reference keeps them (estimates clipped to [min_var_cut,
+    max_var_cut], interp/steps.py:466-476), copied batch by batch while
+    later batches compute; complete once the device is synchronised.
     """
     dev = prob.device
     v = _t64(vals, dev)
@@ -181,11 +187,26 @@ def interp_steps(prob: DeviceProblem, vals, vgs=None, method='OK',
     opts.sk_mean = 1 if sk_mean else 0
     ctx = _ctx_for(dev)
     pst = prob.c_struct()
-    check(lib().spk_interp_steps_dev(
-        ctx.h, ctypes.byref(pst), ctypes.byref(opts), T, v.data_ptr(),
-        vg_dev.data_ptr() if krig else None, est.data_ptr(),
-        var.data_ptr() if wv else None, info.data_ptr(),
-        kappa.data_ptr() if kappa is not None else None))
+    args = (ctx.h, ctypes.byref(pst), ctypes.byref(opts), T, v.data_ptr(),
+            vg_dev.data_ptr() if krig else None, est.data_ptr(),
+            var.data_ptr() if wv else None, info.data_ptr(),
+            kappa.data_ptr() if kappa is not None else None)
+    if out32 is None:
+        check(lib().spk_interp_steps_dev(*args))
+    else:
+        e32, v32 = out32
+        for t in (e32, v32):
+            if t is not None:
+                assert (t.device.type == 'cpu' and t.dtype == torch.float32
+                        and t.is_contiguous() and tuple(t.shape) == (T, C))
+        o = SpkF32Out()
+        o.est32 = e32.data_ptr()
+        o.var32 = v32.data_ptr() if (v32 is not None and wv) else None
+        o.has_min = 0 if min_var_cut is None else 1
+        o.min_cut = 0.0 if min_var_cut is None else float(min_var_cut)
+        o.has_max = 0 if max_var_cut is None else 1
+        o.max_cut = 0.0 if max_var_cut is None else float(max_var_cut)
+        check(lib().spk_interp_steps_f32_dev(*args, ctypes.byref(o)))
     return est, (var if wv else None), info
 
 
