@@ -482,17 +482,25 @@ __global__ __launch_bounds__(256) void k_var_reduce(
 // writes the partial norm of its 64 rows to qpart[(step * nrb + rb) * ldq]
 // with rb = 2 * (row tile) + (wave row), so the 64-row partials of the
 // generic kernel and this one have the same layout.
+// LOOP: grid (row tiles, cell tiles, 1), each block taking the tile of
+// every one of the nsteps steps that the selection keeps in turn, for
+// launches where few steps or none are selected (the mixed path's
+// promoted steps): a block per (tile, step) that only exits still costs
+// its dispatch.
+template <bool LOOP>
 __global__ __launch_bounds__(256, 2) void k_tri64(
     const double *__restrict__ A, int64_t lda, int64_t strideA,
     const double *__restrict__ B, int64_t ldb, int64_t strideB,
     const StepSc *__restrict__ sc, int64_t Cs, double *__restrict__ qpart,
-    int64_t ldq, int nrb, int sel) {
+    int64_t ldq, int nrb, int sel, int nsteps) {
   constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
   const int G = gridDim.x * gridDim.y;
   int lin = blockIdx.x + blockIdx.y * gridDim.x;
   if ((G & 7) == 0) lin = (lin & 7) * (G >> 3) + (lin >> 3);
   const int rti = lin % gridDim.x, cti = lin / gridDim.x;
-  const int sl = blockIdx.z;
+  __shared__ __align__(16) double As[2][BK][BM + PAD];
+  __shared__ __align__(16) double Bs[2][BK][BN + PAD];
+  auto tile = [&](const int sl) {
   const StepSc st = sc[sl];
   if (st.status) return;
   if (sel == 1 && (st.flags & (SPK_ST_FP64VAR | SPK_ST_INT8VAR))) return;
@@ -503,10 +511,8 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
   const int64_t c0 = cti * (int64_t)BN;
   if (c0 >= Cs) return;
   const int kend = min(n, r0 + BM);
-  A += (int64_t)sl * strideA;
-  B += (int64_t)sl * strideB;
-  __shared__ __align__(16) double As[2][BK][BM + PAD];
-  __shared__ __align__(16) double Bs[2][BK][BN + PAD];
+  const double *Ap = A + (int64_t)sl * strideA;
+  const double *Bp = B + (int64_t)sl * strideB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
@@ -523,7 +529,7 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
       // n, so a pair starting below n lies inside the column)
       if (gk < kend && r0 + r < n) {
         double2 v = *reinterpret_cast<const double2 *>(
-            A + (r0 + r) + (int64_t)gk * lda);
+            Ap + (r0 + r) + (int64_t)gk * lda);
         if (r0 + r + 1 >= n) v.y = 0.0;
         ra[q] = v;
       } else {
@@ -532,7 +538,7 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
       const int cc = (e & 63) * 2, kb = e >> 6;  // B: row kb, cell pair cc
       const int gkb = k0 + kb;
       rb[q] = gkb < kend ? *reinterpret_cast<const double2 *>(
-                               B + (int64_t)gkb * ldb + c0 + cc)
+                               Bp + (int64_t)gkb * ldb + c0 + cc)
                          : double2{0.0, 0.0};
     }
   };
@@ -600,7 +606,25 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
     const int64_t col = c0 + wc * 64 + tj * 16 + li;
     if (col < Cs) qpart[((int64_t)sl * nrb + prow) * ldq + col] = part[tj];
   }
+  };
+  if (!LOOP) {
+    tile(blockIdx.z);
+    return;
+  }
+  for (int sl = 0; sl < nsteps; ++sl) {
+    // the previous step's tile is done with LDS
+    __syncthreads();
+    tile(sl);
+  }
 }
+template __global__ void k_tri64<false>(const double *, int64_t, int64_t,
+                                        const double *, int64_t, int64_t,
+                                        const StepSc *, int64_t, double *,
+                                        int64_t, int, int, int);
+template __global__ void k_tri64<true>(const double *, int64_t, int64_t,
+                                       const double *, int64_t, int64_t,
+                                       const StepSc *, int64_t, double *,
+                                       int64_t, int, int, int);
 
 // ---- batched path: fused distance -> covariance -------------------------------
 // For each active station i and cell c: k_ic = c0 - G(d_ic) (OK/EDK) or the
@@ -2434,11 +2458,14 @@ static int interp_steps_impl(spk_ctx *ctx, const spk_problem *prob,
         }
         {
           ProfScope ps(ctx, "gemm_tri_f64");
+          // the mixed path's promoted steps are few (and known only on
+          // the device): one block per tile walks the steps
           const dim3 g64((unsigned)ceil_div(N, 128),
-                         (unsigned)ceil_div(cn, 128), Sb);
-          hipLaunchKernelGGL(k_tri64, g64, dim3(256), 0, sm, q.Li, ld, strideM,
+                         (unsigned)ceil_div(cn, 128), mixed ? 1 : Sb);
+          hipLaunchKernelGGL(mixed ? k_tri64<true> : k_tri64<false>, g64,
+                             dim3(256), 0, sm, q.Li, ld, strideM,
                              (const double *)K64, ldk, ld * ldk, q.sc, cn,
-                             qpart, ldk, nrb, mixed ? 2 : 0);
+                             qpart, ldk, nrb, mixed ? 2 : 0, Sb);
         }
         SPK_TRY(check_launch("k_gemm<tri>"));
         {
