@@ -611,10 +611,30 @@ __global__ __launch_bounds__(256, 2) void k_tri64(
     tile(blockIdx.z);
     return;
   }
-  for (int sl = 0; sl < nsteps; ++sl) {
+  // the selected steps (at most kMaxStepsPerBatch), read at once by the
+  // first wave; the block then takes their tiles in step order
+  __shared__ int s_sel[kMaxStepsPerBatch];
+  __shared__ int s_cnt;
+  if (threadIdx.x < 64) {
+    const int t = threadIdx.x;
+    bool keep = false;
+    if (t < nsteps && t < kMaxStepsPerBatch) {
+      const StepSc *p = sc + t;
+      const int fl = p->flags;
+      keep = p->status == 0 && rti * 128 < p->n &&
+             !(sel == 1 && (fl & (SPK_ST_FP64VAR | SPK_ST_INT8VAR))) &&
+             !(sel == 2 && !(fl & SPK_ST_FP64VAR));
+    }
+    const unsigned long long m = __ballot(keep);
+    if (keep) s_sel[__popcll(m & ((1ull << t) - 1ull))] = t;
+    if (t == 0) s_cnt = __popcll(m);
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
+  for (int i = 0; i < cnt; ++i) {
     // the previous step's tile is done with LDS
-    __syncthreads();
-    tile(sl);
+    if (i > 0) __syncthreads();
+    tile(s_sel[i]);
   }
 }
 template __global__ void k_tri64<false>(const double *, int64_t, int64_t,
