@@ -41,8 +41,8 @@ sys.path.insert(0, str(ROOT))
 # (stripes of 10 880 cells).
 TRI64_TRAFFIC = 2 * 4537366 * 1024 + 22939 * 1024
 TRI32_TRAFFIC = 2 * 5403411 * 1024 + 202497 * 1024
-TRI8_TRAFFIC = 2 * 14086536 * 1024 + 171559 * 1024
-TRIBF_TRAFFIC = 2 * 1639751 * 1024 + 116188 * 1024
+TRI8_TRAFFIC = 2 * 14085710 * 1024 + 171559 * 1024
+TRIBF_TRAFFIC = 2 * 1603109 * 1024 + 115980 * 1024
 
 
 def log(*a):
