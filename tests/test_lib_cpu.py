@@ -61,3 +61,13 @@ def test_vg_array_layout():
     assert arr['n'].tolist() == [1, 2, 1]
     assert arr['sill'][1, :2].tolist() == [3.0, 1.0]
     assert arr['type'][1, :2].tolist() == [3, 1]
+
+
+def test_f32_out_struct_layout():
+    """SpkF32Out mirrors include/spk.h's spk_f32_out: two host pointers,
+    two int32 flags, two doubles (40 bytes, the doubles 8-aligned)."""
+    S = _lib.SpkF32Out
+    assert ctypes.sizeof(S) == 40
+    assert S.est32.offset == 0 and S.var32.offset == 8
+    assert S.has_min.offset == 16 and S.has_max.offset == 20
+    assert S.min_cut.offset == 24 and S.max_cut.offset == 32
